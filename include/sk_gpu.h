@@ -1,0 +1,274 @@
+/*
+ * sk_gpu.h - C ABI of the MI355X restarted-PDHG LP engine (library: libsk_gpu.so).
+ *
+ * Conventions follow the reference's C API (include/sankhya/sankhya.h:16-28): plain C types,
+ * opaque caller-owned handles freed with their _free or _destroy call, int status codes, a thread-local
+ * last-error string, and no exception or C++ type crosses the boundary. Handles are not
+ * internally synchronised; distinct handles may be used from distinct threads.
+ *
+ * Status codes are numbered as sankhya::gpu::GpuStatus (include/sankhya/gpu_runtime.hpp:19-30)
+ * with one addition, SK_GPU_INVALID_ARGUMENT. Solve statuses are numbered as
+ * sankhya::SolveStatus (include/sankhya/model.hpp:62-80).
+ */
+#ifndef SK_GPU_H
+#define SK_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes (gpu_runtime.hpp:19-30) ---------------------------------------------- */
+#define SK_GPU_SUCCESS 0
+#define SK_GPU_UNAVAILABLE 1        /* kCudaUnavailable: the HIP runtime cannot be used */
+#define SK_GPU_NO_DEVICE 2          /* kNoCompatibleDevice */
+#define SK_GPU_INIT_FAILED 3        /* kInitFailed */
+#define SK_GPU_OUT_OF_MEMORY 4      /* kOutOfMemory */
+#define SK_GPU_LAUNCH_FAILED 5      /* kKernelLaunchFailed */
+#define SK_GPU_COPY_FAILED 6        /* kCopyFailed */
+#define SK_GPU_SYNC_FAILED 7        /* kSyncFailed */
+#define SK_GPU_INVALID_STATE 8      /* kInvalidDeviceState */
+#define SK_GPU_NON_FINITE 9         /* kNonFiniteOutput */
+#define SK_GPU_INVALID_ARGUMENT 10  /* malformed model or argument; no reference counterpart */
+
+/* ---- solve statuses (model.hpp:62-80) ---------------------------------------------------- */
+#define SK_SOLVE_NOT_SOLVED 0
+#define SK_SOLVE_OPTIMAL 1
+#define SK_SOLVE_FEASIBLE 5
+#define SK_SOLVE_ITERATION_LIMIT 6
+#define SK_SOLVE_TIME_LIMIT 7
+#define SK_SOLVE_NUMERICAL_ERROR 9
+#define SK_SOLVE_MODEL_ERROR 10
+
+/* ---- runtime (replaces include/sankhya/gpu_runtime.hpp:32-105) --------------------------- */
+
+typedef struct {
+  int32_t device_id;
+  char name[256];
+  uint64_t total_memory_bytes;
+  int32_t gcn_arch_major; /* replaces compute_capability_major */
+  int32_t gcn_arch_minor; /* replaces compute_capability_minor */
+  int32_t multiprocessor_count;
+  int32_t warp_size; /* 64 on CDNA */
+  int32_t max_threads_per_block;
+  char gcn_arch_name[64];
+} sk_gpu_device_info;
+
+/* gpu_runtime.hpp:32-56 to_string(GpuStatus) */
+const char* sk_gpu_status_string(int32_t status);
+/* gpu_runtime.hpp:70-71 is_cuda_available: 1 when a HIP device can be used */
+int32_t sk_gpu_is_available(void);
+/* gpu_runtime.hpp:73-74 get_gpu_device_count */
+int32_t sk_gpu_device_count(void);
+/* gpu_runtime.hpp:76-77 get_device_info */
+int32_t sk_gpu_get_device_info(int32_t device_id, sk_gpu_device_info* out_info);
+/* gpu_runtime.hpp:79-80 set_device */
+int32_t sk_gpu_set_device(int32_t device_id);
+/* gpu_runtime.hpp:82-83 synchronize_device */
+int32_t sk_gpu_synchronize(void);
+/* gpu_runtime.hpp:85-86 get_last_error_string (thread-local; valid until the next call) */
+const char* sk_gpu_last_error(void);
+/* gpu_runtime.hpp:90-105 allocate/free/copy/memset */
+int32_t sk_gpu_malloc(void** dev_ptr, size_t bytes);
+int32_t sk_gpu_free(void* dev_ptr);
+int32_t sk_gpu_copy_h2d(void* dev_dst, const void* host_src, size_t bytes);
+int32_t sk_gpu_copy_d2h(void* host_dst, const void* dev_src, size_t bytes);
+int32_t sk_gpu_memset(void* dev_ptr, int32_t value, size_t bytes);
+
+/* ---- device operators (replace include/sankhya/gpu_ops.hpp:50-62; launch_gpu_*) ----------
+ * Device pointers, FP64, asynchronous on the library's stream unless stated; call
+ * sk_gpu_synchronize before reading results. */
+
+/* gpu_ops.hpp:52-53 launch_gpu_axpy: y += alpha x */
+int32_t sk_gpu_axpy(int32_t n, double alpha, const double* d_x, double* d_y);
+/* gpu_ops.hpp:55-57 launch_gpu_dot: synchronous, deterministic two-stage FP64 reduction */
+int32_t sk_gpu_dot(int32_t n, const double* d_x, const double* d_y, double* host_result);
+/* gpu_ops.hpp:59-62 launch_gpu_pdhg_primal_step:
+ *   x_next = proj_[lo,hi](x - tau (cost + at_y)), extrapolated = 2 x_next - x */
+int32_t sk_gpu_pdhg_primal_step(int32_t n, double tau, const double* d_x, const double* d_cost,
+                                const double* d_at_y, const double* d_col_lower,
+                                const double* d_col_upper, double* d_x_next,
+                                double* d_extrapolated);
+/* gpu_ops.hpp:64-67 launch_gpu_spmv_csc on a CSC matrix.
+ *   transpose = 1: y = alpha A^T x, one wavefront per column (deterministic).
+ *   transpose = 0: y = alpha A x by column scatter with FP64 atomics, as the reference's
+ *   k_spmv_csc_scatter (cuda_kernels.cu:96-112); summation order is not deterministic. The
+ *   engine never uses this form: it keeps a CSR of A instead (see sk_gpu_spmv_csr). */
+int32_t sk_gpu_spmv_csc(int32_t num_rows, int32_t num_cols, const int32_t* d_col_starts,
+                        const int32_t* d_row_indices, const double* d_values, const double* d_x,
+                        double* d_y, double alpha, int32_t transpose);
+/* Deterministic y = alpha A x on a CSR matrix, one wavefront per row. New (the reference has
+ * no CSR operator); equals sk_gpu_spmv_csc(transpose=1) applied to the CSC of A^T. */
+int32_t sk_gpu_spmv_csr(int32_t num_rows, int32_t num_cols, const int32_t* d_row_starts,
+                        const int32_t* d_col_indices, const double* d_values, const double* d_x,
+                        double* d_y, double alpha);
+
+/* gpu_ops.hpp:73-78, 85-87 NumericalComparisonResult / compare_vectors (host function):
+ * fails on any non-finite entry, or where |a-b| > tol and |a-b| / max(1,|a|,|b|) > tol. */
+typedef struct {
+  int32_t passed;
+  int32_t contains_nan_or_inf;
+  double max_abs_diff;
+  double max_rel_diff;
+} sk_gpu_comparison;
+int32_t sk_gpu_compare_vectors(int32_t n, const double* test_vec, const double* ref_vec,
+                               double tol, sk_gpu_comparison* out);
+
+/* ---- LP models ---------------------------------------------------------------------------
+ * An LP in the reference Model layout (model.hpp:131-225): CSC matrix (rows ascending inside
+ * each column, no duplicates), cost, column bounds, two-sided row bounds; a bound is absent
+ * when infinite or |v| >= 1e30 (types.hpp:32-39). sense: +1 minimise, -1 maximise. */
+typedef struct sk_lp sk_lp;
+
+int32_t sk_lp_create(int32_t num_rows, int32_t num_cols, int32_t sense, double objective_offset,
+                     const int32_t* col_starts, const int32_t* row_indices,
+                     const double* values, const double* col_cost, const double* col_lower,
+                     const double* col_upper, const double* row_lower, const double* row_upper,
+                     sk_lp** out);
+/* Seeded generator. config 1..5 are the five workloads: 1 = 500x1000 blending LP (seed =
+ * instance), 2 = 200k x 500k power-law LP with 2.5M nnz (seed = instance), 3 = member
+ * `instance` of the 256-LP batch, 4 = serving request `instance`, 5 = 2M x 5M power-law LP
+ * with 25M nnz (seed = instance). */
+int32_t sk_lp_generate(int32_t config, uint64_t instance, sk_lp** out);
+
+typedef struct {
+  uint64_t seed;
+  int32_t num_rows;
+  int32_t num_cols;
+  int32_t num_equality_rows;
+  int32_t degree_kind; /* 0 fixed degree_lo, 1 uniform [lo, hi], 2 power law on [lo, hi] */
+  int32_t degree_lo;
+  int32_t degree_hi;
+  double power_exponent;
+  int64_t target_nnz; /* power law: trim/pad to exactly this many nonzeros (0: no change) */
+  int32_t value_kind; /* 0 |a| uniform on [lo, hi], 1 log-uniform */
+  double value_lo;
+  double value_hi;
+  double negative_probability;
+  double upper_lo, upper_hi;
+  double slack_lo, slack_hi;
+  double cost_lo, cost_hi;
+  int32_t fix_empty_rows;
+} sk_gen_params;
+int32_t sk_gen_preset(int32_t config, uint64_t instance, sk_gen_params* out);
+int32_t sk_lp_generate_params(const sk_gen_params* params, sk_lp** out);
+
+int32_t sk_lp_dims(const sk_lp* lp, int32_t* num_rows, int32_t* num_cols, int64_t* nnz);
+/* Copies the model out; every pointer may be NULL to skip that array. */
+int32_t sk_lp_export(const sk_lp* lp, int32_t* sense, double* objective_offset,
+                     int32_t* col_starts, int32_t* row_indices, double* values,
+                     double* col_cost, double* col_lower, double* col_upper, double* row_lower,
+                     double* row_upper);
+void sk_lp_free(sk_lp* lp);
+
+/* ---- PDHG options (the options solve_pdhg reads: pdhg.cpp:304-315) ------------------------ */
+typedef struct {
+  double tolerance;          /* pdhg_tolerance, default 1e-4 (tolerances.hpp:159) */
+  int64_t iteration_limit;   /* iteration_limit, -1 = 1,000,000 (pdhg.cpp:312-313) */
+  double time_limit;         /* time_limit in seconds, default DBL_MAX */
+  int32_t restart;           /* pdhg_restart, default 1 */
+  int32_t stop_at_request;   /* pdhg_stop_at_request, default 0 */
+  int32_t absolute_primal;   /* pdhg_absolute_primal, default 1: the stop test also needs
+                                absolute primal <= 1e-7 (pdhg.cpp:119-122); 0 stops on the
+                                relative KKT error alone */
+  int64_t random_seed;       /* random_seed, default 0 */
+  int32_t precision;         /* pdhg_precision: 0 "fp32" (FP32 iterates, FP64 evaluation,
+                                default), 1 "fp64" */
+  int32_t steps_per_block;   /* PDHG steps per captured graph, default 40 */
+  int32_t use_graph;         /* 1: hipGraph-captured step blocks (default) */
+  int32_t spmv_stage;        /* SpMV schedule: -1 autotune (default), 0 direct, 1 LDS-staged */
+  int32_t tile_nnz;          /* row-block capacity in nonzeros, 0 = automatic */
+  int32_t compute_quality;   /* 1: fill the recompute_quality measures (default) */
+} sk_pdhg_options;
+
+void sk_pdhg_default_options(sk_pdhg_options* options);
+/* String-keyed setter over the reference's option names (src/util/options.cpp:78-455):
+ * algorithm (must be "pdhg"), backend ("gpu"|"auto"), gpu, pdhg_tolerance, iteration_limit,
+ * time_limit, pdhg_restart, pdhg_stop_at_request, random_seed, pdhg_polish (accepted, the
+ * finish is not part of this library: must be false), plus pdhg_precision ("fp32"|"fp64"),
+ * pdhg_absolute_primal and pdhg_steps_per_block. Unknown keys and malformed values are
+ * rejected with SK_GPU_INVALID_ARGUMENT, as options.cpp:481-565 rejects them. */
+int32_t sk_pdhg_set_option(sk_pdhg_options* options, const char* key, const char* value);
+
+/* Residuals (pdhg.cpp:81-132) of the reported point and Solution quality measures
+ * (model.cpp:228-422). */
+typedef struct {
+  int32_t status;
+  int32_t converged;
+  int64_t iterations;
+  int64_t restarts;
+  int64_t evaluations;
+  int64_t attempts;
+  double objective;
+  double dual_bound;
+  double spectral_norm;
+  double solve_seconds;
+  double rel_primal;
+  double rel_dual;
+  double rel_gap;
+  double abs_primal;
+  double abs_dual;
+  double gap_as_verified;
+  double complementarity;
+  double primal_objective;
+  double dual_objective;
+  double primal_infeasibility;
+  double primal_infeasibility_scaled;
+  double dual_infeasibility;
+  double dual_infeasibility_scaled;
+  double complementarity_violation;
+  double absolute_gap;
+  double relative_gap;
+  char algorithm[16]; /* "pdhg-gpu" (verify.cpp:86-89 audits on "gpu") */
+  char message[256];
+} sk_pdhg_result;
+
+/* The drop-in for pdhg::solve_pdhg (include/sankhya/pdhg.hpp:15; src/pdhg/pdhg.cpp:267-726)
+ * on one LP. Output arrays are caller-allocated (NULL skips one): col_value and col_dual
+ * num_cols, row_dual and row_activity num_rows. Signs as the reference Solution:
+ * col_dual = sense * (c_min + A^T y), row_dual = -sense * y (pdhg.cpp:652-661). */
+int32_t sk_pdhg_solve(const sk_lp* lp, const sk_pdhg_options* options, sk_pdhg_result* result,
+                      double* col_value, double* row_dual, double* col_dual,
+                      double* row_activity);
+
+/* ---- batches: many LPs of uneven size in one block-diagonal solve -------------------------- */
+typedef struct sk_batch sk_batch;
+
+typedef struct {
+  double setup_ms;  /* device scaling, power iteration, initial state */
+  double loop_ms;   /* PDHG graph launches until every instance stopped */
+  double report_ms; /* final evaluation and download */
+  int64_t graph_launches;
+  int64_t total_rows;
+  int64_t total_cols;
+  int64_t total_nnz;
+  int32_t instances;
+  int32_t spmv_stage;
+} sk_batch_stats;
+
+int32_t sk_batch_create(const sk_pdhg_options* options, sk_batch** out);
+/* The LP is borrowed: it must outlive sk_batch_upload. Returns the instance index. */
+int32_t sk_batch_add(sk_batch* batch, const sk_lp* lp, int32_t* index);
+/* Host CSR build, tiles and device allocation + upload (inputs resident afterwards). */
+int32_t sk_batch_upload(sk_batch* batch);
+/* One complete solve of every instance from the uploaded data, on the device. May be called
+ * again; each call starts from scratch. */
+int32_t sk_batch_solve(sk_batch* batch);
+int32_t sk_batch_result(sk_batch* batch, int32_t index, sk_pdhg_result* result,
+                        double* col_value, double* row_dual, double* col_dual,
+                        double* row_activity);
+int32_t sk_batch_stats_get(const sk_batch* batch, sk_batch_stats* out);
+/* Mean duration (ms) of one launch of a step kernel over the whole batch, measured with HIP
+ * events on the batch's stream after a solve: kernel 0 primal step, 1 A xbar + dual step,
+ * 2 A^T y + interaction. `bytes` receives the algorithmic bytes of one launch. */
+int32_t sk_batch_time_kernel(sk_batch* batch, int32_t kernel, int32_t reps, double* mean_ms,
+                             double* bytes);
+void sk_batch_destroy(sk_batch* batch);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SK_GPU_H */

@@ -1,0 +1,248 @@
+"""Python view of the C ABI (include/sk_gpu.h): LP handles, options, single solves, batches.
+
+Every call goes through libsk_gpu.so; there is no other execution path. Arrays cross as
+numpy buffers (int32 indices, float64 values).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+from typing import Any
+
+import numpy as np
+
+from . import _ffi
+
+_lib: C.CDLL | None = None
+
+STATUS_NAMES = {0: "not_solved", 1: "optimal", 2: "infeasible", 3: "unbounded",
+                4: "infeasible_or_unbounded", 5: "feasible", 6: "iteration_limit",
+                7: "time_limit", 8: "node_limit", 9: "numerical_error", 10: "model_error"}
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        _lib = _ffi.load()
+    return _lib
+
+
+class SkError(RuntimeError):
+    def __init__(self, status: int, where: str):
+        self.status = status
+        text = lib().sk_gpu_status_string(status).decode()
+        detail = lib().sk_gpu_last_error().decode()
+        super().__init__(f"{where}: {text} ({status}): {detail}")
+
+
+def check(status: int, where: str) -> None:
+    if status != 0:
+        raise SkError(status, where)
+
+
+def _ptr(a: np.ndarray | None) -> C.c_void_p | None:
+    return None if a is None else C.c_void_p(a.ctypes.data)
+
+
+def gpu_available() -> bool:
+    return bool(lib().sk_gpu_is_available())
+
+
+def device_info(device: int = 0) -> dict[str, Any]:
+    info = _ffi.DeviceInfo()
+    check(lib().sk_gpu_get_device_info(device, C.byref(info)), "sk_gpu_get_device_info")
+    return {f: (getattr(info, f).decode() if isinstance(getattr(info, f), bytes) else getattr(info, f))
+            for f, _ in info._fields_}
+
+
+class Lp:
+    """An LP held by the library (reference Model layout: CSC, two-sided rows)."""
+
+    def __init__(self, handle: C.c_void_p):
+        self._h = handle
+
+    @classmethod
+    def from_arrays(cls, num_rows: int, num_cols: int, col_starts, row_indices, values, cost,
+                    col_lower, col_upper, row_lower, row_upper, sense: int = 1,
+                    offset: float = 0.0) -> "Lp":
+        arrs = [np.ascontiguousarray(col_starts, np.int32), np.ascontiguousarray(row_indices, np.int32),
+                np.ascontiguousarray(values, np.float64), np.ascontiguousarray(cost, np.float64),
+                np.ascontiguousarray(col_lower, np.float64), np.ascontiguousarray(col_upper, np.float64),
+                np.ascontiguousarray(row_lower, np.float64), np.ascontiguousarray(row_upper, np.float64)]
+        h = C.c_void_p()
+        check(lib().sk_lp_create(num_rows, num_cols, sense, offset, *[_ptr(a) for a in arrs],
+                                 C.byref(h)), "sk_lp_create")
+        return cls(h)
+
+    @classmethod
+    def from_dense(cls, rows, row_lower, row_upper, cost, col_lower=None, col_upper=None,
+                   sense: int = 1, offset: float = 0.0) -> "Lp":
+        a = np.asarray(rows, np.float64).reshape(len(row_lower), len(cost))
+        m, n = a.shape
+        starts, idx, val = [0], [], []
+        for j in range(n):
+            for i in range(m):
+                if a[i, j] != 0.0:
+                    idx.append(i)
+                    val.append(a[i, j])
+            starts.append(len(idx))
+        lo = np.zeros(n) if col_lower is None else np.asarray(col_lower, float)
+        hi = np.full(n, np.inf) if col_upper is None else np.asarray(col_upper, float)
+        return cls.from_arrays(m, n, starts, idx, val, cost, lo, hi, row_lower, row_upper, sense, offset)
+
+    @classmethod
+    def generate(cls, config: int, instance: int) -> "Lp":
+        h = C.c_void_p()
+        check(lib().sk_lp_generate(config, instance, C.byref(h)), "sk_lp_generate")
+        return cls(h)
+
+    @staticmethod
+    def preset(config: int, instance: int) -> dict[str, Any]:
+        p = _ffi.GenParams()
+        check(lib().sk_gen_preset(config, instance, C.byref(p)), "sk_gen_preset")
+        return {f: getattr(p, f) for f, _ in p._fields_}
+
+    @classmethod
+    def generate_params(cls, **params) -> "Lp":
+        base = cls.preset(1, 1)
+        base.update(params)
+        p = _ffi.GenParams(**base)
+        h = C.c_void_p()
+        check(lib().sk_lp_generate_params(C.byref(p), C.byref(h)), "sk_lp_generate_params")
+        return cls(h)
+
+    @property
+    def dims(self) -> tuple[int, int, int]:
+        m, n, z = C.c_int32(), C.c_int32(), C.c_int64()
+        check(lib().sk_lp_dims(self._h, C.byref(m), C.byref(n), C.byref(z)), "sk_lp_dims")
+        return m.value, n.value, z.value
+
+    def export(self) -> dict[str, Any]:
+        m, n, z = self.dims
+        out = {"num_rows": m, "num_cols": n,
+               "col_starts": np.empty(n + 1, np.int32), "row_indices": np.empty(z, np.int32),
+               "values": np.empty(z), "cost": np.empty(n), "col_lower": np.empty(n),
+               "col_upper": np.empty(n), "row_lower": np.empty(m), "row_upper": np.empty(m)}
+        sense, off = C.c_int32(), C.c_double()
+        check(lib().sk_lp_export(self._h, C.byref(sense), C.byref(off),
+                                 *[_ptr(out[k]) for k in ("col_starts", "row_indices", "values", "cost",
+                                                          "col_lower", "col_upper", "row_lower",
+                                                          "row_upper")]), "sk_lp_export")
+        out["sense"] = sense.value
+        out["offset"] = off.value
+        return out
+
+    def close(self) -> None:
+        if self._h:
+            lib().sk_lp_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def make_options(**kw) -> _ffi.PdhgOptions:
+    """Options with the reference defaults; keyword names are the struct fields
+    (tolerance, iteration_limit, ...) or reference option names (pdhg_tolerance, ...)."""
+    o = _ffi.PdhgOptions()
+    lib().sk_pdhg_default_options(C.byref(o))
+    fields = {f for f, _ in o._fields_}
+    for k, v in kw.items():
+        if k in fields:
+            setattr(o, k, v)
+        else:
+            val = ("true" if v else "false") if isinstance(v, bool) else str(v)
+            check(lib().sk_pdhg_set_option(C.byref(o), k.encode(), val.encode()), f"option {k}")
+    return o
+
+
+@dataclass
+class SolveResult:
+    status: int
+    status_name: str
+    info: dict[str, Any]
+    col_value: np.ndarray
+    row_dual: np.ndarray
+    col_dual: np.ndarray
+    row_activity: np.ndarray
+    extra: dict[str, Any] = field(default_factory=dict)
+
+    def __getattr__(self, name):
+        info = self.__dict__.get("info", {})
+        if name in info:
+            return info[name]
+        raise AttributeError(name)
+
+
+def _result(res: _ffi.PdhgResult, x, yd, d, act) -> SolveResult:
+    info = {f: getattr(res, f) for f, _ in res._fields_}
+    info["algorithm"] = info["algorithm"].decode()
+    info["message"] = info["message"].decode()
+    return SolveResult(res.status, STATUS_NAMES.get(res.status, "?"), info, x, yd, d, act)
+
+
+def solve(lp: Lp, options: _ffi.PdhgOptions | None = None, **kw) -> SolveResult:
+    """The drop-in of pdhg::solve_pdhg on one LP (runs on the GPU)."""
+    o = options if options is not None else make_options(**kw)
+    m, n, _ = lp.dims
+    x, d, yd, act = np.zeros(n), np.zeros(n), np.zeros(m), np.zeros(m)
+    res = _ffi.PdhgResult()
+    check(lib().sk_pdhg_solve(lp._h, C.byref(o), C.byref(res), _ptr(x), _ptr(yd), _ptr(d), _ptr(act)),
+          "sk_pdhg_solve")
+    return _result(res, x, yd, d, act)
+
+
+class Batch:
+    """Many LPs solved together in one block-diagonal device solve."""
+
+    def __init__(self, options: _ffi.PdhgOptions | None = None, **kw):
+        self.options = options if options is not None else make_options(**kw)
+        h = C.c_void_p()
+        check(lib().sk_batch_create(C.byref(self.options), C.byref(h)), "sk_batch_create")
+        self._h = h
+        self._lps: list[Lp] = []
+
+    def add(self, lp: Lp) -> int:
+        k = C.c_int32()
+        check(lib().sk_batch_add(self._h, lp._h, C.byref(k)), "sk_batch_add")
+        self._lps.append(lp)
+        return k.value
+
+    def upload(self) -> None:
+        check(lib().sk_batch_upload(self._h), "sk_batch_upload")
+
+    def solve(self) -> None:
+        check(lib().sk_batch_solve(self._h), "sk_batch_solve")
+
+    def result(self, k: int) -> SolveResult:
+        m, n, _ = self._lps[k].dims
+        x, d, yd, act = np.zeros(n), np.zeros(n), np.zeros(m), np.zeros(m)
+        res = _ffi.PdhgResult()
+        check(lib().sk_batch_result(self._h, k, C.byref(res), _ptr(x), _ptr(yd), _ptr(d), _ptr(act)),
+              "sk_batch_result")
+        return _result(res, x, yd, d, act)
+
+    def stats(self) -> dict[str, Any]:
+        s = _ffi.BatchStats()
+        check(lib().sk_batch_stats_get(self._h, C.byref(s)), "sk_batch_stats_get")
+        return {f: getattr(s, f) for f, _ in s._fields_}
+
+    def time_kernel(self, kernel: int, reps: int = 20) -> tuple[float, float]:
+        ms, b = C.c_double(), C.c_double()
+        check(lib().sk_batch_time_kernel(self._h, kernel, reps, C.byref(ms), C.byref(b)),
+              "sk_batch_time_kernel")
+        return ms.value, b.value
+
+    def close(self) -> None:
+        if self._h:
+            lib().sk_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,526 @@
+// extern "C" entry points of include/sk_gpu.h for models, options, single solves and batches.
+// No C++ exception crosses this boundary: every entry point catches and reports a status, as
+// the reference's C API does with its `guarded` firewall (src/api/c_api.cpp:129-140).
+#include <algorithm>
+#include <cctype>
+#include <cfloat>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/sk_gpu.h"
+#include "engine.hpp"
+#include "lp.hpp"
+#include "quality.hpp"
+#include "status.hpp"
+
+struct sk_lp {
+  sk::HostLp lp;
+};
+
+struct sk_batch {
+  sk_pdhg_options options;
+  std::unique_ptr<sk::Engine> engine;
+  std::vector<const sk_lp*> lps;
+  bool uploaded = false;
+  bool solved = false;
+};
+
+namespace {
+
+template <class F>
+int32_t guarded(F&& body) {
+  try {
+    return body();
+  } catch (const std::bad_alloc&) {
+    sk::set_last_error("host out of memory");
+    return SK_GPU_OUT_OF_MEMORY;
+  } catch (const std::exception& e) {
+    sk::set_last_error(std::string("internal error: ") + e.what());
+    return SK_GPU_INVALID_STATE;
+  } catch (...) {
+    sk::set_last_error("internal error");
+    return SK_GPU_INVALID_STATE;
+  }
+}
+
+sk::EngineOptions to_engine(const sk_pdhg_options& o) {
+  sk::EngineOptions e;
+  e.tolerance = o.tolerance;
+  e.iteration_limit = o.iteration_limit;
+  e.time_limit = o.time_limit;
+  e.restart = o.restart;
+  e.stop_at_request = o.stop_at_request;
+  e.absolute_clause = o.absolute_primal;
+  e.random_seed = o.random_seed;
+  e.precision = o.precision;
+  e.steps_per_block = std::max(1, o.steps_per_block);
+  e.use_graph = o.use_graph;
+  e.stage = o.spmv_stage;
+  e.tile_nnz = o.tile_nnz;
+  return e;
+}
+
+std::string lower(const char* s) {
+  std::string r(s ? s : "");
+  for (auto& c : r) c = static_cast<char>(std::tolower(static_cast<unsigned char>(c)));
+  return r;
+}
+
+bool parse_bool(const std::string& v, int32_t* out) {
+  if (v == "true" || v == "1" || v == "on" || v == "yes") return *out = 1, true;
+  if (v == "false" || v == "0" || v == "off" || v == "no") return *out = 0, true;
+  return false;
+}
+
+bool parse_double(const std::string& v, double* out) {
+  if (v.empty()) return false;
+  char* end = nullptr;
+  const double d = std::strtod(v.c_str(), &end);
+  if (*end != '\0' || std::isnan(d)) return false;
+  *out = d;
+  return true;
+}
+
+bool parse_int(const std::string& v, int64_t* out) {
+  if (v.empty()) return false;
+  char* end = nullptr;
+  const long long d = std::strtoll(v.c_str(), &end, 10);
+  if (*end != '\0') return false;
+  *out = d;
+  return true;
+}
+
+void fill_message(const sk::InstanceResult& r, const sk_pdhg_options& o, char* out, size_t size) {
+  const sk::Residuals& f = r.final_residuals;
+  if (r.status == SK_SOLVE_OPTIMAL) {
+    std::snprintf(out, size,
+                  "converged after %lld iterations and %lld restarts; absolute primal %.3e, "
+                  "dual %.3e, relative gap %.3e",
+                  static_cast<long long>(r.iterations), static_cast<long long>(r.restarts),
+                  f.absolute_primal, f.absolute_dual, f.gap_as_verified);
+  } else if (r.status == SK_SOLVE_FEASIBLE) {
+    std::snprintf(out, size,
+                  "met the requested relative tolerance %.1e after %lld iterations, but NOT the "
+                  "absolute standard (primal %.3e vs 1e-07, dual %.3e vs 1e-07, relative gap "
+                  "%.3e vs 1e-09)",
+                  o.tolerance, static_cast<long long>(r.iterations), f.absolute_primal,
+                  f.absolute_dual, f.gap_as_verified);
+  } else if (r.status == SK_SOLVE_NUMERICAL_ERROR) {
+    std::snprintf(out, size, "a PDHG step produced a non-finite value after %lld iterations",
+                  static_cast<long long>(r.iterations));
+  } else {
+    std::snprintf(out, size,
+                  "stopped at relative primal %.3e, dual %.3e, gap %.3e after %lld iterations "
+                  "and %lld restarts (target %.1e)",
+                  f.primal, f.dual, f.gap, static_cast<long long>(r.iterations),
+                  static_cast<long long>(r.restarts), o.tolerance);
+  }
+}
+
+int32_t fill_result(const sk::HostLp& lp, const sk::InstanceResult& r, const sk_pdhg_options& o,
+                    double seconds, sk_pdhg_result* res, double* col_value, double* row_dual,
+                    double* col_dual, double* row_activity) {
+  const auto n = static_cast<size_t>(lp.num_cols), m = static_cast<size_t>(lp.num_rows);
+  if (col_value) std::copy(r.col_value.begin(), r.col_value.end(), col_value);
+  if (col_dual) std::copy(r.col_dual.begin(), r.col_dual.end(), col_dual);
+  if (row_dual) std::copy(r.row_dual.begin(), r.row_dual.end(), row_dual);
+  if (row_activity) std::copy(r.row_activity.begin(), r.row_activity.end(), row_activity);
+  if (!res) return SK_GPU_SUCCESS;
+  std::memset(res, 0, sizeof(*res));
+  res->status = r.status;
+  res->converged = r.converged;
+  res->iterations = r.iterations;
+  res->restarts = r.restarts;
+  res->evaluations = r.evaluations;
+  res->attempts = r.attempts;
+  res->dual_bound = r.dual_bound;
+  res->spectral_norm = r.spectral_norm;
+  res->solve_seconds = seconds;
+  const sk::Residuals& f = r.final_residuals;
+  res->rel_primal = f.primal;
+  res->rel_dual = f.dual;
+  res->rel_gap = f.gap;
+  res->abs_primal = f.absolute_primal;
+  res->abs_dual = f.absolute_dual;
+  res->gap_as_verified = f.gap_as_verified;
+  res->complementarity = f.complementarity;
+  res->primal_objective = f.primal_objective;
+  res->dual_objective = f.dual_objective;
+  // objective in the model's own sense, with its offset (model.cpp:99-106)
+  double objective = lp.objective_offset;
+  for (size_t j = 0; j < n; ++j) objective += lp.cost[j] * r.col_value[j];
+  res->objective = objective;
+  res->absolute_gap = std::fabs(objective - r.dual_bound);
+  res->relative_gap = res->absolute_gap / std::max(1.0, std::fabs(objective));
+  if (o.compute_quality && r.col_value.size() == n && r.row_dual.size() == m) {
+    sk::Quality q;
+    sk::recompute_quality(lp, r.col_value.data(), r.row_dual.data(), r.col_dual.data(),
+                          r.dual_bound, &q);
+    res->objective = q.objective;
+    res->primal_infeasibility = q.primal_infeasibility;
+    res->primal_infeasibility_scaled = q.primal_infeasibility_scaled;
+    res->dual_infeasibility = q.dual_infeasibility;
+    res->dual_infeasibility_scaled = q.dual_infeasibility_scaled;
+    res->complementarity_violation = q.complementarity_violation;
+    res->absolute_gap = q.absolute_gap;
+    res->relative_gap = q.relative_gap;
+  }
+  std::snprintf(res->algorithm, sizeof(res->algorithm), "pdhg-gpu");
+  fill_message(r, o, res->message, sizeof(res->message));
+  return SK_GPU_SUCCESS;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t sk_lp_create(int32_t num_rows, int32_t num_cols, int32_t sense, double objective_offset,
+                     const int32_t* col_starts, const int32_t* row_indices, const double* values,
+                     const double* col_cost, const double* col_lower, const double* col_upper,
+                     const double* row_lower, const double* row_upper, sk_lp** out) {
+  return guarded([&]() -> int32_t {
+    if (!out) return SK_GPU_INVALID_ARGUMENT;
+    *out = nullptr;
+    if (num_rows < 0 || num_cols < 0 || (sense != 1 && sense != -1) || !col_starts ||
+        (num_cols > 0 && (!col_cost || !col_lower || !col_upper)) ||
+        (num_rows > 0 && (!row_lower || !row_upper))) {
+      sk::set_last_error("sk_lp_create: bad dimensions, sense or null array");
+      return SK_GPU_INVALID_ARGUMENT;
+    }
+    auto h = std::make_unique<sk_lp>();
+    sk::HostLp& lp = h->lp;
+    lp.num_rows = num_rows;
+    lp.num_cols = num_cols;
+    lp.sense = sense;
+    lp.objective_offset = objective_offset;
+    lp.col_starts.assign(col_starts, col_starts + num_cols + 1);
+    const int64_t nnz = lp.col_starts.back();
+    if (nnz < 0 || (nnz > 0 && (!row_indices || !values))) {
+      sk::set_last_error("sk_lp_create: bad nonzero count or null matrix arrays");
+      return SK_GPU_INVALID_ARGUMENT;
+    }
+    lp.row_indices.assign(row_indices, row_indices + nnz);
+    lp.values.assign(values, values + nnz);
+    lp.cost.assign(col_cost, col_cost + num_cols);
+    lp.col_lower.assign(col_lower, col_lower + num_cols);
+    lp.col_upper.assign(col_upper, col_upper + num_cols);
+    lp.row_lower.assign(row_lower, row_lower + num_rows);
+    lp.row_upper.assign(row_upper, row_upper + num_rows);
+    const std::string problem = lp.validate();
+    if (!problem.empty()) {
+      sk::set_last_error(problem);
+      return SK_GPU_INVALID_ARGUMENT;
+    }
+    *out = h.release();
+    return SK_GPU_SUCCESS;
+  });
+}
+
+int32_t sk_gen_preset(int32_t config, uint64_t instance, sk_gen_params* out) {
+  if (!out) return SK_GPU_INVALID_ARGUMENT;
+  sk::GenParams p;
+  if (!sk::preset_params(config, instance, &p)) {
+    sk::set_last_error("unknown workload; expected 1..5");
+    return SK_GPU_INVALID_ARGUMENT;
+  }
+  out->seed = p.seed;
+  out->num_rows = p.num_rows;
+  out->num_cols = p.num_cols;
+  out->num_equality_rows = p.num_equality_rows;
+  out->degree_kind = p.degree_kind;
+  out->degree_lo = p.degree_lo;
+  out->degree_hi = p.degree_hi;
+  out->power_exponent = p.power_exponent;
+  out->target_nnz = p.target_nnz;
+  out->value_kind = p.value_kind;
+  out->value_lo = p.value_lo;
+  out->value_hi = p.value_hi;
+  out->negative_probability = p.negative_probability;
+  out->upper_lo = p.upper_lo;
+  out->upper_hi = p.upper_hi;
+  out->slack_lo = p.slack_lo;
+  out->slack_hi = p.slack_hi;
+  out->cost_lo = p.cost_lo;
+  out->cost_hi = p.cost_hi;
+  out->fix_empty_rows = p.fix_empty_rows;
+  return SK_GPU_SUCCESS;
+}
+
+int32_t sk_lp_generate_params(const sk_gen_params* in, sk_lp** out) {
+  return guarded([&]() -> int32_t {
+    if (!in || !out || in->num_rows < 0 || in->num_cols < 0) return SK_GPU_INVALID_ARGUMENT;
+    sk::GenParams p;
+    p.seed = in->seed;
+    p.num_rows = in->num_rows;
+    p.num_cols = in->num_cols;
+    p.num_equality_rows = in->num_equality_rows;
+    p.degree_kind = in->degree_kind;
+    p.degree_lo = in->degree_lo;
+    p.degree_hi = in->degree_hi;
+    p.power_exponent = in->power_exponent;
+    p.target_nnz = in->target_nnz;
+    p.value_kind = in->value_kind;
+    p.value_lo = in->value_lo;
+    p.value_hi = in->value_hi;
+    p.negative_probability = in->negative_probability;
+    p.upper_lo = in->upper_lo;
+    p.upper_hi = in->upper_hi;
+    p.slack_lo = in->slack_lo;
+    p.slack_hi = in->slack_hi;
+    p.cost_lo = in->cost_lo;
+    p.cost_hi = in->cost_hi;
+    p.fix_empty_rows = in->fix_empty_rows;
+    auto h = std::make_unique<sk_lp>();
+    h->lp = sk::generate_lp(p);
+    *out = h.release();
+    return SK_GPU_SUCCESS;
+  });
+}
+
+int32_t sk_lp_generate(int32_t config, uint64_t instance, sk_lp** out) {
+  sk_gen_params p;
+  const int32_t st = sk_gen_preset(config, instance, &p);
+  if (st) return st;
+  return sk_lp_generate_params(&p, out);
+}
+
+int32_t sk_lp_dims(const sk_lp* lp, int32_t* num_rows, int32_t* num_cols, int64_t* nnz) {
+  if (!lp) return SK_GPU_INVALID_ARGUMENT;
+  if (num_rows) *num_rows = lp->lp.num_rows;
+  if (num_cols) *num_cols = lp->lp.num_cols;
+  if (nnz) *nnz = lp->lp.nnz();
+  return SK_GPU_SUCCESS;
+}
+
+int32_t sk_lp_export(const sk_lp* h, int32_t* sense, double* offset, int32_t* col_starts,
+                     int32_t* row_indices, double* values, double* col_cost, double* col_lower,
+                     double* col_upper, double* row_lower, double* row_upper) {
+  if (!h) return SK_GPU_INVALID_ARGUMENT;
+  const sk::HostLp& lp = h->lp;
+  if (sense) *sense = lp.sense;
+  if (offset) *offset = lp.objective_offset;
+  auto put = [](const auto& v, auto* dst) {
+    if (dst) std::copy(v.begin(), v.end(), dst);
+  };
+  put(lp.col_starts, col_starts);
+  put(lp.row_indices, row_indices);
+  put(lp.values, values);
+  put(lp.cost, col_cost);
+  put(lp.col_lower, col_lower);
+  put(lp.col_upper, col_upper);
+  put(lp.row_lower, row_lower);
+  put(lp.row_upper, row_upper);
+  return SK_GPU_SUCCESS;
+}
+
+void sk_lp_free(sk_lp* lp) { delete lp; }
+
+void sk_pdhg_default_options(sk_pdhg_options* o) {
+  if (!o) return;
+  std::memset(o, 0, sizeof(*o));
+  o->tolerance = 1e-4;
+  o->iteration_limit = -1;
+  o->time_limit = DBL_MAX;
+  o->restart = 1;
+  o->stop_at_request = 0;
+  o->absolute_primal = 1;
+  o->random_seed = 0;
+  o->precision = 0;
+  o->steps_per_block = 40;
+  o->use_graph = 1;
+  o->spmv_stage = -1;
+  o->tile_nnz = 0;
+  o->compute_quality = 1;
+}
+
+int32_t sk_pdhg_set_option(sk_pdhg_options* o, const char* key_in, const char* value_in) {
+  if (!o || !key_in || !value_in) return SK_GPU_INVALID_ARGUMENT;
+  const std::string key = lower(key_in);
+  const std::string v = lower(value_in);
+  int32_t b = 0;
+  double d = 0.0;
+  int64_t i = 0;
+  auto bad = [&](const char* what) {
+    sk::set_last_error("option " + key + ": " + what + " '" + value_in + "'");
+    return SK_GPU_INVALID_ARGUMENT;
+  };
+  if (key == "algorithm") {
+    if (v != "pdhg") return bad("this library runs only algorithm=pdhg, not");
+    return SK_GPU_SUCCESS;
+  }
+  if (key == "backend") {
+    if (v != "gpu" && v != "auto") return bad("this library is the gpu backend; cannot take");
+    return SK_GPU_SUCCESS;
+  }
+  if (key == "gpu") {
+    if (!parse_bool(v, &b)) return bad("expected a boolean, got");
+    return SK_GPU_SUCCESS;
+  }
+  if (key == "pdhg_polish") {
+    if (!parse_bool(v, &b)) return bad("expected a boolean, got");
+    if (b) return bad("the interior-point finish is not part of this library; cannot take");
+    return SK_GPU_SUCCESS;
+  }
+  if (key == "pdhg_tolerance") {
+    if (!parse_double(v, &d) || d < 1e-14 || d > 1e-1) return bad("expected a number in [1e-14, 0.1], got");
+    o->tolerance = d;
+    return SK_GPU_SUCCESS;
+  }
+  if (key == "iteration_limit") {
+    if (!parse_int(v, &i) || i < -1) return bad("expected an integer >= -1, got");
+    o->iteration_limit = i;
+    return SK_GPU_SUCCESS;
+  }
+  if (key == "time_limit") {
+    if (!parse_double(v, &d) || d < 0.0) return bad("expected a non-negative number, got");
+    o->time_limit = d;
+    return SK_GPU_SUCCESS;
+  }
+  if (key == "random_seed") {
+    if (!parse_int(v, &i) || i < 0) return bad("expected a non-negative integer, got");
+    o->random_seed = i;
+    return SK_GPU_SUCCESS;
+  }
+  if (key == "pdhg_restart" || key == "pdhg_stop_at_request" || key == "pdhg_absolute_primal") {
+    if (!parse_bool(v, &b)) return bad("expected a boolean, got");
+    if (key == "pdhg_restart") o->restart = b;
+    else if (key == "pdhg_stop_at_request") o->stop_at_request = b;
+    else o->absolute_primal = b;
+    return SK_GPU_SUCCESS;
+  }
+  if (key == "pdhg_precision") {
+    if (v == "fp32") o->precision = 0;
+    else if (v == "fp64") o->precision = 1;
+    else return bad("expected fp32 or fp64, got");
+    return SK_GPU_SUCCESS;
+  }
+  if (key == "pdhg_steps_per_block") {
+    if (!parse_int(v, &i) || i < 1 || i > 400) return bad("expected an integer in [1, 400], got");
+    o->steps_per_block = static_cast<int32_t>(i);
+    return SK_GPU_SUCCESS;
+  }
+  sk::set_last_error("unknown option '" + std::string(key_in) + "'");
+  return SK_GPU_INVALID_ARGUMENT;
+}
+
+int32_t sk_pdhg_solve(const sk_lp* lp, const sk_pdhg_options* options, sk_pdhg_result* result,
+                      double* col_value, double* row_dual, double* col_dual, double* row_activity) {
+  return guarded([&]() -> int32_t {
+    if (!lp) return SK_GPU_INVALID_ARGUMENT;
+    sk_pdhg_options o;
+    if (options) o = *options;
+    else sk_pdhg_default_options(&o);
+    const auto t0 = std::chrono::steady_clock::now();
+    auto engine = sk::make_engine(to_engine(o));
+    engine->add(&lp->lp);
+    int32_t st = engine->upload();
+    if (st) return st;
+    if ((st = engine->solve())) return st;
+    sk::InstanceResult r;
+    if ((st = engine->result(0, &r))) return st;
+    const double seconds =
+        std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return fill_result(lp->lp, r, o, seconds, result, col_value, row_dual, col_dual, row_activity);
+  });
+}
+
+int32_t sk_batch_create(const sk_pdhg_options* options, sk_batch** out) {
+  return guarded([&]() -> int32_t {
+    if (!out) return SK_GPU_INVALID_ARGUMENT;
+    auto b = std::make_unique<sk_batch>();
+    if (options) b->options = *options;
+    else sk_pdhg_default_options(&b->options);
+    b->engine = sk::make_engine(to_engine(b->options));
+    *out = b.release();
+    return SK_GPU_SUCCESS;
+  });
+}
+
+int32_t sk_batch_add(sk_batch* b, const sk_lp* lp, int32_t* index) {
+  if (!b || !lp) return SK_GPU_INVALID_ARGUMENT;
+  if (b->uploaded) {
+    sk::set_last_error("sk_batch_add after sk_batch_upload");
+    return SK_GPU_INVALID_STATE;
+  }
+  return guarded([&]() -> int32_t {
+    const int32_t k = b->engine->add(&lp->lp);
+    b->lps.push_back(lp);
+    if (index) *index = k;
+    return SK_GPU_SUCCESS;
+  });
+}
+
+int32_t sk_batch_upload(sk_batch* b) {
+  if (!b) return SK_GPU_INVALID_ARGUMENT;
+  return guarded([&]() -> int32_t {
+    const int32_t st = b->engine->upload();
+    if (!st) b->uploaded = true;
+    return st;
+  });
+}
+
+int32_t sk_batch_solve(sk_batch* b) {
+  if (!b) return SK_GPU_INVALID_ARGUMENT;
+  if (!b->uploaded) {
+    sk::set_last_error("sk_batch_solve before sk_batch_upload");
+    return SK_GPU_INVALID_STATE;
+  }
+  return guarded([&]() -> int32_t {
+    const int32_t st = b->engine->solve();
+    b->solved = st == SK_GPU_SUCCESS;
+    return st;
+  });
+}
+
+int32_t sk_batch_result(sk_batch* b, int32_t index, sk_pdhg_result* result, double* col_value,
+                        double* row_dual, double* col_dual, double* row_activity) {
+  if (!b || !b->solved || index < 0 || index >= static_cast<int32_t>(b->lps.size())) {
+    sk::set_last_error("sk_batch_result: no solved instance with that index");
+    return SK_GPU_INVALID_ARGUMENT;
+  }
+  return guarded([&]() -> int32_t {
+    sk::InstanceResult r;
+    const int32_t st = b->engine->result(index, &r);
+    if (st) return st;
+    const auto& t = b->engine->timings();
+    const double seconds = (t.setup_ms + t.loop_ms + t.report_ms) * 1e-3;
+    return fill_result(b->lps[static_cast<size_t>(index)]->lp, r, b->options, seconds, result,
+                       col_value, row_dual, col_dual, row_activity);
+  });
+}
+
+int32_t sk_batch_stats_get(const sk_batch* b, sk_batch_stats* out) {
+  if (!b || !out) return SK_GPU_INVALID_ARGUMENT;
+  const auto& t = b->engine->timings();
+  out->setup_ms = t.setup_ms;
+  out->loop_ms = t.loop_ms;
+  out->report_ms = t.report_ms;
+  out->graph_launches = t.graph_launches;
+  out->total_rows = b->engine->total_rows();
+  out->total_cols = b->engine->total_cols();
+  out->total_nnz = b->engine->total_nnz();
+  out->instances = b->engine->size();
+  out->spmv_stage = b->engine->stage_choice();
+  return SK_GPU_SUCCESS;
+}
+
+int32_t sk_batch_time_kernel(sk_batch* b, int32_t kernel, int32_t reps, double* mean_ms,
+                             double* bytes) {
+  if (!b || !mean_ms || kernel < 0 || kernel > 2 || !b->uploaded) return SK_GPU_INVALID_ARGUMENT;
+  return guarded([&]() -> int32_t {
+    const int32_t st = b->engine->time_kernel(kernel, reps, mean_ms);
+    if (bytes) *bytes = b->engine->kernel_bytes(kernel);
+    return st;
+  });
+}
+
+void sk_batch_destroy(sk_batch* b) { delete b; }
+
+}  // extern "C"

@@ -1,0 +1,137 @@
+// Device helpers shared by every HIP translation unit of the library: bound tests with the
+// reference's semantics, projection, and deterministic wavefront/workgroup reductions, and
+// the row-block sparse product used by all CSR kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+
+#include "device_types.hpp"
+
+namespace sk {
+
+// ---- small device helpers ------------------------------------------------------------------
+
+__device__ __forceinline__ bool dev_finite_bound(double v) { return !(isinf(v) || fabs(v) >= 1e30); }
+__device__ __forceinline__ double dev_max(double a, double b) { return a < b ? b : a; }  // std::max
+__device__ __forceinline__ double dev_min(double a, double b) { return b < a ? b : a; }  // std::min
+
+// Bounds reach the kernels normalised: an absent side is +-infinity, so a plain comparison
+// projects (pdhg.cpp:63-67).
+template <class T>
+__device__ __forceinline__ T dev_project(T v, T lo, T hi) {
+  if (v < lo) return lo;
+  if (v > hi) return hi;
+  return v;
+}
+
+struct D2 {
+  double a, b;
+};
+__device__ __forceinline__ D2 operator+(D2 x, D2 y) { return D2{x.a + y.a, x.b + y.b}; }
+
+__device__ __forceinline__ float shfl_xor(float v, int m) { return __shfl_xor(v, m, kWave); }
+__device__ __forceinline__ double shfl_xor(double v, int m) { return __shfl_xor(v, m, kWave); }
+__device__ __forceinline__ D2 shfl_xor(D2 v, int m) {
+  return D2{__shfl_xor(v.a, m, kWave), __shfl_xor(v.b, m, kWave)};
+}
+template <class A>
+__device__ __forceinline__ A zero_of() {
+  return A{};
+}
+
+// Butterfly over the `width` lanes of a group (width a power of two <= 64). Every lane of
+// the group ends with the same value; the order of additions is fixed.
+template <class A>
+__device__ __forceinline__ A group_sum(A v, int width) {
+  for (int o = width >> 1; o >= 1; o >>= 1) v = v + shfl_xor(v, o);
+  return v;
+}
+
+// Deterministic workgroup reductions; the result is valid in every thread.
+__device__ __forceinline__ double block_sum(double v, double* lds) {
+  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+  v = group_sum(v, kWave);
+  if (lane == 0) lds[wave] = v;
+  __syncthreads();
+  double r = 0.0;
+  for (int w = 0; w < kBlock / kWave; ++w) r += lds[w];
+  __syncthreads();
+  return r;
+}
+
+__device__ __forceinline__ double block_max(double v, double* lds) {
+  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+  for (int o = kWave >> 1; o >= 1; o >>= 1) v = dev_max(v, shfl_xor(v, o));
+  if (lane == 0) lds[wave] = v;
+  __syncthreads();
+  double r = lds[0];
+  for (int w = 1; w < kBlock / kWave; ++w) r = dev_max(r, lds[w]);
+  __syncthreads();
+  return r;
+}
+
+template <class A>
+__device__ __forceinline__ A block_sum_any(A v, A* lds) {
+  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+  v = group_sum(v, kWave);
+  if (lane == 0) lds[wave] = v;
+  __syncthreads();
+  A r = lds[0];
+  for (int w = 1; w < kBlock / kWave; ++w) r = r + lds[w];
+  __syncthreads();
+  return r;
+}
+
+__device__ __forceinline__ int pow2_floor(int v) {
+  int p = 1;
+  while (p * 2 <= v) p *= 2;
+  return p;
+}
+
+// ---- the row-block product ---------------------------------------------------------------
+// Computes acc_r = sum_p prod(p) for every row r of tile t of a CSR matrix and calls
+// epi(r, acc_r) exactly once per row, on one lane.
+//
+// Stream tile (rows > 1 or nnz <= cap): when STAGE, all lanes first write prod(p) for the
+// tile's nonzeros into LDS with coalesced loads (CSR-Stream; Greathouse & Daga, SC'14), then
+// G = 256 / rows lanes (power of two, <= 64) reduce each row from LDS. Without STAGE the
+// same G-lane groups read their row straight from memory.
+// Vector tile (one row with nnz > cap): the workgroup strides over the row and reduces.
+template <class A, bool STAGE, class Prod, class Epi>
+__device__ __forceinline__ void rowblock_product(const Tile t, const int32_t* __restrict__ ptr,
+                                                 Prod prod, Epi epi, A* stage, A* red_lds) {
+  const int tid = threadIdx.x;
+  const int r0 = t.begin, nrows = t.end - t.begin;
+  const int p0 = ptr[r0];
+  const int p1 = ptr[t.end];
+  const int nnz = p1 - p0;
+  if (nrows == 1 && nnz > t.cap) {
+    A acc = zero_of<A>();
+    for (int p = p0 + tid; p < p1; p += kBlock) acc = acc + prod(p);
+    acc = block_sum_any(acc, red_lds);
+    if (tid == 0) epi(r0, acc);
+    return;
+  }
+  if (STAGE) {
+    for (int k = tid; k < nnz; k += kBlock) stage[k] = prod(p0 + k);
+    __syncthreads();
+  }
+  const int g = nrows > 0 ? min(kWave, pow2_floor(kBlock / nrows)) : kWave;
+  const int row = tid / g, lane = tid % g;
+  A acc = zero_of<A>();
+  if (row < nrows) {
+    const int b = ptr[r0 + row], e = ptr[r0 + row + 1];
+    if (STAGE) {
+      for (int k = b - p0 + lane; k < e - p0; k += g) acc = acc + stage[k];
+    } else {
+      for (int p = b + lane; p < e; p += g) acc = acc + prod(p);
+    }
+  }
+  acc = group_sum(acc, g);
+  if (row < nrows && lane == 0) epi(r0 + row, acc);
+}
+
+}  // namespace sk

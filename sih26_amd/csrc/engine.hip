@@ -1,0 +1,791 @@
+// Host driver of the batched restarted-PDHG engine (see engine.hpp and kernels.hpp).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "engine.hpp"
+#include "kernels.hpp"
+#include "status.hpp"
+
+namespace sk {
+
+namespace {
+
+thread_local std::string g_last_error;
+
+#define SK_HIP(call, code)                                                              \
+  do {                                                                                  \
+    const hipError_t e_ = (call);                                                       \
+    if (e_ != hipSuccess) {                                                             \
+      set_last_error(std::string(#call) + ": " + hipGetErrorString(e_));                \
+      return (code);                                                                    \
+    }                                                                                   \
+  } while (0)
+
+#define SK_LAUNCH(...)                                                                  \
+  do {                                                                                  \
+    __VA_ARGS__;                                                                        \
+    const hipError_t e_ = hipGetLastError();                                            \
+    if (e_ != hipSuccess) {                                                             \
+      set_last_error(std::string("kernel launch: ") + hipGetErrorString(e_));           \
+      return kStatusLaunchFailed;                                                       \
+    }                                                                                   \
+  } while (0)
+
+inline unsigned blocks_for(int64_t n) {
+  return static_cast<unsigned>(std::max<int64_t>(1, (n + kBlock - 1) / kBlock));
+}
+
+// Device buffer owned by the engine.
+struct DevBuf {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+  ~DevBuf() {
+    if (ptr) (void)hipFree(ptr);
+  }
+};
+
+// Row-block tiles over rows [row0, row0 + rows) of a CSR matrix: consecutive rows share a
+// tile while their nonzeros fit `cap` and there are at most kBlock of them; a row with more
+// than `cap` nonzeros gets a tile of its own. Every instance gets at least one tile.
+void build_row_tiles(const std::vector<int32_t>& ptr, int32_t row0, int32_t rows, int32_t inst,
+                     int32_t cap, std::vector<Tile>* out) {
+  const size_t before = out->size();
+  int32_t start = row0;
+  int64_t acc = 0;
+  const int32_t end = row0 + rows;
+  for (int32_t r = row0; r < end; ++r) {
+    const int64_t len = ptr[static_cast<size_t>(r) + 1] - ptr[static_cast<size_t>(r)];
+    if (len > cap) {
+      if (start < r) out->push_back(Tile{start, r, inst, cap});
+      out->push_back(Tile{r, r + 1, inst, cap});
+      start = r + 1;
+      acc = 0;
+      continue;
+    }
+    if (acc + len > cap || r - start == kBlock) {
+      out->push_back(Tile{start, r, inst, cap});
+      start = r;
+      acc = 0;
+    }
+    acc += len;
+  }
+  if (start < end) out->push_back(Tile{start, end, inst, cap});
+  if (out->size() == before) out->push_back(Tile{row0, row0, inst, cap});
+}
+
+void build_chunks(int32_t begin, int32_t count, int32_t inst, std::vector<Tile>* out) {
+  const size_t before = out->size();
+  for (int32_t k = 0; k < count; k += kChunk)
+    out->push_back(Tile{begin + k, begin + std::min(count, k + kChunk), inst, 0});
+  if (out->size() == before) out->push_back(Tile{begin, begin, inst, 0});
+}
+
+template <class T>
+class EngineImpl final : public Engine {
+ public:
+  explicit EngineImpl(const EngineOptions& o) : opt_(o) {}
+  ~EngineImpl() override {
+    if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
+    if (stream_) (void)hipStreamDestroy(stream_);
+    if (host_done_) (void)hipHostFree(host_done_);
+  }
+
+  int32_t add(const HostLp* lp) override {
+    lps_.push_back(lp);
+    return static_cast<int32_t>(lps_.size()) - 1;
+  }
+  int32_t size() const override { return static_cast<int32_t>(lps_.size()); }
+  const EngineTimings& timings() const override { return timings_; }
+  int64_t total_nnz() const override { return nnz_; }
+  int64_t total_rows() const override { return m_; }
+  int64_t total_cols() const override { return n_; }
+  int stage_choice() const override { return stage_; }
+
+  int upload() override;
+  int solve() override;
+  int result(int32_t index, InstanceResult* out) const override;
+  int time_kernel(int kernel, int reps, double* mean_ms) override;
+  double kernel_bytes(int kernel) const override;
+
+ private:
+  template <class U>
+  int alloc(DevBuf& b, size_t count, U** typed) {
+    b.bytes = std::max<size_t>(1, count) * sizeof(U);
+    SK_HIP(hipMalloc(&b.ptr, b.bytes), kStatusOutOfMemory);
+    *typed = static_cast<U*>(b.ptr);
+    return kStatusSuccess;
+  }
+  template <class U>
+  int put(DevBuf& b, const std::vector<U>& host, U** typed) {
+    int st = alloc(b, host.size(), typed);
+    if (st) return st;
+    if (!host.empty())
+      SK_HIP(hipMemcpy(b.ptr, host.data(), host.size() * sizeof(U), hipMemcpyHostToDevice),
+             kStatusCopyFailed);
+    return kStatusSuccess;
+  }
+
+  int setup_device();
+  int run_loop();
+  int report();
+  int enqueue_block();
+  int enqueue_eval(int report_mode);
+
+  EngineOptions opt_;
+  std::vector<const HostLp*> lps_;
+  int64_t m_ = 0, n_ = 0, nnz_ = 0;
+  int32_t K_ = 0;
+  int stage_ = 1;
+
+  // host copies needed after upload
+  std::vector<InstDesc> desc_;
+  std::vector<InstState> init_state_;
+  std::vector<int32_t> sense_;
+  std::vector<double> v0_;  // power-iteration start, all instances
+  std::vector<uint8_t> trivial_norm_;
+  int32_t n_ta_ = 0, n_tt_ = 0, n_tc_ = 0, n_tr_ = 0;
+
+  hipStream_t stream_ = nullptr;
+  hipGraphExec_t graph_exec_ = nullptr;
+  uint32_t* host_done_ = nullptr;
+
+  // device buffers
+  DevBuf b_a_ptr_, b_a_idx_, b_a_orig_, b_t_ptr_, b_t_idx_, b_t_orig_;
+  DevBuf b_sa_, b_st_, b_a_val_, b_t_val_;
+  DevBuf b_cmin_, b_cl_, b_cu_, b_rl_, b_ru_, b_dc_, b_dr_, b_rpass_, b_cpass_;
+  DevBuf b_chat_, b_lhat_, b_uhat_, b_rlhat_, b_ruhat_, b_lhat64_, b_uhat64_, b_scratch64_[3];
+  DevBuf b_x0_, b_x1_, b_y0_, b_y1_, b_aty0_, b_aty1_, b_xbar_, b_xrst_, b_yrst_;
+  DevBuf b_xsum_, b_ysum_, b_xu0_, b_xu1_, b_yu0_, b_yu1_, b_bestx_, b_besty_, b_act_, b_red_;
+  DevBuf b_tc_, b_tr_, b_ta_, b_tt_, b_desc_, b_state_;
+  DevBuf b_ps_c_, b_ps_a_, b_ps_t_, b_pe_c_, b_pe_r_, b_pe_a_, b_pe_t_;
+  DevBuf b_arrive_step_, b_arrive_eval_, b_done_, b_v_, b_av_, b_len_, b_zero_;
+
+  Args<T> args_{};
+  double* d_sa_ = nullptr;
+  double* d_st_ = nullptr;
+  double* d_rpass_ = nullptr;
+  double* d_cpass_ = nullptr;
+  double* d_lhat64_ = nullptr;
+  double* d_uhat64_ = nullptr;
+  double* d_v_ = nullptr;
+  double* d_av_ = nullptr;
+  double* d_len_ = nullptr;
+  double* d_ones_m_ = nullptr;
+  double* d_ones_n_ = nullptr;
+  uint8_t* d_zero_seen_ = nullptr;
+
+  // results
+  std::vector<InstState> final_state_;
+  std::vector<double> bestx_, besty_, act_, red_;
+  EngineTimings timings_;
+};
+
+template <class T>
+int EngineImpl<T>::upload() {
+  K_ = static_cast<int32_t>(lps_.size());
+  m_ = n_ = nnz_ = 0;
+  for (size_t k = 0; k < lps_.size(); ++k) {
+    const HostLp* lp = lps_[k];
+    const std::string problem = lp->validate();
+    if (!problem.empty()) {
+      set_last_error("instance " + std::to_string(k) + ": " + problem);
+      return kStatusInvalidArgument;
+    }
+    m_ += lp->num_rows;
+    n_ += lp->num_cols;
+    nnz_ += lp->nnz();
+  }
+  if (m_ >= INT32_MAX || n_ >= INT32_MAX || nnz_ >= INT32_MAX) {
+    set_last_error("batch exceeds 32-bit row, column or nonzero offsets");
+    return kStatusInvalidArgument;
+  }
+  const auto M = static_cast<size_t>(m_), N = static_cast<size_t>(n_), Z = static_cast<size_t>(nnz_);
+
+  // Block-diagonal CSR(A) (columns ascending in each row, the order of the reference's CSC
+  // scatter) and CSR(A^T) (= the CSC of A, rows ascending in each column).
+  std::vector<int32_t> a_ptr(M + 1, 0), a_idx(Z), t_ptr(N + 1, 0), t_idx(Z);
+  std::vector<double> a_orig(Z), t_orig(Z), cmin(N), cl(N), cu(N), rl(M), ru(M);
+  desc_.assign(static_cast<size_t>(K_), InstDesc{});
+  init_state_.assign(static_cast<size_t>(K_), InstState{});
+  sense_.assign(static_cast<size_t>(K_), 1);
+  trivial_norm_.assign(static_cast<size_t>(K_), 0);
+  v0_.assign(N, 0.0);
+  std::vector<Tile> ta, tt, tc, tr;
+  int32_t row0 = 0, col0 = 0, nz0 = 0;
+  const int64_t limit = opt_.iteration_limit < 0 ? 1000000 : opt_.iteration_limit;
+  for (int32_t k = 0; k < K_; ++k) {
+    const HostLp& lp = *lps_[static_cast<size_t>(k)];
+    const int32_t m = lp.num_rows, n = lp.num_cols;
+    // counts per row
+    for (int32_t p = 0; p < lp.nnz(); ++p) ++a_ptr[static_cast<size_t>(row0 + lp.row_indices[p]) + 1];
+    for (int32_t j = 0; j < n; ++j) {
+      t_ptr[static_cast<size_t>(col0 + j) + 1] = nz0 + lp.col_starts[static_cast<size_t>(j) + 1];
+      for (int32_t p = lp.col_starts[j]; p < lp.col_starts[j + 1]; ++p) {
+        t_idx[static_cast<size_t>(nz0 + p)] = row0 + lp.row_indices[p];
+        t_orig[static_cast<size_t>(nz0 + p)] = lp.values[p];
+      }
+    }
+    if (n == 0) t_ptr[static_cast<size_t>(col0)] = nz0;
+    const double sense = static_cast<double>(lp.sense);
+    sense_[static_cast<size_t>(k)] = lp.sense;
+    double cost_square = 0.0, bound_square = 0.0;
+    for (int32_t j = 0; j < n; ++j) {
+      cmin[static_cast<size_t>(col0 + j)] = sense * lp.cost[j];
+      cost_square += cmin[static_cast<size_t>(col0 + j)] * cmin[static_cast<size_t>(col0 + j)];
+      cl[static_cast<size_t>(col0 + j)] = lp.col_lower[j];
+      cu[static_cast<size_t>(col0 + j)] = lp.col_upper[j];
+    }
+    for (int32_t i = 0; i < m; ++i) {
+      rl[static_cast<size_t>(row0 + i)] = lp.row_lower[i];
+      ru[static_cast<size_t>(row0 + i)] = lp.row_upper[i];
+      const double b = is_finite_bound(lp.row_lower[i])
+                           ? lp.row_lower[i]
+                           : (is_finite_bound(lp.row_upper[i]) ? lp.row_upper[i] : 0.0);
+      bound_square += b * b;
+    }
+    // power-iteration start (scaling.cpp:168-171), the same libstdc++ engine and distribution
+    if (m > 0 && n > 0 && lp.nnz() > 0) {
+      std::mt19937 rng(static_cast<unsigned>(opt_.random_seed) + 1u);
+      std::uniform_real_distribution<double> spread(-1.0, 1.0);
+      for (int32_t j = 0; j < n; ++j) v0_[static_cast<size_t>(col0 + j)] = spread(rng);
+    } else {
+      trivial_norm_[static_cast<size_t>(k)] = 1;
+    }
+    InstDesc& d = desc_[static_cast<size_t>(k)];
+    d.row0 = row0;
+    d.rows = m;
+    d.col0 = col0;
+    d.cols = n;
+    InstState& s = init_state_[static_cast<size_t>(k)];
+    s.tolerance = opt_.tolerance;
+    s.cost_norm = std::sqrt(cost_square);
+    s.bound_norm = std::sqrt(bound_square);
+    s.omega = 1.0;
+    s.restart_kkt = INFINITY;
+    s.best_worst = INFINITY;
+    s.iteration_limit = limit;
+    s.use_restarts = opt_.restart != 0;
+    s.stop_at_request = opt_.stop_at_request != 0;
+    s.absolute_clause = opt_.absolute_clause != 0;
+    s.best.primal = s.best.dual = s.best.gap = INFINITY;
+    row0 += m;
+    col0 += n;
+    nz0 += static_cast<int32_t>(lp.nnz());
+  }
+  for (size_t i = 0; i < M; ++i) a_ptr[i + 1] += a_ptr[i];
+  {
+    std::vector<int32_t> next(a_ptr.begin(), a_ptr.end() - 1);
+    for (size_t j = 0; j < N; ++j) {
+      for (int32_t p = t_ptr[j]; p < t_ptr[j + 1]; ++p) {
+        const auto slot = static_cast<size_t>(next[static_cast<size_t>(t_idx[static_cast<size_t>(p)])]++);
+        a_idx[slot] = static_cast<int32_t>(j);
+        a_orig[slot] = t_orig[static_cast<size_t>(p)];
+      }
+    }
+  }
+  // SpMV schedule: LDS-staged row blocks unless told otherwise.
+  stage_ = opt_.stage < 0 ? 1 : opt_.stage;
+  for (int32_t k = 0; k < K_; ++k) {
+    InstDesc& d = desc_[static_cast<size_t>(k)];
+    const int32_t cap = opt_.tile_nnz > 0 ? std::min(opt_.tile_nnz, kMaxTileNnz) : 2048;
+    d.ta0 = static_cast<int32_t>(ta.size());
+    build_row_tiles(a_ptr, d.row0, d.rows, k, cap, &ta);
+    d.ta1 = static_cast<int32_t>(ta.size());
+    d.tt0 = static_cast<int32_t>(tt.size());
+    build_row_tiles(t_ptr, d.col0, d.cols, k, cap, &tt);
+    d.tt1 = static_cast<int32_t>(tt.size());
+    d.tc0 = static_cast<int32_t>(tc.size());
+    build_chunks(d.col0, d.cols, k, &tc);
+    d.tc1 = static_cast<int32_t>(tc.size());
+    d.tr0 = static_cast<int32_t>(tr.size());
+    build_chunks(d.row0, d.rows, k, &tr);
+    d.tr1 = static_cast<int32_t>(tr.size());
+  }
+  n_ta_ = static_cast<int32_t>(ta.size());
+  n_tt_ = static_cast<int32_t>(tt.size());
+  n_tc_ = static_cast<int32_t>(tc.size());
+  n_tr_ = static_cast<int32_t>(tr.size());
+
+  int dev_count = 0;
+  if (hipGetDeviceCount(&dev_count) != hipSuccess || dev_count == 0) {
+    set_last_error("no HIP device is available");
+    return kStatusNoDevice;
+  }
+  if (!stream_) SK_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), kStatusInitFailed);
+  if (!host_done_) SK_HIP(hipHostMalloc(reinterpret_cast<void**>(&host_done_), 4 * sizeof(uint32_t)), kStatusOutOfMemory);
+
+  Args<T>& a = args_;
+  int st = 0;
+  int32_t *ap, *ai, *tp, *ti;
+  double *ao, *to;
+#define SK_TRY(expr) \
+  if ((st = (expr)) != 0) return st
+  SK_TRY(put(b_a_ptr_, a_ptr, &ap));
+  SK_TRY(put(b_a_idx_, a_idx, &ai));
+  SK_TRY(put(b_a_orig_, a_orig, &ao));
+  SK_TRY(put(b_t_ptr_, t_ptr, &tp));
+  SK_TRY(put(b_t_idx_, t_idx, &ti));
+  SK_TRY(put(b_t_orig_, t_orig, &to));
+  a.a_ptr = ap;
+  a.a_idx = ai;
+  a.a_orig = ao;
+  a.t_ptr = tp;
+  a.t_idx = ti;
+  a.t_orig = to;
+  SK_TRY(alloc(b_sa_, Z, &d_sa_));
+  SK_TRY(alloc(b_st_, Z, &d_st_));
+  T *av, *tv;
+  if (sizeof(T) == sizeof(double)) {
+    av = reinterpret_cast<T*>(d_sa_);
+    tv = reinterpret_cast<T*>(d_st_);
+  } else {
+    SK_TRY(alloc(b_a_val_, Z, &av));
+    SK_TRY(alloc(b_t_val_, Z, &tv));
+  }
+  a.a_val = av;
+  a.t_val = tv;
+  double *pcm, *pcl, *pcu, *prl, *pru, *pdc, *pdr;
+  SK_TRY(put(b_cmin_, cmin, &pcm));
+  SK_TRY(put(b_cl_, cl, &pcl));
+  SK_TRY(put(b_cu_, cu, &pcu));
+  SK_TRY(put(b_rl_, rl, &prl));
+  SK_TRY(put(b_ru_, ru, &pru));
+  SK_TRY(alloc(b_dc_, N, &pdc));
+  SK_TRY(alloc(b_dr_, M, &pdr));
+  a.cmin = pcm;
+  a.cl = pcl;
+  a.cu = pcu;
+  a.rl = prl;
+  a.ru = pru;
+  a.dc = pdc;
+  a.dr = pdr;
+  SK_TRY(alloc(b_rpass_, M, &d_rpass_));
+  SK_TRY(alloc(b_cpass_, N, &d_cpass_));
+  {
+    std::vector<double> ones_m(M, 1.0), ones_n(N, 1.0);
+    SK_TRY(put(b_scratch64_[0], ones_m, &d_ones_m_));
+    SK_TRY(put(b_scratch64_[1], ones_n, &d_ones_n_));
+  }
+  T *chat, *lhat, *uhat, *rlhat, *ruhat;
+  SK_TRY(alloc(b_chat_, N, &chat));
+  SK_TRY(alloc(b_lhat_, N, &lhat));
+  SK_TRY(alloc(b_uhat_, N, &uhat));
+  SK_TRY(alloc(b_rlhat_, M, &rlhat));
+  SK_TRY(alloc(b_ruhat_, M, &ruhat));
+  a.chat = chat;
+  a.lhat = lhat;
+  a.uhat = uhat;
+  a.rlhat = rlhat;
+  a.ruhat = ruhat;
+  SK_TRY(alloc(b_lhat64_, N, &d_lhat64_));
+  SK_TRY(alloc(b_uhat64_, N, &d_uhat64_));
+  SK_TRY(alloc(b_x0_, N, &a.x0));
+  SK_TRY(alloc(b_x1_, N, &a.x1));
+  SK_TRY(alloc(b_y0_, M, &a.y0));
+  SK_TRY(alloc(b_y1_, M, &a.y1));
+  SK_TRY(alloc(b_aty0_, N, &a.aty0));
+  SK_TRY(alloc(b_aty1_, N, &a.aty1));
+  SK_TRY(alloc(b_xbar_, N, &a.xbar));
+  SK_TRY(alloc(b_xrst_, N, &a.xrst));
+  SK_TRY(alloc(b_yrst_, M, &a.yrst));
+  SK_TRY(alloc(b_xsum_, N, &a.xsum));
+  SK_TRY(alloc(b_ysum_, M, &a.ysum));
+  SK_TRY(alloc(b_xu0_, N, &a.xu0));
+  SK_TRY(alloc(b_xu1_, N, &a.xu1));
+  SK_TRY(alloc(b_yu0_, M, &a.yu0));
+  SK_TRY(alloc(b_yu1_, M, &a.yu1));
+  SK_TRY(alloc(b_bestx_, N, &a.bestx));
+  SK_TRY(alloc(b_besty_, M, &a.besty));
+  SK_TRY(alloc(b_act_, M, &a.activity));
+  SK_TRY(alloc(b_red_, N, &a.reduced));
+  Tile *ptc, *ptr_, *pta, *ptt;
+  SK_TRY(put(b_tc_, tc, &ptc));
+  SK_TRY(put(b_tr_, tr, &ptr_));
+  SK_TRY(put(b_ta_, ta, &pta));
+  SK_TRY(put(b_tt_, tt, &ptt));
+  a.tc = ptc;
+  a.tr = ptr_;
+  a.ta = pta;
+  a.tt = ptt;
+  InstDesc* pdesc;
+  SK_TRY(put(b_desc_, desc_, &pdesc));
+  a.desc = pdesc;
+  SK_TRY(alloc(b_state_, static_cast<size_t>(K_), &a.state));
+  SK_TRY(alloc(b_ps_c_, static_cast<size_t>(n_tc_), &a.ps_c));
+  SK_TRY(alloc(b_ps_a_, static_cast<size_t>(n_ta_), &a.ps_a));
+  SK_TRY(alloc(b_ps_t_, static_cast<size_t>(n_tt_), &a.ps_t));
+  SK_TRY(alloc(b_pe_c_, static_cast<size_t>(n_tc_) * 2, &a.pe_c));
+  SK_TRY(alloc(b_pe_r_, static_cast<size_t>(n_tr_) * 2, &a.pe_r));
+  SK_TRY(alloc(b_pe_a_, static_cast<size_t>(n_ta_) * 8, &a.pe_a));
+  SK_TRY(alloc(b_pe_t_, static_cast<size_t>(n_tt_) * 8, &a.pe_t));
+  SK_TRY(alloc(b_arrive_step_, static_cast<size_t>(K_), &a.arrive_step));
+  SK_TRY(alloc(b_arrive_eval_, static_cast<size_t>(K_), &a.arrive_eval));
+  SK_TRY(alloc(b_done_, 1, &a.done_count));
+  SK_TRY(put(b_v_, v0_, &d_v_));
+  SK_TRY(alloc(b_av_, M, &d_av_));
+  SK_TRY(alloc(b_len_, static_cast<size_t>(K_), &d_len_));
+  SK_TRY(alloc(b_zero_, static_cast<size_t>(K_), &d_zero_seen_));
+  a.n_tc = n_tc_;
+  a.force = 0;
+  a.report = 0;
+#undef SK_TRY
+  return kStatusSuccess;
+}
+
+template <class T>
+int EngineImpl<T>::setup_device() {
+  Args<T>& a = args_;
+  hipStream_t s = stream_;
+  const auto M = static_cast<int32_t>(m_), N = static_cast<int32_t>(n_);
+  const auto Z = static_cast<size_t>(nnz_);
+  SK_HIP(hipMemcpyAsync(d_sa_, a.a_orig, Z * sizeof(double), hipMemcpyDeviceToDevice, s), kStatusCopyFailed);
+  SK_HIP(hipMemcpyAsync(d_st_, a.t_orig, Z * sizeof(double), hipMemcpyDeviceToDevice, s), kStatusCopyFailed);
+  SK_HIP(hipMemcpyAsync(const_cast<double*>(a.dr), d_ones_m_, static_cast<size_t>(M) * sizeof(double), hipMemcpyDeviceToDevice, s), kStatusCopyFailed);
+  SK_HIP(hipMemcpyAsync(const_cast<double*>(a.dc), d_ones_n_, static_cast<size_t>(N) * sizeof(double), hipMemcpyDeviceToDevice, s), kStatusCopyFailed);
+  // 10 Ruiz rounds, then one Pock-Chambolle round (scaling.cpp:121-122)
+  for (int pass = 0; pass < 11; ++pass) {
+    const int use_sum = pass == 10;
+    SK_LAUNCH(k_row_pass<<<blocks_for(M), kBlock, 0, s>>>(a.a_ptr, d_sa_, M, use_sum, d_rpass_));
+    SK_LAUNCH(k_row_pass<<<blocks_for(N), kBlock, 0, s>>>(a.t_ptr, d_st_, N, use_sum, d_cpass_));
+    SK_LAUNCH(k_apply_pass<<<blocks_for(M), kBlock, 0, s>>>(a.a_ptr, a.a_idx, d_sa_, M, 1, d_rpass_, d_cpass_));
+    SK_LAUNCH(k_apply_pass<<<blocks_for(N), kBlock, 0, s>>>(a.t_ptr, a.t_idx, d_st_, N, 0, d_rpass_, d_cpass_));
+    SK_LAUNCH(k_scale_mul<<<blocks_for(M), kBlock, 0, s>>>(const_cast<double*>(a.dr), d_rpass_, M));
+    SK_LAUNCH(k_scale_mul<<<blocks_for(N), kBlock, 0, s>>>(const_cast<double*>(a.dc), d_cpass_, N));
+  }
+  const int32_t MN = std::max(M, N);
+  SK_LAUNCH(k_scaled_vectors<T><<<blocks_for(MN), kBlock, 0, s>>>(
+      a, const_cast<T*>(a.chat), const_cast<T*>(a.lhat), const_cast<T*>(a.uhat),
+      const_cast<T*>(a.rlhat), const_cast<T*>(a.ruhat), N, M));
+  {
+    // FP64 scaled column bounds for the reference's initial `best` point
+    Args<double> a64{};
+    a64.dc = a.dc;
+    a64.dr = a.dr;
+    a64.cmin = a.cmin;
+    a64.cl = a.cl;
+    a64.cu = a.cu;
+    a64.rl = a.rl;
+    a64.ru = a.ru;
+    SK_LAUNCH(k_scaled_vectors<double><<<blocks_for(N), kBlock, 0, s>>>(
+        a64, a.xu0, d_lhat64_, d_uhat64_, a.yu0, a.yu1, N, 0));
+  }
+  if (sizeof(T) != sizeof(double)) {
+    SK_LAUNCH(k_convert<T><<<blocks_for(static_cast<int64_t>(Z)), kBlock, 0, s>>>(d_sa_, const_cast<T*>(a.a_val), static_cast<int64_t>(Z)));
+    SK_LAUNCH(k_convert<T><<<blocks_for(static_cast<int64_t>(Z)), kBlock, 0, s>>>(d_st_, const_cast<T*>(a.t_val), static_cast<int64_t>(Z)));
+  }
+  // power iteration on A^T A, 30 steps (scaling.cpp:161-194)
+  SK_HIP(hipMemcpyAsync(d_v_, v0_.data(), static_cast<size_t>(N) * sizeof(double), hipMemcpyHostToDevice, s), kStatusCopyFailed);
+  SK_HIP(hipMemsetAsync(d_zero_seen_, 0, static_cast<size_t>(K_), s), kStatusCopyFailed);
+  std::vector<double> lens(static_cast<size_t>(K_) * 31);
+  for (int step = 0; step <= 30; ++step) {
+    SK_LAUNCH(k_chunk_sumsq<<<n_tc_, kBlock, 0, s>>>(a.tc, d_v_, a.ps_c));
+    SK_LAUNCH(k_inst_norm<<<K_, kBlock, 0, s>>>(a.desc, a.ps_c, d_len_));
+    SK_HIP(hipMemcpyAsync(lens.data() + static_cast<size_t>(step) * K_, d_len_, static_cast<size_t>(K_) * sizeof(double), hipMemcpyDeviceToHost, s), kStatusCopyFailed);
+    if (step == 30) break;
+    SK_LAUNCH(k_chunk_div<<<n_tc_, kBlock, 0, s>>>(a.tc, d_len_, d_v_));
+    SK_LAUNCH(k_spmv_serial<<<blocks_for(M), kBlock, 0, s>>>(a.a_ptr, a.a_idx, d_sa_, d_v_, d_av_, M));
+    SK_LAUNCH(k_spmv_serial<<<blocks_for(N), kBlock, 0, s>>>(a.t_ptr, a.t_idx, d_st_, d_av_, d_v_, N));
+  }
+  SK_HIP(hipStreamSynchronize(s), kStatusSyncFailed);
+  std::vector<InstState> st = init_state_;
+  for (int32_t k = 0; k < K_; ++k) {
+    double norm = 1.0;
+    if (!trivial_norm_[static_cast<size_t>(k)]) {
+      bool zero = false;
+      for (int step = 0; step < 30; ++step) zero = zero || lens[static_cast<size_t>(step) * K_ + k] == 0.0;
+      // lens[30] = ||A^T A v||, so sqrt of it is ||A^T A v||^(1/2)
+      const double r = std::sqrt(lens[static_cast<size_t>(30) * K_ + k]);
+      norm = zero ? 1.0 : (r > 0.0 ? r * 1.01 : 1.0);
+    }
+    InstState& x = st[static_cast<size_t>(k)];
+    x.spectral_norm = norm;
+    x.eta = norm > 0.0 ? 1.0 / norm : 1.0;
+    x.eta_ceiling = 1.0e3 / std::max(norm, 1e-12);
+    if (x.iteration_limit <= 0) {
+      x.done = 1;
+      x.status = kStIterationLimit;
+      x.report_pending = 1;
+    }
+  }
+  uint32_t initially_done = 0;
+  for (const auto& x : st) initially_done += static_cast<uint32_t>(x.done);
+  SK_HIP(hipMemcpyAsync(a.state, st.data(), st.size() * sizeof(InstState), hipMemcpyHostToDevice, s), kStatusCopyFailed);
+  SK_HIP(hipMemsetAsync(a.arrive_step, 0, static_cast<size_t>(K_) * sizeof(uint32_t), s), kStatusCopyFailed);
+  SK_HIP(hipMemsetAsync(a.arrive_eval, 0, static_cast<size_t>(K_) * sizeof(uint32_t), s), kStatusCopyFailed);
+  host_done_[2] = initially_done;
+  SK_HIP(hipMemcpyAsync(a.done_count, &host_done_[2], sizeof(uint32_t), hipMemcpyHostToDevice, s), kStatusCopyFailed);
+  SK_LAUNCH(k_init_iterates<T><<<blocks_for(MN), kBlock, 0, s>>>(a, d_lhat64_, d_uhat64_, N, M));
+  SK_HIP(hipStreamSynchronize(s), kStatusSyncFailed);
+  return kStatusSuccess;
+}
+
+template <class T>
+int EngineImpl<T>::enqueue_eval(int report_mode) {
+  Args<T> a = args_;
+  a.report = report_mode;
+  hipStream_t s = stream_;
+  if (!report_mode) SK_LAUNCH(k_eval_prepare<T><<<n_tc_ + n_tr_, kBlock, 0, s>>>(a));
+  SK_LAUNCH(k_eval_rows<T><<<n_ta_, kBlock, 0, s>>>(a));
+  SK_LAUNCH(k_eval_cols<T><<<n_tt_, kBlock, 0, s>>>(a));
+  if (!report_mode) SK_LAUNCH(k_eval_commit<T><<<n_tc_ + n_tr_, kBlock, 0, s>>>(a));
+  return kStatusSuccess;
+}
+
+template <class T>
+int EngineImpl<T>::enqueue_block() {
+  const Args<T>& a = args_;
+  hipStream_t s = stream_;
+  for (int k = 0; k < opt_.steps_per_block; ++k) {
+    SK_LAUNCH(k_step_primal<T><<<n_tc_, kBlock, 0, s>>>(a));
+    if (stage_) {
+      SK_LAUNCH(k_step_dual<T, true><<<n_ta_, kBlock, 0, s>>>(a));
+      SK_LAUNCH(k_step_aty<T, true><<<n_tt_, kBlock, 0, s>>>(a));
+    } else {
+      SK_LAUNCH(k_step_dual<T, false><<<n_ta_, kBlock, 0, s>>>(a));
+      SK_LAUNCH(k_step_aty<T, false><<<n_tt_, kBlock, 0, s>>>(a));
+    }
+  }
+  return enqueue_eval(0);
+}
+
+template <class T>
+int EngineImpl<T>::run_loop() {
+  hipStream_t s = stream_;
+  int st = 0;
+  if (opt_.use_graph && !graph_exec_) {
+    hipGraph_t graph = nullptr;
+    SK_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), kStatusLaunchFailed);
+    st = enqueue_block();
+    hipGraph_t captured = nullptr;
+    const hipError_t e = hipStreamEndCapture(s, &captured);
+    if (st) return st;
+    if (e != hipSuccess) {
+      set_last_error(std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+      return kStatusLaunchFailed;
+    }
+    graph = captured;
+    const hipError_t ei = hipGraphInstantiate(&graph_exec_, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (ei != hipSuccess) {
+      set_last_error(std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
+      graph_exec_ = nullptr;
+      return kStatusLaunchFailed;
+    }
+  }
+  const auto t0 = std::chrono::steady_clock::now();
+  const uint32_t target = static_cast<uint32_t>(K_);
+  int64_t launched = 0;
+  hipEvent_t ev[2];
+  SK_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming), kStatusInitFailed);
+  SK_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming), kStatusInitFailed);
+  bool finished = host_done_[2] >= target;
+  bool timed_out = false;
+  // Block k copies the finished-instance counter into ring slot k % 2 and records event k % 2.
+  // Block k + 1 is launched before block k is waited on, so the device never idles on the
+  // host's read-back.
+  auto launch = [&]() -> int {
+    if (graph_exec_) {
+      SK_HIP(hipGraphLaunch(graph_exec_, s), kStatusLaunchFailed);
+    } else {
+      const int r = enqueue_block();
+      if (r) return r;
+    }
+    SK_HIP(hipMemcpyAsync(&host_done_[launched & 1], args_.done_count, sizeof(uint32_t),
+                          hipMemcpyDeviceToHost, s), kStatusCopyFailed);
+    SK_HIP(hipEventRecord(ev[launched & 1], s), kStatusLaunchFailed);
+    ++launched;
+    return kStatusSuccess;
+  };
+  if (!finished) st = launch();
+  for (int64_t k = 0; !finished && !st; ++k) {
+    if ((st = launch())) break;  // block k + 1
+    const hipError_t e = hipEventSynchronize(ev[k & 1]);
+    if (e != hipSuccess) {
+      set_last_error(std::string("hipEventSynchronize: ") + hipGetErrorString(e));
+      st = kStatusSyncFailed;
+      break;
+    }
+    finished = host_done_[k & 1] >= target;
+    const double elapsed = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    if (!finished && elapsed > opt_.time_limit) {
+      timed_out = true;
+      break;
+    }
+  }
+  if (!st) {
+    const hipError_t e = hipStreamSynchronize(s);
+    if (e != hipSuccess) {
+      set_last_error(std::string("hipStreamSynchronize: ") + hipGetErrorString(e));
+      st = kStatusSyncFailed;
+    }
+  }
+  (void)hipEventDestroy(ev[0]);
+  (void)hipEventDestroy(ev[1]);
+  timings_.graph_launches = launched;
+  if (st) return st;
+  if (timed_out) {
+    // Mark every unfinished instance as stopped by the time limit (host side, then upload).
+    std::vector<InstState> cur(static_cast<size_t>(K_));
+    SK_HIP(hipMemcpy(cur.data(), args_.state, cur.size() * sizeof(InstState), hipMemcpyDeviceToHost), kStatusCopyFailed);
+    for (auto& x : cur) {
+      if (!x.done) {
+        x.done = 1;
+        x.status = kStTimeLimit;
+        x.report_pending = 1;
+        x.eval_pending = 0;
+      }
+    }
+    SK_HIP(hipMemcpy(args_.state, cur.data(), cur.size() * sizeof(InstState), hipMemcpyHostToDevice), kStatusCopyFailed);
+  }
+  return kStatusSuccess;
+}
+
+template <class T>
+int EngineImpl<T>::report() {
+  int st = enqueue_eval(1);
+  if (st) return st;
+  hipStream_t s = stream_;
+  final_state_.resize(static_cast<size_t>(K_));
+  bestx_.resize(static_cast<size_t>(n_));
+  besty_.resize(static_cast<size_t>(m_));
+  act_.resize(static_cast<size_t>(m_));
+  red_.resize(static_cast<size_t>(n_));
+  SK_HIP(hipMemcpyAsync(final_state_.data(), args_.state, final_state_.size() * sizeof(InstState), hipMemcpyDeviceToHost, s), kStatusCopyFailed);
+  SK_HIP(hipMemcpyAsync(bestx_.data(), args_.bestx, bestx_.size() * sizeof(double), hipMemcpyDeviceToHost, s), kStatusCopyFailed);
+  SK_HIP(hipMemcpyAsync(besty_.data(), args_.besty, besty_.size() * sizeof(double), hipMemcpyDeviceToHost, s), kStatusCopyFailed);
+  SK_HIP(hipMemcpyAsync(act_.data(), args_.activity, act_.size() * sizeof(double), hipMemcpyDeviceToHost, s), kStatusCopyFailed);
+  SK_HIP(hipMemcpyAsync(red_.data(), args_.reduced, red_.size() * sizeof(double), hipMemcpyDeviceToHost, s), kStatusCopyFailed);
+  SK_HIP(hipStreamSynchronize(s), kStatusSyncFailed);
+  return kStatusSuccess;
+}
+
+template <class T>
+int EngineImpl<T>::solve() {
+  if (!stream_) {
+    set_last_error("solve() before upload()");
+    return kStatusInvalidState;
+  }
+  using clock = std::chrono::steady_clock;
+  const auto t0 = clock::now();
+  int st = setup_device();
+  if (st) return st;
+  const auto t1 = clock::now();
+  st = run_loop();
+  if (st) return st;
+  const auto t2 = clock::now();
+  st = report();
+  if (st) return st;
+  const auto t3 = clock::now();
+  timings_.setup_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+  timings_.loop_ms = std::chrono::duration<double, std::milli>(t2 - t1).count();
+  timings_.report_ms = std::chrono::duration<double, std::milli>(t3 - t2).count();
+  return kStatusSuccess;
+}
+
+template <class T>
+int EngineImpl<T>::result(int32_t index, InstanceResult* out) const {
+  if (index < 0 || index >= K_ || final_state_.empty()) {
+    set_last_error("result(): no such solved instance");
+    return kStatusInvalidArgument;
+  }
+  const InstState& s = final_state_[static_cast<size_t>(index)];
+  const InstDesc& d = desc_[static_cast<size_t>(index)];
+  const double sense = static_cast<double>(sense_[static_cast<size_t>(index)]);
+  const HostLp* lp = lps_[static_cast<size_t>(index)];
+  out->status = s.status;
+  out->converged = s.converged;
+  out->iterations = s.iteration;
+  out->restarts = s.restarts;
+  out->evaluations = s.evaluations;
+  out->attempts = s.attempts;
+  out->spectral_norm = s.spectral_norm;
+  out->omega = s.omega;
+  out->eta = s.eta;
+  out->final_residuals = s.report;
+  // pdhg.cpp:647-716: kOptimal only for a converged point meeting the project standard
+  const Residuals& f = s.report;
+  const bool standard = f.absolute_primal <= 1e-7 && f.absolute_dual <= 1e-7 &&
+                        f.gap_as_verified <= 1e-9 && f.complementarity <= 1e-6;
+  const bool verifiable = s.converged && standard;
+  if (s.converged) out->status = verifiable ? kStOptimal : kStFeasible;
+  out->dual_bound = verifiable ? sense * f.dual_objective + (lp ? lp->objective_offset : 0.0)
+                               : (sense < 0 ? INFINITY : -INFINITY);
+  const auto c0 = static_cast<size_t>(d.col0), r0 = static_cast<size_t>(d.row0);
+  out->col_value.assign(bestx_.begin() + static_cast<std::ptrdiff_t>(c0), bestx_.begin() + static_cast<std::ptrdiff_t>(c0 + static_cast<size_t>(d.cols)));
+  out->col_dual.resize(static_cast<size_t>(d.cols));
+  for (int32_t j = 0; j < d.cols; ++j) out->col_dual[static_cast<size_t>(j)] = sense * red_[c0 + static_cast<size_t>(j)];
+  out->row_dual.resize(static_cast<size_t>(d.rows));
+  for (int32_t i = 0; i < d.rows; ++i) out->row_dual[static_cast<size_t>(i)] = sense * (-besty_[r0 + static_cast<size_t>(i)]);
+  out->row_activity.assign(act_.begin() + static_cast<std::ptrdiff_t>(r0), act_.begin() + static_cast<std::ptrdiff_t>(r0 + static_cast<size_t>(d.rows)));
+  return kStatusSuccess;
+}
+
+template <class T>
+int EngineImpl<T>::time_kernel(int kernel, int reps, double* mean_ms) {
+  if (!stream_ || reps <= 0) {
+    set_last_error("time_kernel(): engine not uploaded or reps <= 0");
+    return kStatusInvalidArgument;
+  }
+  Args<T> a = args_;
+  a.force = 1;
+  hipStream_t s = stream_;
+  hipEvent_t e0, e1;
+  SK_HIP(hipEventCreate(&e0), kStatusInitFailed);
+  SK_HIP(hipEventCreate(&e1), kStatusInitFailed);
+  auto one = [&]() -> int {
+    if (kernel == 0) SK_LAUNCH(k_step_primal<T><<<n_tc_, kBlock, 0, s>>>(a));
+    else if (kernel == 1 && stage_) SK_LAUNCH(k_step_dual<T, true><<<n_ta_, kBlock, 0, s>>>(a));
+    else if (kernel == 1) SK_LAUNCH(k_step_dual<T, false><<<n_ta_, kBlock, 0, s>>>(a));
+    else if (stage_) SK_LAUNCH(k_step_aty<T, true><<<n_tt_, kBlock, 0, s>>>(a));
+    else SK_LAUNCH(k_step_aty<T, false><<<n_tt_, kBlock, 0, s>>>(a));
+    return kStatusSuccess;
+  };
+  int st = one();  // warm
+  if (st) return st;
+  SK_HIP(hipEventRecord(e0, s), kStatusLaunchFailed);
+  for (int r = 0; r < reps; ++r)
+    if ((st = one())) return st;
+  SK_HIP(hipEventRecord(e1, s), kStatusLaunchFailed);
+  SK_HIP(hipEventSynchronize(e1), kStatusSyncFailed);
+  float ms = 0.0f;
+  SK_HIP(hipEventElapsedTime(&ms, e0, e1), kStatusSyncFailed);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  *mean_ms = static_cast<double>(ms) / reps;
+  return kStatusSuccess;
+}
+
+// Algorithmic bytes per launch (each array element counted once):
+//   0 primal: per column read x, c, l, u, A^T y (5 words), write x', xbar (2 words),
+//             read-modify-write the FP64 running sum (16 B)
+//   1 dual:   CSR(A): (w + 4) B per nonzero + 4 (m + 1) row pointers + w n for xbar;
+//             per row read y, rl, ru (3 words), write y' (1 word), sum RMW (16 B)
+//   2 A^T:    CSR(A^T): (w + 4) B per nonzero + 4 (n + 1) + w m for y';
+//             per column read x, x', A^T y (3 words), write A^T y' (1 word)
+template <class T>
+double EngineImpl<T>::kernel_bytes(int kernel) const {
+  const double w = sizeof(T), z = static_cast<double>(nnz_);
+  const double m = static_cast<double>(m_), n = static_cast<double>(n_);
+  if (kernel == 0) return n * (7.0 * w + 16.0);
+  if (kernel == 1) return (w + 4.0) * z + 4.0 * (m + 1.0) + w * n + m * (4.0 * w + 16.0);
+  return (w + 4.0) * z + 4.0 * (n + 1.0) + w * m + n * 4.0 * w;
+}
+
+}  // namespace
+
+void set_last_error(const std::string& message) { g_last_error = message; }
+const char* last_error() { return g_last_error.c_str(); }
+
+std::unique_ptr<Engine> make_engine(const EngineOptions& options) {
+  if (options.precision == 1) return std::make_unique<EngineImpl<double>>(options);
+  return std::make_unique<EngineImpl<float>>(options);
+}
+
+}  // namespace sk

@@ -1,0 +1,84 @@
+// Host driver of the batched restarted-PDHG engine on one HIP device.
+//
+// A batch is any number of LPs ("instances") laid out block-diagonally in device memory. The
+// solve runs entirely on the device: scaling, power iteration, the PDHG loop (captured as a
+// hipGraph of `steps_per_block` iterations plus one evaluation stage) and the final report.
+// The host only reads a finished-instance counter between graph launches.
+#pragma once
+
+#include <cfloat>
+#include <cstdint>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "device_types.hpp"
+#include "lp.hpp"
+
+namespace sk {
+
+struct EngineOptions {
+  double tolerance = 1e-4;        // pdhg_tolerance
+  int64_t iteration_limit = -1;   // -1: 1,000,000 (pdhg.cpp:312-313)
+  double time_limit = DBL_MAX;    // seconds
+  int32_t restart = 1;            // pdhg_restart
+  int32_t stop_at_request = 0;    // pdhg_stop_at_request
+  int32_t absolute_clause = 1;    // meets_request also needs absolute primal <= 1e-7
+  int64_t random_seed = 0;        // power iteration uses std::mt19937(seed + 1)
+  int32_t precision = 0;          // 0: FP32 iterates + FP64 evaluation; 1: FP64 throughout
+  int32_t steps_per_block = 40;   // PDHG steps per captured graph
+  int32_t use_graph = 1;
+  int32_t stage = -1;             // SpMV schedule: -1 autotune, 0 direct, 1 LDS-staged
+  int32_t tile_nnz = 0;           // row-block capacity; 0 picks per instance
+};
+
+struct InstanceResult {
+  int32_t status = 0;
+  int32_t converged = 0;
+  int64_t iterations = 0;
+  int64_t restarts = 0;
+  int64_t evaluations = 0;
+  int64_t attempts = 0;
+  double spectral_norm = 0.0;
+  double omega = 0.0;
+  double eta = 0.0;
+  Residuals final_residuals{};
+  double dual_bound = 0.0;
+  std::vector<double> col_value, col_dual, row_dual, row_activity;
+};
+
+struct EngineTimings {
+  double setup_ms = 0.0;   // device scaling + power iteration + initial state
+  double loop_ms = 0.0;    // PDHG graph launches until every instance stopped
+  double report_ms = 0.0;  // final evaluation and download
+  int64_t graph_launches = 0;
+};
+
+class Engine {
+ public:
+  virtual ~Engine() = default;
+  // Instances are borrowed: they must stay alive until upload() returns.
+  virtual int32_t add(const HostLp* lp) = 0;
+  virtual int upload() = 0;  // host CSR/tiles + device allocation + H2D (not part of a solve)
+  virtual int solve() = 0;   // everything on the device, from the uploaded originals
+  virtual int result(int32_t index, InstanceResult* out) const = 0;
+  virtual const EngineTimings& timings() const = 0;
+  virtual int32_t size() const = 0;
+  // Launch one of the step kernels `reps` times in timing mode between two HIP events on the
+  // engine's stream; returns the mean duration in ms. kernel: 0 primal, 1 dual (A), 2 A^T.
+  virtual int time_kernel(int kernel, int reps, double* mean_ms) = 0;
+  // Algorithmic bytes of one launch of that kernel over the whole batch.
+  virtual double kernel_bytes(int kernel) const = 0;
+  virtual int64_t total_nnz() const = 0;
+  virtual int64_t total_rows() const = 0;
+  virtual int64_t total_cols() const = 0;
+  virtual int stage_choice() const = 0;
+};
+
+std::unique_ptr<Engine> make_engine(const EngineOptions& options);
+
+// Thread-local description of the last failure inside the library.
+void set_last_error(const std::string& message);
+const char* last_error();
+
+}  // namespace sk

@@ -1,0 +1,728 @@
+// HIP kernels of the batched restarted-PDHG engine for CDNA4 (gfx950).
+//
+// One iteration of the reference loop (src/pdhg/pdhg.cpp:383-496) is three launches:
+//   k_step_primal  columns:   x' = proj(x - tau (c + A^T y)), xbar = 2x' - x       (:392-400)
+//   k_step_dual    rows of A: y' = prox(y + sigma A xbar)                           (:402-408)
+//   k_step_aty     rows of A^T: A^T y', interaction (x'-x)^T (A^T y' - A^T y), and the
+//                  adaptive step-size decision by the last tile of each instance   (:410-496)
+// The carried A^T y replaces the reference's third product A (x'-x) (pdhg.cpp:432):
+// (y'-y)^T A (x'-x) = (x'-x)^T (A^T y' - A^T y), and A^T y' is the next primal step's input.
+//
+// Every 40 accepted iterations (or on a step without information) an instance runs the
+// FP64 evaluation stage (pdhg.cpp:502-643):
+//   k_eval_prepare  unscale current and average candidates; ||x - x_restart||, ||y - y_restart||
+//   k_eval_rows     A x per candidate on the unscaled A (FP64): primal residual, row terms
+//   k_eval_cols     c + A^T y per candidate (FP64): dual residual, objectives; the last tile
+//                   of an instance picks the candidate, tests termination and restarts
+//   k_eval_commit   copy the chosen candidate into `best`, reset sums on restart
+//
+// Reductions are deterministic: per-tile partials are reduced in a fixed order by the last
+// tile of the instance, so an instance gives bitwise the same trajectory alone or in a batch.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+
+#include "device_types.hpp"
+#include "device_util.hpp"
+
+namespace sk {
+
+// ---- kernel arguments ----------------------------------------------------------------------
+
+// Everything a kernel of the engine may touch, passed by value. T is the iterate precision
+// (float for the FP32/FP64 split engine, double for the plain FP64 engine).
+template <class T>
+struct Args {
+  // CSR(A-hat) and CSR(A-hat^T) in the iterate precision (the scaled matrix)
+  const int32_t* __restrict__ a_ptr;
+  const int32_t* __restrict__ a_idx;
+  const T* __restrict__ a_val;
+  const int32_t* __restrict__ t_ptr;
+  const int32_t* __restrict__ t_idx;
+  const T* __restrict__ t_val;
+  // the original A in the same two layouts, FP64 (evaluation)
+  const double* __restrict__ a_orig;
+  const double* __restrict__ t_orig;
+  // scaled problem data in the iterate precision; absent bounds are +-inf
+  const T* __restrict__ chat;
+  const T* __restrict__ lhat;
+  const T* __restrict__ uhat;
+  const T* __restrict__ rlhat;
+  const T* __restrict__ ruhat;
+  // original problem data, FP64
+  const double* __restrict__ cmin;  // sense * cost
+  const double* __restrict__ cl;
+  const double* __restrict__ cu;
+  const double* __restrict__ rl;
+  const double* __restrict__ ru;
+  const double* __restrict__ dc;  // column scaling D_c
+  const double* __restrict__ dr;  // row scaling D_r
+  // iterates (scaled), double-buffered by instance parity
+  T* x0;
+  T* x1;
+  T* y0;
+  T* y1;
+  T* aty0;
+  T* aty1;
+  T* xbar;
+  T* xrst;  // restart point
+  T* yrst;
+  double* xsum;  // running sums since the last restart
+  double* ysum;
+  // unscaled candidates: *u0 current, *u1 running average; best; report outputs
+  double* xu0;
+  double* xu1;
+  double* yu0;
+  double* yu1;
+  double* bestx;
+  double* besty;
+  double* activity;
+  double* reduced;
+  // tile tables and per-instance records
+  const Tile* __restrict__ tc;  // column chunks
+  const Tile* __restrict__ tr;  // row chunks
+  const Tile* __restrict__ ta;  // row blocks of CSR(A)
+  const Tile* __restrict__ tt;  // row blocks of CSR(A^T)
+  const InstDesc* __restrict__ desc;
+  InstState* state;
+  // per-tile partials
+  double* ps_c;  // step: ||x'-x||^2 per column chunk
+  double* ps_a;  // step: ||y'-y||^2 per CSR(A) block
+  double* ps_t;  // step: interaction per CSR(A^T) block
+  double* pe_c;  // eval: 2 per column chunk
+  double* pe_r;  // eval: 2 per row chunk
+  double* pe_a;  // eval: 8 per CSR(A) block
+  double* pe_t;  // eval: 8 per CSR(A^T) block
+  uint32_t* arrive_step;
+  uint32_t* arrive_eval;
+  uint32_t* done_count;
+  int32_t n_tc;    // number of column-chunk tiles (eval kernels cover tc then tr)
+  int32_t force;   // timing mode: every instance steps, nothing is decided
+  int32_t report;  // evaluation kernels: 0 in-loop candidates, 1 final report of best
+};
+
+__device__ __forceinline__ bool stepping(const InstState& s) { return !s.done && !s.eval_pending; }
+
+// ---- step kernels ---------------------------------------------------------------------------
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_step_primal(Args<T> a) {
+  __shared__ double red[kBlock / kWave];
+  const Tile t = a.tc[blockIdx.x];
+  const InstState& s = a.state[t.inst];
+  if (!a.force && !stepping(s)) return;
+  const int p = s.parity;
+  const bool acc = a.force || s.pending_accept;
+  const T tau = static_cast<T>(s.eta / s.omega);
+  const T* __restrict__ x = p ? a.x1 : a.x0;
+  T* __restrict__ xn = p ? a.x0 : a.x1;
+  const T* __restrict__ aty = p ? a.aty1 : a.aty0;
+  double d2 = 0.0;
+  for (int j = t.begin + threadIdx.x; j < t.end; j += kBlock) {
+    const T xj = x[j];
+    if (acc) a.xsum[j] += static_cast<double>(xj);
+    const T g = a.chat[j] + aty[j];
+    const T v = dev_project<T>(xj - tau * g, a.lhat[j], a.uhat[j]);
+    xn[j] = v;
+    a.xbar[j] = static_cast<T>(2) * v - xj;
+    const double d = static_cast<double>(v - xj);
+    d2 += d * d;
+  }
+  d2 = block_sum(d2, red);
+  if (threadIdx.x == 0) a.ps_c[blockIdx.x] = d2;
+}
+
+template <class T, bool STAGE>
+__global__ __launch_bounds__(kBlock) void k_step_dual(Args<T> a) {
+  __shared__ T stage[STAGE ? kMaxTileNnz : 1];
+  __shared__ T red_t[kBlock / kWave];
+  __shared__ double red[kBlock / kWave];
+  const Tile t = a.ta[blockIdx.x];
+  const InstState& s = a.state[t.inst];
+  if (!a.force && !stepping(s)) return;
+  const int p = s.parity;
+  const bool acc = a.force || s.pending_accept;
+  const T sigma = static_cast<T>(s.eta * s.omega);
+  const T* __restrict__ y = p ? a.y1 : a.y0;
+  T* __restrict__ yn = p ? a.y0 : a.y1;
+  const T* __restrict__ xbar = a.xbar;
+  const T* __restrict__ val = a.a_val;
+  const int32_t* __restrict__ idx = a.a_idx;
+  double d2 = 0.0;
+  rowblock_product<T, STAGE>(
+      t, a.a_ptr, [&](int q) { return val[q] * xbar[idx[q]]; },
+      [&](int i, T r) {
+        const T yi = y[i];
+        if (acc) a.ysum[i] += static_cast<double>(yi);
+        const T v = yi + sigma * r;
+        const T ynew = v - sigma * dev_project<T>(v / sigma, a.rlhat[i], a.ruhat[i]);
+        yn[i] = ynew;
+        const double d = static_cast<double>(ynew - yi);
+        d2 += d * d;
+      },
+      stage, red_t);
+  d2 = block_sum(d2, red);
+  if (threadIdx.x == 0) a.ps_a[blockIdx.x] = d2;
+}
+
+// Reduce `count` per-tile partials starting at tile `base` with the whole workgroup in a
+// fixed order (result valid in every thread).
+__device__ __forceinline__ double tile_sum(const double* part, int base, int count, int stride,
+                                           int slot, double* red) {
+  double v = 0.0;
+  for (int k = threadIdx.x; k < count; k += kBlock) v += part[(size_t)(base + k) * stride + slot];
+  return block_sum(v, red);
+}
+__device__ __forceinline__ double tile_max(const double* part, int base, int count, int stride,
+                                           int slot, double* red) {
+  double v = 0.0;
+  for (int k = threadIdx.x; k < count; k += kBlock)
+    v = dev_max(v, part[(size_t)(base + k) * stride + slot]);
+  return block_max(v, red);
+}
+
+// Arrival counter of the "last tile finalises" pattern: returns true in every thread of the
+// workgroup that is the last of the instance's `tiles` to arrive.
+__device__ __forceinline__ bool arrive_last(uint32_t* counter, int tiles, int* lds_flag) {
+  __threadfence();
+  if (threadIdx.x == 0) {
+    const uint32_t prev = atomicAdd(counter, 1u);
+    *lds_flag = prev == static_cast<uint32_t>(tiles - 1);
+  }
+  __syncthreads();
+  const bool last = *lds_flag != 0;
+  if (last) __threadfence();
+  return last;
+}
+
+__device__ __forceinline__ void finish(InstState& s, int status, uint32_t* done_count) {
+  s.done = 1;
+  s.status = status;
+  s.report_pending = 1;
+  atomicAdd(done_count, 1u);
+}
+
+// The adaptive step-size rule and acceptance (pdhg.cpp:410-503), applied by the last tile.
+__device__ void decide_step(InstState& s, double dx2, double dy2, double inter_sum, int has_rows,
+                            uint32_t* done_count) {
+  const double omega = s.omega;
+  const double movement = 0.5 * omega * dx2 + 0.5 * dy2 / omega;
+  const double interaction = has_rows ? fabs(inter_sum) : 0.0;
+  s.movement = movement;
+  s.interaction = interaction;
+  s.attempts += 1;
+  s.copy_best = 0;
+  s.do_restart = 0;
+  if (!isfinite(movement) || !isfinite(interaction)) {
+    s.nonfinite = 1;
+    finish(s, kStNumericalError, done_count);
+    return;
+  }
+  const bool no_information = interaction <= 0.0;
+  const double limit = no_information ? INFINITY : movement / interaction;
+  const double exponent = static_cast<double>(s.iteration + 1 > 2 ? s.iteration + 1 : 2);
+  const double shrink = 1.0 - pow(exponent, -0.3);
+  const double grow = 1.0 + pow(exponent, -0.6);
+  const double proposed = dev_min(shrink * limit, grow * s.eta);
+  if (s.eta <= limit) {
+    s.parity ^= 1;
+    s.averaged += 1;
+    s.iteration += 1;
+    s.pending_accept = 1;
+  } else {
+    s.pending_accept = 0;
+  }
+  if (!no_information) {
+    s.eta = proposed < 1e-12 ? 1e-12 : (s.eta_ceiling < proposed ? s.eta_ceiling : proposed);
+  }
+  if (s.iteration != 0 && (s.iteration % 40 == 0 || no_information)) {
+    s.eval_pending = 1;
+  } else if (s.iteration >= s.iteration_limit) {
+    finish(s, kStIterationLimit, done_count);
+  }
+}
+
+template <class T, bool STAGE>
+__global__ __launch_bounds__(kBlock) void k_step_aty(Args<T> a) {
+  __shared__ T stage[STAGE ? kMaxTileNnz : 1];
+  __shared__ T red_t[kBlock / kWave];
+  __shared__ double red[kBlock / kWave];
+  __shared__ int last_flag;
+  const Tile t = a.tt[blockIdx.x];
+  InstState& s = a.state[t.inst];
+  if (!a.force && !stepping(s)) return;
+  const int p = s.parity;
+  const T* __restrict__ x = p ? a.x1 : a.x0;
+  const T* __restrict__ xn = p ? a.x0 : a.x1;
+  const T* __restrict__ yn = p ? a.y0 : a.y1;
+  const T* __restrict__ aty = p ? a.aty1 : a.aty0;
+  T* __restrict__ atyn = p ? a.aty0 : a.aty1;
+  const T* __restrict__ val = a.t_val;
+  const int32_t* __restrict__ idx = a.t_idx;
+  double inter = 0.0;
+  rowblock_product<T, STAGE>(
+      t, a.t_ptr, [&](int q) { return val[q] * yn[idx[q]]; },
+      [&](int j, T r) {
+        atyn[j] = r;
+        inter += static_cast<double>(xn[j] - x[j]) * static_cast<double>(r - aty[j]);
+      },
+      stage, red_t);
+  inter = block_sum(inter, red);
+  if (threadIdx.x == 0) a.ps_t[blockIdx.x] = inter;
+  if (a.force) return;
+
+  const InstDesc d = a.desc[t.inst];
+  if (!arrive_last(&a.arrive_step[t.inst], d.tt1 - d.tt0, &last_flag)) return;
+  const double dx2 = tile_sum(a.ps_c, d.tc0, d.tc1 - d.tc0, 1, 0, red);
+  const double dy2 = tile_sum(a.ps_a, d.ta0, d.ta1 - d.ta0, 1, 0, red);
+  const double isum = tile_sum(a.ps_t, d.tt0, d.tt1 - d.tt0, 1, 0, red);
+  if (threadIdx.x == 0) {
+    a.arrive_step[t.inst] = 0;
+    decide_step(s, dx2, dy2, isum, d.rows > 0, a.done_count);
+  }
+}
+
+// ---- evaluation kernels (FP64) ---------------------------------------------------------------
+
+// k_eval_prepare: for instances with a pending evaluation, fold the last accepted iterate into
+// the running sums and write the unscaled candidates (pdhg.cpp:357-366, 505-531), plus the
+// squared distances to the restart point used by the primal-weight update (pdhg.cpp:613-624).
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_eval_prepare(Args<T> a) {
+  __shared__ double red[kBlock / kWave];
+  const bool is_col = static_cast<int>(blockIdx.x) < a.n_tc;
+  const int tile = is_col ? static_cast<int>(blockIdx.x) : static_cast<int>(blockIdx.x) - a.n_tc;
+  const Tile t = is_col ? a.tc[tile] : a.tr[tile];
+  const InstState& s = a.state[t.inst];
+  if (!s.eval_pending) return;
+  const int p = s.parity;
+  const bool acc = s.pending_accept;
+  const double count = static_cast<double>(s.averaged);
+  const bool have_avg = s.averaged > 0;
+  const T* __restrict__ v = is_col ? (p ? a.x1 : a.x0) : (p ? a.y1 : a.y0);
+  const T* __restrict__ rst = is_col ? a.xrst : a.yrst;
+  double* __restrict__ sum = is_col ? a.xsum : a.ysum;
+  const double* __restrict__ scale = is_col ? a.dc : a.dr;
+  double* __restrict__ u0 = is_col ? a.xu0 : a.yu0;
+  double* __restrict__ u1 = is_col ? a.xu1 : a.yu1;
+  double d2 = 0.0;
+  for (int k = t.begin + threadIdx.x; k < t.end; k += kBlock) {
+    const T vk = v[k];
+    double total = sum[k];
+    if (acc) {
+      total += static_cast<double>(vk);
+      sum[k] = total;
+    }
+    u0[k] = static_cast<double>(vk) * scale[k];
+    u1[k] = have_avg ? (total / count) * scale[k] : 0.0;
+    const double d = static_cast<double>(vk) - static_cast<double>(rst[k]);
+    d2 += d * d;
+  }
+  d2 = block_sum(d2, red);
+  if (threadIdx.x == 0) (is_col ? a.pe_c : a.pe_r)[tile * 2] = d2;
+}
+
+// k_eval_rows: activity = A x for both candidates on the original A, and every row term of
+// evaluate() (pdhg.cpp:154-170, 203-222, 227-238). Partials per tile, per candidate c:
+// [4c+0] sum violation^2, [4c+1] support, [4c+2] dual violation from y, [4c+3] complementarity.
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_eval_rows(Args<T> a) {
+  __shared__ D2 red2[kBlock / kWave];
+  __shared__ double red[kBlock / kWave];
+  const Tile t = a.ta[blockIdx.x];
+  const InstState& s = a.state[t.inst];
+  if (!(a.report ? s.report_pending : s.eval_pending)) return;
+  const double* __restrict__ xa = a.report ? a.bestx : a.xu0;
+  const double* __restrict__ xb = a.report ? a.bestx : a.xu1;
+  const double* __restrict__ ya = a.report ? a.besty : a.yu0;
+  const double* __restrict__ yb = a.report ? a.besty : a.yu1;
+  const double* __restrict__ val = a.a_orig;
+  const int32_t* __restrict__ idx = a.a_idx;
+  double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  rowblock_product<D2, false>(
+      t, a.a_ptr,
+      [&](int q) {
+        const int c = idx[q];
+        return D2{val[q] * xa[c], val[q] * xb[c]};
+      },
+      [&](int i, D2 act) {
+        const double lo = a.rl[i], hi = a.ru[i];
+        const bool lo_f = dev_finite_bound(lo), hi_f = dev_finite_bound(hi);
+        if (a.report) a.activity[i] = act.a;
+        for (int c = 0; c < 2; ++c) {
+          const double ai = c == 0 ? act.a : act.b;
+          const double yi = c == 0 ? ya[i] : yb[i];
+          double viol = 0.0;
+          if (lo_f) viol = dev_max(viol, lo - ai);
+          if (hi_f) viol = dev_max(viol, ai - hi);
+          acc[4 * c + 0] += viol * viol;
+          if (yi > 0.0) {
+            if (hi_f) acc[4 * c + 1] += yi * hi;
+            else acc[4 * c + 2] += yi * yi;
+          } else if (yi < 0.0) {
+            if (lo_f) acc[4 * c + 1] += yi * lo;
+            else acc[4 * c + 2] += yi * yi;
+          }
+          if (lo != hi) {
+            const double ls = lo_f ? ai - lo : INFINITY;
+            const double us = hi_f ? hi - ai : INFINITY;
+            acc[4 * c + 3] = dev_max(acc[4 * c + 3], fabs(yi) * dev_min(ls, us));
+          }
+        }
+      },
+      (D2*)nullptr, red2);
+  for (int k = 0; k < 8; ++k) {
+    const double r = (k & 3) == 3 ? block_max(acc[k], red) : block_sum(acc[k], red);
+    if (threadIdx.x == 0) a.pe_a[(size_t)blockIdx.x * 8 + k] = r;
+  }
+}
+
+__device__ __forceinline__ double worst_of(const Residuals& r) {
+  return dev_max(dev_max(r.primal, r.dual), r.gap);
+}
+
+__device__ __forceinline__ bool meets_request(const Residuals& r, double tol, int absolute_clause) {
+  return r.primal <= tol && r.dual <= tol && r.gap <= tol &&
+         (!absolute_clause || r.absolute_primal <= 1e-7);
+}
+
+__device__ __forceinline__ bool meets_project_standard(const Residuals& r) {
+  return r.absolute_primal <= 1e-7 && r.absolute_dual <= 1e-7 && r.gap_as_verified <= 1e-9 &&
+         r.complementarity <= 1e-6;
+}
+
+// pdhg.cpp:169-170, 223-224, 252-261
+__device__ Residuals make_residuals(const InstState& s, double pv2, double dviol, double support,
+                                    double bound, double compl_max, double pobj) {
+  Residuals r;
+  r.absolute_primal = sqrt(pv2);
+  r.primal = r.absolute_primal / (1.0 + s.bound_norm);
+  r.absolute_dual = sqrt(dviol);
+  r.dual = r.absolute_dual / (1.0 + s.cost_norm);
+  r.complementarity = compl_max;
+  r.primal_objective = pobj;
+  r.dual_objective = bound - support;
+  const double absolute_gap = fabs(r.primal_objective - r.dual_objective);
+  r.gap = absolute_gap / (1.0 + fabs(r.primal_objective) + fabs(r.dual_objective));
+  r.gap_as_verified = absolute_gap / dev_max(1.0, fabs(r.primal_objective));
+  return r;
+}
+
+// Candidate choice, best tracking, termination and restart (pdhg.cpp:512-643).
+__device__ void decide_evaluation(InstState& s, const Residuals& cur, const Residuals& avg,
+                                  double dx2, double dy2, uint32_t* done_count) {
+  s.evaluations += 1;
+  s.current = cur;
+  s.average = avg;
+  const bool have_avg = s.averaged > 0;
+  const bool pick_avg = have_avg && worst_of(avg) < worst_of(cur);
+  const Residuals chosen = pick_avg ? avg : cur;
+  s.chosen_avg = pick_avg;
+  s.copy_best = 0;
+  s.do_restart = 0;
+  if (worst_of(chosen) < s.best_worst) {
+    s.best = chosen;
+    s.best_worst = worst_of(chosen);
+    s.copy_best = 1;
+  }
+  s.eval_pending = 0;
+  s.pending_accept = 0;  // folded into the sums by k_eval_prepare
+  if (meets_request(chosen, s.tolerance, s.absolute_clause) &&
+      (s.stop_at_request || meets_project_standard(chosen))) {
+    s.best = chosen;
+    s.best_worst = worst_of(chosen);
+    s.copy_best = 1;
+    s.converged = 1;
+    finish(s, meets_project_standard(chosen) ? kStOptimal : kStFeasible, done_count);
+    return;
+  }
+  if (s.use_restarts) {
+    const double kkt = worst_of(chosen);
+    const int64_t since = s.iteration - s.last_restart;
+    const bool sufficient = kkt <= 0.2 * s.restart_kkt;
+    const int64_t floor36 = static_cast<int64_t>(0.36 * static_cast<double>(s.iteration));
+    const bool artificial = since >= (floor36 > 40 ? floor36 : 40);
+    if (sufficient || artificial) {
+      const double dxn = sqrt(dx2), dyn = sqrt(dy2);
+      s.dx_norm = dxn;
+      s.dy_norm = dyn;
+      if (dxn > 1e-12 && dyn > 1e-12) {
+        double w = exp(0.5 * log(dyn / dxn) + 0.5 * log(s.omega));
+        s.omega = w < 1e-6 ? 1e-6 : (1e6 < w ? 1e6 : w);
+      }
+      s.do_restart = 1;
+      s.averaged = 0;
+      s.restart_kkt = kkt;
+      s.last_restart = s.iteration;
+      s.restarts += 1;
+    }
+  }
+  if (s.iteration >= s.iteration_limit) finish(s, kStIterationLimit, done_count);
+}
+
+// k_eval_cols: reduced cost d = c + A^T y per candidate on the original A and every column
+// term of evaluate() (pdhg.cpp:175-199, 239-257); the last tile of an instance reduces all
+// partials, forms both Residuals and decides. Partials per tile, per candidate c:
+// [4c+0] dual violation from d, [4c+1] bound contribution, [4c+2] complementarity,
+// [4c+3] primal objective.
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_eval_cols(Args<T> a) {
+  __shared__ D2 red2[kBlock / kWave];
+  __shared__ double red[kBlock / kWave];
+  __shared__ int last_flag;
+  const Tile t = a.tt[blockIdx.x];
+  InstState& s = a.state[t.inst];
+  if (!(a.report ? s.report_pending : s.eval_pending)) return;
+  const double* __restrict__ xa = a.report ? a.bestx : a.xu0;
+  const double* __restrict__ xb = a.report ? a.bestx : a.xu1;
+  const double* __restrict__ ya = a.report ? a.besty : a.yu0;
+  const double* __restrict__ yb = a.report ? a.besty : a.yu1;
+  const double* __restrict__ val = a.t_orig;
+  const int32_t* __restrict__ idx = a.t_idx;
+  double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  rowblock_product<D2, false>(
+      t, a.t_ptr,
+      [&](int q) {
+        const int r = idx[q];
+        return D2{val[q] * ya[r], val[q] * yb[r]};
+      },
+      [&](int j, D2 dot) {
+        const double lo = a.cl[j], hi = a.cu[j];
+        const bool lo_f = dev_finite_bound(lo), hi_f = dev_finite_bound(hi);
+        const double cj = a.cmin[j];
+        for (int c = 0; c < 2; ++c) {
+          const double d = cj + (c == 0 ? dot.a : dot.b);
+          const double xj = c == 0 ? xa[j] : xb[j];
+          if (c == 0 && a.report) a.reduced[j] = d;
+          if (d > 0.0) {
+            if (lo_f) acc[4 * c + 1] += d * lo;
+            else acc[4 * c + 0] += d * d;
+          } else if (d < 0.0) {
+            if (hi_f) acc[4 * c + 1] += d * hi;
+            else acc[4 * c + 0] += d * d;
+          }
+          if (lo != hi) {
+            const double ls = lo_f ? xj - lo : INFINITY;
+            const double us = hi_f ? hi - xj : INFINITY;
+            acc[4 * c + 2] = dev_max(acc[4 * c + 2], fabs(d) * dev_min(ls, us));
+          }
+          acc[4 * c + 3] += cj * xj;
+        }
+      },
+      (D2*)nullptr, red2);
+  for (int k = 0; k < 8; ++k) {
+    const double r = (k & 3) == 2 ? block_max(acc[k], red) : block_sum(acc[k], red);
+    if (threadIdx.x == 0) a.pe_t[(size_t)blockIdx.x * 8 + k] = r;
+  }
+
+  const InstDesc d = a.desc[t.inst];
+  if (!arrive_last(&a.arrive_eval[t.inst], d.tt1 - d.tt0, &last_flag)) return;
+  double v[18];
+  for (int c = 0; c < 2; ++c) {
+    v[8 * c + 0] = tile_sum(a.pe_a, d.ta0, d.ta1 - d.ta0, 8, 4 * c + 0, red);  // viol^2
+    v[8 * c + 1] = tile_sum(a.pe_a, d.ta0, d.ta1 - d.ta0, 8, 4 * c + 1, red);  // support
+    v[8 * c + 2] = tile_sum(a.pe_a, d.ta0, d.ta1 - d.ta0, 8, 4 * c + 2, red);  // dviol (rows)
+    v[8 * c + 3] = tile_max(a.pe_a, d.ta0, d.ta1 - d.ta0, 8, 4 * c + 3, red);  // compl (rows)
+    v[8 * c + 4] = tile_sum(a.pe_t, d.tt0, d.tt1 - d.tt0, 8, 4 * c + 0, red);  // dviol (cols)
+    v[8 * c + 5] = tile_sum(a.pe_t, d.tt0, d.tt1 - d.tt0, 8, 4 * c + 1, red);  // bound
+    v[8 * c + 6] = tile_max(a.pe_t, d.tt0, d.tt1 - d.tt0, 8, 4 * c + 2, red);  // compl (cols)
+    v[8 * c + 7] = tile_sum(a.pe_t, d.tt0, d.tt1 - d.tt0, 8, 4 * c + 3, red);  // pobj
+  }
+  v[16] = a.report ? 0.0 : tile_sum(a.pe_c, d.tc0, d.tc1 - d.tc0, 2, 0, red);
+  v[17] = a.report ? 0.0 : tile_sum(a.pe_r, d.tr0, d.tr1 - d.tr0, 2, 0, red);
+  if (threadIdx.x != 0) return;
+  a.arrive_eval[t.inst] = 0;
+  Residuals r[2];
+  for (int c = 0; c < 2; ++c) {
+    const double* q = v + 8 * c;
+    r[c] = make_residuals(s, q[0], q[4] + q[2], q[1], q[5], dev_max(q[3], q[6]), q[7]);
+  }
+  if (a.report) {
+    s.report = r[0];
+    s.report_pending = 0;
+    return;
+  }
+  decide_evaluation(s, r[0], r[1], v[16], v[17], a.done_count);
+}
+
+// k_eval_commit: apply the decisions of k_eval_cols (pdhg.cpp:541-545, 592-594, 632-638).
+// Idempotent, so an instance that stopped may see it again without effect.
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_eval_commit(Args<T> a) {
+  const bool is_col = static_cast<int>(blockIdx.x) < a.n_tc;
+  const int tile = is_col ? static_cast<int>(blockIdx.x) : static_cast<int>(blockIdx.x) - a.n_tc;
+  const Tile t = is_col ? a.tc[tile] : a.tr[tile];
+  const InstState& s = a.state[t.inst];
+  if (!s.copy_best && !s.do_restart) return;
+  const int p = s.parity;
+  const bool copy = s.copy_best, restart = s.do_restart, from_avg = s.chosen_avg;
+  const T* __restrict__ v = is_col ? (p ? a.x1 : a.x0) : (p ? a.y1 : a.y0);
+  T* __restrict__ rst = is_col ? a.xrst : a.yrst;
+  double* __restrict__ sum = is_col ? a.xsum : a.ysum;
+  const double* __restrict__ src = is_col ? (from_avg ? a.xu1 : a.xu0) : (from_avg ? a.yu1 : a.yu0);
+  double* __restrict__ best = is_col ? a.bestx : a.besty;
+  for (int k = t.begin + threadIdx.x; k < t.end; k += kBlock) {
+    if (copy) best[k] = src[k];
+    if (restart) {
+      sum[k] = 0.0;
+      rst[k] = v[k];
+    }
+  }
+}
+
+// ---- set-up kernels (FP64, no FMA contraction: bit-for-bit the reference's arithmetic) -------
+
+#pragma clang fp contract(off)
+
+// Ruiz (max) or Pock-Chambolle (sum) statistic of every row of a CSR matrix, as the pass
+// multiplier 1/sqrt(stat) (1 for an empty row). Sums run in stored order, which is the
+// reference's accumulation order (scaling.cpp:44-100). One lane per row.
+__global__ __launch_bounds__(kBlock) void k_row_pass(const int32_t* __restrict__ ptr,
+                                                      const double* __restrict__ val, int32_t rows,
+                                                      int32_t use_sum, double* __restrict__ pass) {
+  const int r = blockIdx.x * kBlock + threadIdx.x;
+  if (r >= rows) return;
+  double acc = 0.0;
+  for (int p = ptr[r]; p < ptr[r + 1]; ++p) {
+    const double m = fabs(val[p]);
+    acc = use_sum ? acc + m : dev_max(acc, m);
+  }
+  pass[r] = acc > 0.0 ? 1.0 / sqrt(acc) : 1.0;
+}
+
+// value * row_pass[i] * column_pass[j] in both layouts (scaling.cpp:14-40). `row_major`:
+// rows of this CSR are rows of A (index array holds columns); otherwise rows are columns.
+__global__ __launch_bounds__(kBlock) void k_apply_pass(const int32_t* __restrict__ ptr,
+                                                        const int32_t* __restrict__ idx,
+                                                        double* __restrict__ val, int32_t rows,
+                                                        int32_t row_major,
+                                                        const double* __restrict__ row_pass,
+                                                        const double* __restrict__ col_pass) {
+  const int r = blockIdx.x * kBlock + threadIdx.x;
+  if (r >= rows) return;
+  for (int p = ptr[r]; p < ptr[r + 1]; ++p) {
+    const double ri = row_major ? row_pass[r] : row_pass[idx[p]];
+    const double cj = row_major ? col_pass[idx[p]] : col_pass[r];
+    val[p] = val[p] * ri * cj;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_scale_mul(double* __restrict__ acc,
+                                                       const double* __restrict__ pass, int32_t n) {
+  const int k = blockIdx.x * kBlock + threadIdx.x;
+  if (k < n) acc[k] *= pass[k];
+}
+
+// Scaled vectors (scaling.cpp:127-145) in the iterate precision, absent bounds as +-inf.
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_scaled_vectors(Args<T> a, T* chat, T* lhat, T* uhat,
+                                                            T* rlhat, T* ruhat, int32_t ncols,
+                                                            int32_t nrows) {
+  const int k = blockIdx.x * kBlock + threadIdx.x;
+  if (k < ncols) {
+    const double dc = a.dc[k];
+    chat[k] = static_cast<T>(a.cmin[k] * dc);
+    const double lo = a.cl[k], hi = a.cu[k];
+    lhat[k] = dev_finite_bound(lo) ? static_cast<T>(lo * (1.0 / dc)) : static_cast<T>(-INFINITY);
+    uhat[k] = dev_finite_bound(hi) ? static_cast<T>(hi * (1.0 / dc)) : static_cast<T>(INFINITY);
+  }
+  if (k < nrows) {
+    const double dr = a.dr[k];
+    const double lo = a.rl[k], hi = a.ru[k];
+    rlhat[k] = dev_finite_bound(lo) ? static_cast<T>(lo * dr) : static_cast<T>(-INFINITY);
+    ruhat[k] = dev_finite_bound(hi) ? static_cast<T>(hi * dr) : static_cast<T>(INFINITY);
+  }
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_convert(const double* __restrict__ src, T* __restrict__ dst,
+                                                     int64_t n) {
+  const int64_t k = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+  if (k < n) dst[k] = static_cast<T>(src[k]);
+}
+
+// ---- power iteration (scaling.cpp:161-194) ---------------------------------------------------
+
+// y_r = sum_p val[p] x[idx[p]] in stored order, one lane per row (the reference's per-row
+// accumulation order for both A x and A^T y).
+__global__ __launch_bounds__(kBlock) void k_spmv_serial(const int32_t* __restrict__ ptr,
+                                                         const int32_t* __restrict__ idx,
+                                                         const double* __restrict__ val,
+                                                         const double* __restrict__ x,
+                                                         double* __restrict__ y, int32_t rows) {
+  const int r = blockIdx.x * kBlock + threadIdx.x;
+  if (r >= rows) return;
+  double acc = 0.0;
+  for (int p = ptr[r]; p < ptr[r + 1]; ++p) acc += val[p] * x[idx[p]];
+  y[r] = acc;
+}
+
+// Per column chunk: sum of v^2 (slot 0 of pe_c).
+__global__ __launch_bounds__(kBlock) void k_chunk_sumsq(const Tile* __restrict__ tc,
+                                                         const double* __restrict__ v,
+                                                         double* __restrict__ part) {
+  __shared__ double red[kBlock / kWave];
+  const Tile t = tc[blockIdx.x];
+  double acc = 0.0;
+  for (int k = t.begin + threadIdx.x; k < t.end; k += kBlock) acc += v[k] * v[k];
+  acc = block_sum(acc, red);
+  if (threadIdx.x == 0) part[blockIdx.x] = acc;
+}
+
+// One workgroup per instance: length = sqrt(sum of its chunk partials).
+__global__ __launch_bounds__(kBlock) void k_inst_norm(const InstDesc* __restrict__ desc,
+                                                       const double* __restrict__ part,
+                                                       double* __restrict__ out) {
+  __shared__ double red[kBlock / kWave];
+  const InstDesc d = desc[blockIdx.x];
+  const double s = tile_sum(part, d.tc0, d.tc1 - d.tc0, 1, 0, red);
+  if (threadIdx.x == 0) out[blockIdx.x] = sqrt(s);
+}
+
+// v /= length of the instance (a zero length leaves v as is; the host then reports 1).
+__global__ __launch_bounds__(kBlock) void k_chunk_div(const Tile* __restrict__ tc,
+                                                       const double* __restrict__ length,
+                                                       double* __restrict__ v) {
+  const Tile t = tc[blockIdx.x];
+  const double len = length[t.inst];
+  if (len == 0.0) return;
+  for (int k = t.begin + threadIdx.x; k < t.end; k += kBlock) v[k] /= len;
+}
+
+#pragma clang fp contract(on)
+
+// ---- initial state (pdhg.cpp:327-379) ----------------------------------------------------------
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_init_iterates(Args<T> a, const double* __restrict__ lhat64,
+                                                           const double* __restrict__ uhat64,
+                                                           int32_t ncols, int32_t nrows) {
+  const int k = blockIdx.x * kBlock + threadIdx.x;
+  if (k < ncols) {
+    const T x = dev_project<T>(static_cast<T>(0), a.lhat[k], a.uhat[k]);
+    a.x0[k] = x;
+    a.x1[k] = x;
+    a.xrst[k] = x;
+    a.xbar[k] = x;
+    a.aty0[k] = static_cast<T>(0);
+    a.aty1[k] = static_cast<T>(0);
+    a.xsum[k] = 0.0;
+    // pdhg.cpp:377: best starts as the scaled start point, in FP64
+    double b = 0.0;
+    if (dev_finite_bound(lhat64[k]) && b < lhat64[k]) b = lhat64[k];
+    else if (dev_finite_bound(uhat64[k]) && b > uhat64[k]) b = uhat64[k];
+    a.bestx[k] = b;
+  }
+  if (k < nrows) {
+    a.y0[k] = static_cast<T>(0);
+    a.y1[k] = static_cast<T>(0);
+    a.yrst[k] = static_cast<T>(0);
+    a.ysum[k] = 0.0;
+    a.besty[k] = 0.0;
+  }
+}
+
+}  // namespace sk

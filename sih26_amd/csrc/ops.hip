@@ -1,0 +1,315 @@
+// HIP runtime wrapper and standalone device operators behind the C ABI (include/sk_gpu.h):
+// the counterparts of the reference's gpu_runtime.hpp and gpu_ops.hpp launch_gpu_* functions
+// (src/gpu/gpu_runtime_cuda.cu:15-121, src/gpu/cuda_kernels.cu:25-196).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+
+#include "../../include/sk_gpu.h"
+#include "engine.hpp"
+#include "device_util.hpp"
+#include "status.hpp"
+
+namespace sk {
+namespace {
+
+int fail(hipError_t e, const char* what, int code) {
+  set_last_error(std::string(what) + ": " + hipGetErrorString(e));
+  return code;
+}
+
+int launch_status(const char* what) {
+  const hipError_t e = hipGetLastError();
+  return e == hipSuccess ? kStatusSuccess : fail(e, what, kStatusLaunchFailed);
+}
+
+unsigned grid_for(int64_t n, int per_block) {
+  return static_cast<unsigned>(std::max<int64_t>(1, (n + per_block - 1) / per_block));
+}
+
+// Grow-only device scratch for the two-stage dot product.
+double* g_dot_scratch = nullptr;
+size_t g_dot_scratch_count = 0;
+
+__global__ __launch_bounds__(kBlock) void k_axpy(int32_t n, double alpha, const double* __restrict__ x,
+                                                  double* __restrict__ y) {
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) y[i] += alpha * x[i];
+}
+
+// Stage 1 of the dot product: each workgroup reduces a contiguous slice in a fixed order.
+__global__ __launch_bounds__(kBlock) void k_dot_partial(int32_t n, int32_t per_block,
+                                                         const double* __restrict__ x,
+                                                         const double* __restrict__ y,
+                                                         double* __restrict__ part) {
+  __shared__ double red[kBlock / kWave];
+  const int begin = blockIdx.x * per_block;
+  const int end = min(n, begin + per_block);
+  double acc = 0.0;
+  for (int i = begin + threadIdx.x; i < end; i += kBlock) acc += x[i] * y[i];
+  acc = block_sum(acc, red);
+  if (threadIdx.x == 0) part[blockIdx.x] = acc;
+}
+
+// Stage 2: one workgroup reduces the partials.
+__global__ __launch_bounds__(kBlock) void k_dot_final(int32_t count, const double* __restrict__ part,
+                                                       double* __restrict__ out) {
+  __shared__ double red[kBlock / kWave];
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < count; i += kBlock) acc += part[i];
+  acc = block_sum(acc, red);
+  if (threadIdx.x == 0) *out = acc;
+}
+
+// cuda_kernels.cu:60-75 counterpart, FP64.
+__global__ __launch_bounds__(kBlock) void k_primal_step(int32_t n, double tau, const double* __restrict__ x,
+                                                         const double* __restrict__ cost,
+                                                         const double* __restrict__ at_y,
+                                                         const double* __restrict__ lo,
+                                                         const double* __restrict__ hi,
+                                                         double* __restrict__ x_next,
+                                                         double* __restrict__ xbar) {
+  const int j = blockIdx.x * kBlock + threadIdx.x;
+  if (j >= n) return;
+  const double g = cost[j] + at_y[j];
+  double v = x[j] - tau * g;
+  if (dev_finite_bound(lo[j]) && v < lo[j]) v = lo[j];
+  else if (dev_finite_bound(hi[j]) && v > hi[j]) v = hi[j];
+  x_next[j] = v;
+  xbar[j] = 2.0 * v - x[j];
+}
+
+// One wavefront per compressed row: lanes stride the row, then a fixed butterfly.
+__global__ __launch_bounds__(kBlock) void k_wave_rows(int32_t rows, const int32_t* __restrict__ ptr,
+                                                       const int32_t* __restrict__ idx,
+                                                       const double* __restrict__ val,
+                                                       const double* __restrict__ x,
+                                                       double* __restrict__ y, double alpha) {
+  const int row = blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
+  const int lane = threadIdx.x & (kWave - 1);
+  if (row >= rows) return;  // whole wavefronts exit together
+  double acc = 0.0;
+  for (int p = ptr[row] + lane; p < ptr[row + 1]; p += kWave) acc += val[p] * x[idx[p]];
+  acc = group_sum(acc, kWave);
+  if (lane == 0) y[row] = alpha * acc;
+}
+
+__global__ __launch_bounds__(kBlock) void k_csc_scatter(int32_t cols, const int32_t* __restrict__ ptr,
+                                                         const int32_t* __restrict__ idx,
+                                                         const double* __restrict__ val,
+                                                         const double* __restrict__ x,
+                                                         double* __restrict__ y, double alpha) {
+  const int j = blockIdx.x * kBlock + threadIdx.x;
+  if (j >= cols) return;
+  const double xj = alpha * x[j];
+  if (xj == 0.0) return;
+  for (int p = ptr[j]; p < ptr[j + 1]; ++p) atomicAdd(&y[idx[p]], val[p] * xj);
+}
+
+}  // namespace
+}  // namespace sk
+
+using namespace sk;
+
+extern "C" {
+
+const char* sk_gpu_status_string(int32_t status) {
+  switch (status) {
+    case SK_GPU_SUCCESS: return "success";
+    case SK_GPU_UNAVAILABLE: return "hip unavailable";
+    case SK_GPU_NO_DEVICE: return "no compatible hip device";
+    case SK_GPU_INIT_FAILED: return "initialization failed";
+    case SK_GPU_OUT_OF_MEMORY: return "out of device memory";
+    case SK_GPU_LAUNCH_FAILED: return "kernel launch failed";
+    case SK_GPU_COPY_FAILED: return "memory copy failed";
+    case SK_GPU_SYNC_FAILED: return "device synchronization failed";
+    case SK_GPU_INVALID_STATE: return "invalid device state";
+    case SK_GPU_NON_FINITE: return "non-finite numerical output detected";
+    case SK_GPU_INVALID_ARGUMENT: return "invalid argument";
+    default: return "unknown status";
+  }
+}
+
+int32_t sk_gpu_device_count(void) {
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess) return 0;
+  return count;
+}
+
+int32_t sk_gpu_is_available(void) { return sk_gpu_device_count() > 0 ? 1 : 0; }
+
+int32_t sk_gpu_get_device_info(int32_t device_id, sk_gpu_device_info* out) {
+  if (!out) {
+    set_last_error("sk_gpu_get_device_info: null output");
+    return kStatusInvalidArgument;
+  }
+  const int count = sk_gpu_device_count();
+  if (count == 0) {
+    set_last_error("no HIP device is available");
+    return kStatusNoDevice;
+  }
+  if (device_id < 0 || device_id >= count) {
+    set_last_error("device id out of range");
+    return kStatusInvalidState;
+  }
+  hipDeviceProp_t p;
+  const hipError_t e = hipGetDeviceProperties(&p, device_id);
+  if (e != hipSuccess) return fail(e, "hipGetDeviceProperties", kStatusInitFailed);
+  std::memset(out, 0, sizeof(*out));
+  out->device_id = device_id;
+  std::strncpy(out->name, p.name, sizeof(out->name) - 1);
+  std::strncpy(out->gcn_arch_name, p.gcnArchName, sizeof(out->gcn_arch_name) - 1);
+  out->total_memory_bytes = p.totalGlobalMem;
+  out->gcn_arch_major = p.major;
+  out->gcn_arch_minor = p.minor;
+  out->multiprocessor_count = p.multiProcessorCount;
+  out->warp_size = p.warpSize;
+  out->max_threads_per_block = p.maxThreadsPerBlock;
+  return kStatusSuccess;
+}
+
+int32_t sk_gpu_set_device(int32_t device_id) {
+  if (sk_gpu_device_count() == 0) {
+    set_last_error("no HIP device is available");
+    return kStatusNoDevice;
+  }
+  const hipError_t e = hipSetDevice(device_id);
+  return e == hipSuccess ? kStatusSuccess : fail(e, "hipSetDevice", kStatusInvalidState);
+}
+
+int32_t sk_gpu_synchronize(void) {
+  const hipError_t e = hipDeviceSynchronize();
+  return e == hipSuccess ? kStatusSuccess : fail(e, "hipDeviceSynchronize", kStatusSyncFailed);
+}
+
+const char* sk_gpu_last_error(void) { return last_error(); }
+
+int32_t sk_gpu_malloc(void** dev_ptr, size_t bytes) {
+  if (!dev_ptr) return kStatusInvalidArgument;
+  *dev_ptr = nullptr;
+  const hipError_t e = hipMalloc(dev_ptr, bytes);
+  if (e != hipSuccess) {
+    *dev_ptr = nullptr;
+    return fail(e, "hipMalloc", kStatusOutOfMemory);
+  }
+  return kStatusSuccess;
+}
+
+int32_t sk_gpu_free(void* dev_ptr) {
+  const hipError_t e = hipFree(dev_ptr);
+  return e == hipSuccess ? kStatusSuccess : fail(e, "hipFree", kStatusInvalidState);
+}
+
+int32_t sk_gpu_copy_h2d(void* dst, const void* src, size_t bytes) {
+  const hipError_t e = hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice);
+  return e == hipSuccess ? kStatusSuccess : fail(e, "hipMemcpy H2D", kStatusCopyFailed);
+}
+
+int32_t sk_gpu_copy_d2h(void* dst, const void* src, size_t bytes) {
+  const hipError_t e = hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost);
+  return e == hipSuccess ? kStatusSuccess : fail(e, "hipMemcpy D2H", kStatusCopyFailed);
+}
+
+int32_t sk_gpu_memset(void* dev_ptr, int32_t value, size_t bytes) {
+  const hipError_t e = hipMemset(dev_ptr, value, bytes);
+  return e == hipSuccess ? kStatusSuccess : fail(e, "hipMemset", kStatusCopyFailed);
+}
+
+int32_t sk_gpu_axpy(int32_t n, double alpha, const double* d_x, double* d_y) {
+  if (n <= 0) return kStatusSuccess;
+  if (!d_x || !d_y) return kStatusInvalidState;
+  k_axpy<<<std::min<unsigned>(grid_for(n, kBlock), 4096u), kBlock>>>(n, alpha, d_x, d_y);
+  return launch_status("k_axpy");
+}
+
+int32_t sk_gpu_dot(int32_t n, const double* d_x, const double* d_y, double* host_result) {
+  if (!host_result) return kStatusInvalidArgument;
+  *host_result = 0.0;
+  if (n <= 0) return kStatusSuccess;
+  if (!d_x || !d_y) return kStatusInvalidState;
+  const int32_t per_block = 8 * kBlock;
+  const auto blocks = static_cast<int32_t>(grid_for(n, per_block));
+  if (static_cast<size_t>(blocks) + 1 > g_dot_scratch_count) {
+    if (g_dot_scratch) (void)hipFree(g_dot_scratch);
+    g_dot_scratch = nullptr;
+    g_dot_scratch_count = 0;
+    const hipError_t e = hipMalloc(&g_dot_scratch, (static_cast<size_t>(blocks) + 1) * sizeof(double));
+    if (e != hipSuccess) return fail(e, "hipMalloc", kStatusOutOfMemory);
+    g_dot_scratch_count = static_cast<size_t>(blocks) + 1;
+  }
+  k_dot_partial<<<blocks, kBlock>>>(n, per_block, d_x, d_y, g_dot_scratch + 1);
+  int st = launch_status("k_dot_partial");
+  if (st) return st;
+  k_dot_final<<<1, kBlock>>>(blocks, g_dot_scratch + 1, g_dot_scratch);
+  if ((st = launch_status("k_dot_final"))) return st;
+  const hipError_t e = hipMemcpy(host_result, g_dot_scratch, sizeof(double), hipMemcpyDeviceToHost);
+  return e == hipSuccess ? kStatusSuccess : fail(e, "hipMemcpy D2H", kStatusCopyFailed);
+}
+
+int32_t sk_gpu_pdhg_primal_step(int32_t n, double tau, const double* d_x, const double* d_cost,
+                                const double* d_at_y, const double* d_lo, const double* d_hi,
+                                double* d_x_next, double* d_xbar) {
+  if (n <= 0) return kStatusSuccess;
+  if (!d_x || !d_cost || !d_at_y || !d_lo || !d_hi || !d_x_next || !d_xbar) return kStatusInvalidState;
+  k_primal_step<<<grid_for(n, kBlock), kBlock>>>(n, tau, d_x, d_cost, d_at_y, d_lo, d_hi, d_x_next, d_xbar);
+  return launch_status("k_primal_step");
+}
+
+int32_t sk_gpu_spmv_csc(int32_t num_rows, int32_t num_cols, const int32_t* d_col_starts,
+                        const int32_t* d_row_indices, const double* d_values, const double* d_x,
+                        double* d_y, double alpha, int32_t transpose) {
+  if (num_rows <= 0 || num_cols <= 0) return kStatusSuccess;
+  if (!d_col_starts || !d_row_indices || !d_values || !d_x || !d_y) return kStatusInvalidState;
+  if (transpose) {
+    k_wave_rows<<<grid_for(num_cols, kBlock / kWave), kBlock>>>(num_cols, d_col_starts, d_row_indices,
+                                                                d_values, d_x, d_y, alpha);
+    return launch_status("k_wave_rows");
+  }
+  const hipError_t e = hipMemsetAsync(d_y, 0, static_cast<size_t>(num_rows) * sizeof(double), nullptr);
+  if (e != hipSuccess) return fail(e, "hipMemsetAsync", kStatusCopyFailed);
+  k_csc_scatter<<<grid_for(num_cols, kBlock), kBlock>>>(num_cols, d_col_starts, d_row_indices,
+                                                        d_values, d_x, d_y, alpha);
+  return launch_status("k_csc_scatter");
+}
+
+int32_t sk_gpu_spmv_csr(int32_t num_rows, int32_t num_cols, const int32_t* d_row_starts,
+                        const int32_t* d_col_indices, const double* d_values, const double* d_x,
+                        double* d_y, double alpha) {
+  if (num_rows <= 0 || num_cols <= 0) return kStatusSuccess;
+  if (!d_row_starts || !d_col_indices || !d_values || !d_x || !d_y) return kStatusInvalidState;
+  k_wave_rows<<<grid_for(num_rows, kBlock / kWave), kBlock>>>(num_rows, d_row_starts, d_col_indices,
+                                                              d_values, d_x, d_y, alpha);
+  return launch_status("k_wave_rows");
+}
+
+int32_t sk_gpu_compare_vectors(int32_t n, const double* t, const double* r, double tol,
+                               sk_gpu_comparison* out) {
+  if (!out) return kStatusInvalidArgument;
+  std::memset(out, 0, sizeof(*out));
+  out->passed = 1;
+  if (n <= 0) return kStatusSuccess;
+  if (!t || !r) {
+    out->passed = 0;
+    return kStatusSuccess;
+  }
+  double max_abs = 0.0, max_rel = 0.0;
+  for (int32_t i = 0; i < n; ++i) {
+    if (!std::isfinite(t[i]) || !std::isfinite(r[i])) {
+      out->contains_nan_or_inf = 1;
+      out->passed = 0;
+      return kStatusSuccess;
+    }
+    const double abs_diff = std::fabs(t[i] - r[i]);
+    max_abs = std::max(max_abs, abs_diff);
+    const double rel = abs_diff / std::max(1.0, std::max(std::fabs(t[i]), std::fabs(r[i])));
+    max_rel = std::max(max_rel, rel);
+    if (abs_diff > tol && rel > tol) out->passed = 0;
+  }
+  out->max_abs_diff = max_abs;
+  out->max_rel_diff = max_rel;
+  return kStatusSuccess;
+}
+
+}  // extern "C"

@@ -1,0 +1,154 @@
+"""The GPU restarted-PDHG engine against the FP64 CPU oracle (restatement of
+src/pdhg/pdhg.cpp) on seeded inputs, and against closed-form optima."""
+import numpy as np
+import pytest
+
+from tests.lp_cases import CLOSED_FORM, closed_form, kkt_lp
+
+pytestmark = pytest.mark.gpu
+
+
+def to_lp(sk, d):
+    return sk.Lp.from_arrays(d["num_rows"], d["num_cols"], d["col_starts"], d["row_indices"],
+                             d["values"], d["cost"], d["col_lower"], d["col_upper"],
+                             d["row_lower"], d["row_upper"], d.get("sense", 1), d.get("offset", 0.0))
+
+
+def kkt_of(oracle_mod, d, r):
+    """Relative KKT (pdhg.cpp:110 worst()) of a returned point, measured by the oracle in FP64
+    on the original model. row_dual = -sense * y (pdhg.cpp:657-661)."""
+    y = -d.get("sense", 1) * r.row_dual
+    return oracle_mod.evaluate(d, r.col_value, y)
+
+
+@pytest.mark.parametrize("name", [c[0] for c in CLOSED_FORM])
+def test_closed_form_fp64_reference_rule(gpu, name):
+    # test_pdhg.cpp:95-146 with the reference's own stop rule at 1e-8
+    d, optimum, xs = closed_form(name)
+    r = gpu.solve(to_lp(gpu, d), pdhg_tolerance=1e-8, pdhg_precision="fp64", iteration_limit=200000)
+    assert r.status_name == "optimal", r.info["message"]
+    assert r.info["algorithm"] == "pdhg-gpu"
+    assert abs(r.objective - optimum) <= 1e-6 * max(1.0, abs(optimum))
+    if xs is not None:
+        np.testing.assert_allclose(r.col_value, xs, atol=1e-4)
+    assert r.primal_infeasibility <= 1e-7
+
+
+@pytest.mark.parametrize("name", [c[0] for c in CLOSED_FORM])
+def test_closed_form_fp32_relative_rule(gpu, oracle_mod, name):
+    d, optimum, _ = closed_form(name)
+    r = gpu.solve(to_lp(gpu, d), pdhg_tolerance=1e-6, pdhg_absolute_primal=False,
+                  pdhg_stop_at_request=True, iteration_limit=200000)
+    assert r.converged, r.info["message"]
+    ev = kkt_of(oracle_mod, d, r)
+    assert ev["worst"] <= 1e-6 * 1.000001
+    assert abs(r.objective - optimum) <= 1e-4 * max(1.0, abs(optimum))
+
+
+def test_iteration_limit_is_honest(gpu):
+    # test_pdhg.cpp:148-157
+    d, _, _ = closed_form("sum")
+    r = gpu.solve(to_lp(gpu, d), pdhg_tolerance=1e-12, iteration_limit=20)
+    assert r.status_name != "optimal"
+    assert np.isinf(r.dual_bound)
+    assert r.info["message"]
+    assert r.iterations == 20
+
+
+def test_step_size_survives_first_iteration(gpu):
+    # test_pdhg.cpp:248-263
+    d, _, _ = closed_form("sum")
+    r = gpu.solve(to_lp(gpu, d), pdhg_tolerance=1e-8, pdhg_precision="fp64")
+    assert r.status_name == "optimal", r.info["message"]
+    assert r.iterations < 500
+
+
+def test_large_rhs_never_claims_optimal_unverified(gpu):
+    # test_pdhg.cpp:224-242, 265-294
+    d = {"num_rows": 1, "num_cols": 2, "col_starts": np.array([0, 1, 2], np.int32),
+         "row_indices": np.array([0, 0], np.int32), "values": np.array([1.0, 1.0]),
+         "cost": np.array([1.0, 1.0]), "col_lower": np.zeros(2), "col_upper": np.full(2, np.inf),
+         "row_lower": np.array([1e5]), "row_upper": np.array([np.inf])}
+    for tol in (1e-4, 1e-6, 1e-8):
+        r = gpu.solve(to_lp(gpu, d), pdhg_tolerance=tol, pdhg_precision="fp64", iteration_limit=200000)
+        if r.status_name == "optimal":
+            assert r.primal_infeasibility <= 1e-7
+        assert r.status_name != "feasible" or r.primal_infeasibility <= 1e-7
+
+
+def test_kkt_backwards_instances_fp64(gpu, oracle_mod):
+    # the AgreesWithTheSimplexOnGeneratedInstances idea (test_pdhg.cpp:178-222): analytic optima
+    rng = np.random.default_rng(20260904)
+    compared = 0
+    for _ in range(40):
+        d, opt = kkt_lp(rng, int(rng.integers(2, 7)), int(rng.integers(2, 7)))
+        r = gpu.solve(to_lp(gpu, d), pdhg_tolerance=1e-9, pdhg_precision="fp64",
+                      iteration_limit=200000)
+        if r.status_name != "optimal":
+            continue
+        compared += 1
+        assert abs(r.objective - opt) <= 1e-6 * max(1.0, abs(opt)), (r.objective, opt)
+    assert compared > 20
+
+
+def test_fp64_engine_tracks_the_oracle_on_c1(gpu, oracle_mod):
+    d = gpu.Lp.generate(1, 1).export()
+    ref = oracle_mod.solve_pdhg(d, tolerance=1e-6, stop_at_request=True, absolute_primal=False)
+    r = gpu.solve(gpu.Lp.generate(1, 1), pdhg_tolerance=1e-6, pdhg_stop_at_request=True,
+                  pdhg_absolute_primal=False, pdhg_precision="fp64")
+    assert r.converged and ref["converged"]
+    # same method in FP64: only reduction order differs, so the spectral norm agrees closely
+    assert abs(r.spectral_norm - ref["spectral_norm"]) <= 1e-12 * ref["spectral_norm"]
+    assert abs(r.objective - ref["objective"]) <= 1e-5 * abs(ref["objective"])
+    assert abs(r.iterations - ref["iterations"]) <= 0.25 * ref["iterations"] + 80
+    ev = kkt_of(oracle_mod, d, r)
+    assert ev["worst"] <= 1e-6 * 1.000001
+
+
+def test_fp32_split_meets_1e4_on_c1(gpu, oracle_mod):
+    d = gpu.Lp.generate(1, 1).export()
+    tight = oracle_mod.solve_pdhg(d, tolerance=1e-8, stop_at_request=True, absolute_primal=False)
+    r = gpu.solve(gpu.Lp.generate(1, 1), pdhg_tolerance=1e-4, pdhg_stop_at_request=True,
+                  pdhg_absolute_primal=False)
+    assert r.converged, r.info["message"]
+    ev = kkt_of(oracle_mod, d, r)
+    assert ev["worst"] <= 1e-4 * 1.000001
+    # a 1e-4 relative KKT point is within a few 1e-4 of the optimum objective
+    assert abs(r.objective - tight["objective"]) <= 1e-3 * abs(tight["objective"])
+
+
+def test_batch_of_c3_instances_each_meet_1e4(gpu, oracle_mod):
+    ks = list(range(12))
+    lps = [gpu.Lp.generate(3, k) for k in ks]
+    b = gpu.Batch(pdhg_tolerance=1e-4, pdhg_stop_at_request=True, pdhg_absolute_primal=False)
+    for lp in lps:
+        b.add(lp)
+    b.upload()
+    b.solve()
+    for k, lp in zip(ks, lps):
+        r = b.result(k)
+        d = lp.export()
+        assert r.converged, (k, r.info["message"])
+        ev = kkt_of(oracle_mod, d, r)
+        assert ev["worst"] <= 1e-4 * 1.000001, (k, ev["worst"])
+    # solo equals in-batch, bit for bit (per-instance tiles and fixed reduction order)
+    for k in (0, 5):
+        solo = gpu.solve(lps[k], pdhg_tolerance=1e-4, pdhg_stop_at_request=True,
+                         pdhg_absolute_primal=False)
+        mixed = b.result(k)
+        assert solo.iterations == mixed.iterations
+        np.testing.assert_array_equal(solo.col_value, mixed.col_value)
+        np.testing.assert_array_equal(solo.row_dual, mixed.row_dual)
+
+
+def test_batch_resolve_is_reproducible(gpu):
+    lps = [gpu.Lp.generate(3, k) for k in (20, 21, 22)]
+    b = gpu.Batch(pdhg_tolerance=1e-4, pdhg_stop_at_request=True, pdhg_absolute_primal=False)
+    for lp in lps:
+        b.add(lp)
+    b.upload()
+    b.solve()
+    first = [b.result(k).col_value.copy() for k in range(3)]
+    b.solve()
+    for k in range(3):
+        np.testing.assert_array_equal(first[k], b.result(k).col_value)
