@@ -1,0 +1,198 @@
+"""Benchmark: LPs solved/s and PDHG iterations/s at 1e-4 relative KKT on one MI355X.
+
+Default workload: the 256-LP batch of uneven blending LPs (rows U{200..5000}, columns 2-3x
+rows, 4-8 nonzeros per column, seeds 1000+k), solved as one block-diagonal batch by the
+FP32/FP64-split engine with the relative-KKT stop (pdhg_stop_at_request, no absolute-primal
+clause). One "step" is one complete solve of the batch from inputs resident in HBM: device
+scaling, power iteration, the PDHG loop until every instance stops, and the final report.
+
+Prints one JSON line. Extra fields:
+  roofline      the dominant step kernel, timed with HIP events on the engine's stream
+  cpu_baseline  the FP64 CPU oracle on a bounded sample of the same instances, same stop rule
+  fp64_single   the plain FP64 engine solving instances one at a time (the x20 denominator)
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import statistics
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+WORKLOADS = {
+    "c3": "256 blending LPs, rows U{200..5000}, cols 2-3x rows, 4-8 nnz/col, block-diagonal batch",
+    "c1": "one 500x1000 blending LP, 5 nnz/col",
+    "c2": "one 200k x 500k power-law LP, 2.5M nnz, |a| log-uniform [1e-3, 1e3]",
+    "c5": "one 2M x 5M power-law LP, 25M nnz",
+}
+KERNEL_NAMES = {0: "k_step_primal", 1: "k_step_dual", 2: "k_step_aty"}
+
+
+def solve_options(sk, precision: str = "fp32"):
+    return sk.make_options(pdhg_tolerance=1e-4, pdhg_stop_at_request=True,
+                           pdhg_absolute_primal=False, pdhg_precision=precision,
+                           compute_quality=0)
+
+
+def instances(sk, workload: str, count: int):
+    if workload == "c3":
+        return [sk.Lp.generate(3, k) for k in range(count)]
+    cfg = {"c1": 1, "c2": 2, "c5": 5}[workload]
+    seed = {"c1": 1, "c2": 11, "c5": 5}[workload]
+    return [sk.Lp.generate(cfg, seed)]
+
+
+def cpu_baseline(lps, budget_s: float) -> dict:
+    """The FP64 CPU oracle (restatement of src/pdhg/pdhg.cpp), one thread, same stop rule,
+    on instances 0, 1, 2, ... until `budget_s` of CPU time is used."""
+    from oracle import oracle
+    oracle.build(ref=False, arch_flags="-march=native")
+    oracle.lib()
+    done, iters, t0 = 0, 0, time.perf_counter()
+    for lp in lps:
+        r = oracle.solve_pdhg(lp.export(), tolerance=1e-4, stop_at_request=True,
+                              absolute_primal=False)
+        done += 1
+        iters += r["iterations"]
+        if time.perf_counter() - t0 > budget_s:
+            break
+    dt = time.perf_counter() - t0
+    return {"value": done / dt, "unit": "LPs/s", "cores": 1, "kind": "port",
+            "sample": f"first {done} of the {len(lps)} instances, FP64 oracle, 1 thread, "
+                      f"{dt:.1f} s, {iters / dt:.0f} PDHG iterations/s",
+            "iters_per_s": iters / dt}
+
+
+def fp64_single(sk, lps, budget_s: float) -> dict:
+    """The plain FP64 engine, one instance per solve, inputs resident, until budget_s."""
+    batches = []
+    for lp in lps:
+        b = sk.Batch(solve_options(sk, "fp64"))
+        b.add(lp)
+        b.upload()
+        batches.append(b)
+    batches[0].solve()  # warm (graph capture)
+    sk.lib().sk_gpu_synchronize()
+    done, iters, t0 = 0, 0, time.perf_counter()
+    for b in batches:
+        b.solve()
+        done += 1
+        iters += b.result(0).iterations
+        if time.perf_counter() - t0 > budget_s:
+            break
+    dt = time.perf_counter() - t0
+    for b in batches:
+        b.close()
+    return {"lps_per_s": done / dt, "iters_per_s": iters / dt, "instances": done,
+            "seconds": dt}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="c3", choices=sorted(WORKLOADS))
+    ap.add_argument("--instances", type=int, default=256)
+    ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--fp64-budget", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--kernel-reps", type=int, default=50)
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported "
+              "(batches shard by instance across GPUs; that path is not built)", file=sys.stderr)
+        return 2
+
+    import sih26_amd as sk
+    if not sk.gpu_available():
+        print("no HIP device", file=sys.stderr)
+        return 1
+
+    lps = instances(sk, args.workload, args.instances)
+    batch = sk.Batch(solve_options(sk))
+    for lp in lps:
+        batch.add(lp)
+    batch.upload()
+    for _ in range(args.warmup):
+        batch.solve()
+    sk.lib().sk_gpu_synchronize()
+
+    times = []
+    t_start = time.perf_counter()
+    for _ in range(args.steps):
+        t0 = time.perf_counter()
+        batch.solve()
+        times.append(time.perf_counter() - t0)
+    sk.lib().sk_gpu_synchronize()
+    total = time.perf_counter() - t_start
+
+    results = [batch.result(k) for k in range(len(lps))]
+    solved = sum(1 for r in results if r.converged)
+    iterations = sum(r.iterations for r in results)
+    stats = batch.stats()
+    n = len(lps)
+
+    # dominant kernel: time every step kernel with HIP events on the engine's stream
+    kern = {}
+    for k in (0, 1, 2):
+        ms, nbytes = batch.time_kernel(k, args.kernel_reps)
+        kern[k] = {"name": KERNEL_NAMES[k], "ms": ms, "bytes": nbytes,
+                   "gbs": nbytes / (ms * 1e-3) / 1e9}
+    dom = max((1, 2), key=lambda k: kern[k]["ms"])
+    traffic = None
+    tf = ROOT / "profiles" / f"traffic_{args.workload}.json"
+    if tf.exists():
+        traffic = json.loads(tf.read_text()).get(kern[dom]["name"])
+
+    line = {
+        "metric": "LPs solved/sec + PDHG iters/sec at 1e-4 rel KKT; p50/p99 solve latency",
+        "value": solved * args.steps / total,
+        "unit": "LPs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * total / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32 (SpMV, iterates) + fp64 (KKT, gap, restart reductions)",
+        "data": "synthetic (seeded generator in sih26_amd/csrc/generator.cpp)",
+        "config": {"workload": WORKLOADS[args.workload], "instances": n,
+                   "rows": stats["total_rows"], "cols": stats["total_cols"],
+                   "nnz": stats["total_nnz"], "tolerance": 1e-4,
+                   "stop": "relative KKT (primal, dual, gap) <= 1e-4, FP64 on the original model"},
+        "pdhg_iters_per_s": iterations * args.steps / total,
+        "solved": solved,
+        "instance_iterations": {"sum": iterations, "max": max(r.iterations for r in results),
+                                "median": statistics.median(r.iterations for r in results)},
+        "latency_ms": {"p50": 1e3 * statistics.median(times),
+                       "p99": 1e3 * sorted(times)[max(0, int(round(0.99 * len(times))) - 1)]},
+        "phase_ms": {"setup": stats["setup_ms"], "loop": stats["loop_ms"],
+                     "report": stats["report_ms"], "graph_launches": stats["graph_launches"]},
+        "kernels": {v["name"]: {"ms": v["ms"], "GB/s": v["gbs"], "bytes": v["bytes"]}
+                    for v in kern.values()},
+        "roofline": {"bound": "hbm", "kernel": kern[dom]["name"], "achieved": kern[dom]["gbs"],
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": kern[dom]["gbs"] / HBM_PEAK_GBS, "traffic": traffic},
+    }
+    if not args.no_fp64 and args.workload == "c3":
+        f = fp64_single(sk, lps, args.fp64_budget)
+        line["fp64_single"] = f
+        line["speedup_vs_fp64_single"] = line["value"] / f["lps_per_s"]
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(lps, args.cpu_budget)
+    print(json.dumps(line))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -83,6 +83,7 @@ def lib() -> C.CDLL:
         _lib.ok_recompute_quality.argtypes = [C.POINTER(Model), C.POINTER(Solution)]
         _lib.ok_spectral_norm.argtypes = [i32, i32, vp, vp, vp, i32, C.c_uint32]
         _lib.ok_mt19937_uniform.argtypes = [C.c_uint32, i32, vp]
+        _lib.ok_mt19937_raw.argtypes = [C.c_uint32, i32, vp]
         _lib.ok_cpu_axpy.argtypes = [i32, f64, vp, vp]
         _lib.ok_cpu_dot.argtypes = [i32, vp, vp]
         _lib.ok_cpu_primal_step.argtypes = [i32, f64, vp, vp, vp, vp, vp, vp, vp]

@@ -285,6 +285,12 @@ static double mt_uniform(mt19937* g, double a, double b) {
   return u * (b - a) + a;
 }
 
+void ok_mt19937_raw(uint32_t seed, int32_t count, uint32_t* out) {
+  mt19937 g;
+  mt_seed(&g, seed);
+  for (int32_t k = 0; k < count; ++k) out[k] = mt_next(&g);
+}
+
 void ok_mt19937_uniform(uint32_t seed, int32_t count, double* out) {
   mt19937 g;
   mt_seed(&g, seed);

@@ -162,6 +162,7 @@ double ok_spectral_norm(int32_t rows, int32_t cols, const int32_t* col_starts,
 /* The MT19937 + libstdc++ uniform_real_distribution<double>(-1, 1) draws used above,
  * exposed so the restatement of the generator can be checked on its own. */
 void ok_mt19937_uniform(uint32_t seed, int32_t count, double* out);
+void ok_mt19937_raw(uint32_t seed, int32_t count, uint32_t* out);
 
 /* pdhg.cpp:143-263 (evaluate). activity (m) and reduced (n) are outputs. */
 ok_residuals ok_evaluate(const ok_model* model, const double* min_cost, double bound_norm,

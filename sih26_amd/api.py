@@ -151,7 +151,7 @@ def make_options(**kw) -> _ffi.PdhgOptions:
     lib().sk_pdhg_default_options(C.byref(o))
     fields = {f for f, _ in o._fields_}
     for k, v in kw.items():
-        if k in fields:
+        if k in fields and not isinstance(v, str):
             setattr(o, k, v)
         else:
             val = ("true" if v else "false") if isinstance(v, bool) else str(v)

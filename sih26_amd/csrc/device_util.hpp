@@ -85,6 +85,17 @@ __device__ __forceinline__ A block_sum_any(A v, A* lds) {
   return r;
 }
 
+// Workgroups are dispatched round-robin over the 8 XCDs of the chip (workgroup b runs on
+// XCD b % 8), and each XCD has its own L2. Tile tables are ordered by instance, so giving
+// XCD x the x-th contiguous eighth of a table keeps each instance's tiles, and the slices of
+// the vectors they gather from, in one L2. The map is a bijection of [0, n).
+constexpr int kXcds = 8;
+__device__ __forceinline__ int xcd_tile(int b, int n) {
+  const int x = b % kXcds, k = b / kXcds;
+  const int per = n / kXcds, extra = n % kXcds;
+  return x * per + min(x, extra) + k;
+}
+
 __device__ __forceinline__ int pow2_floor(int v) {
   int p = 1;
   while (p * 2 <= v) p *= 2;

@@ -165,7 +165,7 @@ class EngineImpl final : public Engine {
   DevBuf b_xsum_, b_ysum_, b_xu0_, b_xu1_, b_yu0_, b_yu1_, b_bestx_, b_besty_, b_act_, b_red_;
   DevBuf b_tc_, b_tr_, b_ta_, b_tt_, b_desc_, b_state_;
   DevBuf b_ps_c_, b_ps_a_, b_ps_t_, b_pe_c_, b_pe_r_, b_pe_a_, b_pe_t_;
-  DevBuf b_arrive_step_, b_arrive_eval_, b_done_, b_v_, b_av_, b_len_, b_zero_;
+  DevBuf b_done_, b_v_, b_av_, b_len_, b_zero_;
 
   Args<T> args_{};
   double* d_sa_ = nullptr;
@@ -425,8 +425,6 @@ int EngineImpl<T>::upload() {
   SK_TRY(alloc(b_pe_r_, static_cast<size_t>(n_tr_) * 2, &a.pe_r));
   SK_TRY(alloc(b_pe_a_, static_cast<size_t>(n_ta_) * 8, &a.pe_a));
   SK_TRY(alloc(b_pe_t_, static_cast<size_t>(n_tt_) * 8, &a.pe_t));
-  SK_TRY(alloc(b_arrive_step_, static_cast<size_t>(K_), &a.arrive_step));
-  SK_TRY(alloc(b_arrive_eval_, static_cast<size_t>(K_), &a.arrive_eval));
   SK_TRY(alloc(b_done_, 1, &a.done_count));
   SK_TRY(put(b_v_, v0_, &d_v_));
   SK_TRY(alloc(b_av_, M, &d_av_));
@@ -517,8 +515,6 @@ int EngineImpl<T>::setup_device() {
   uint32_t initially_done = 0;
   for (const auto& x : st) initially_done += static_cast<uint32_t>(x.done);
   SK_HIP(hipMemcpyAsync(a.state, st.data(), st.size() * sizeof(InstState), hipMemcpyHostToDevice, s), kStatusCopyFailed);
-  SK_HIP(hipMemsetAsync(a.arrive_step, 0, static_cast<size_t>(K_) * sizeof(uint32_t), s), kStatusCopyFailed);
-  SK_HIP(hipMemsetAsync(a.arrive_eval, 0, static_cast<size_t>(K_) * sizeof(uint32_t), s), kStatusCopyFailed);
   host_done_[2] = initially_done;
   SK_HIP(hipMemcpyAsync(a.done_count, &host_done_[2], sizeof(uint32_t), hipMemcpyHostToDevice, s), kStatusCopyFailed);
   SK_LAUNCH(k_init_iterates<T><<<blocks_for(MN), kBlock, 0, s>>>(a, d_lhat64_, d_uhat64_, N, M));
@@ -534,6 +530,7 @@ int EngineImpl<T>::enqueue_eval(int report_mode) {
   if (!report_mode) SK_LAUNCH(k_eval_prepare<T><<<n_tc_ + n_tr_, kBlock, 0, s>>>(a));
   SK_LAUNCH(k_eval_rows<T><<<n_ta_, kBlock, 0, s>>>(a));
   SK_LAUNCH(k_eval_cols<T><<<n_tt_, kBlock, 0, s>>>(a));
+  SK_LAUNCH(k_eval_finalize<T><<<K_, kBlock, 0, s>>>(a));
   if (!report_mode) SK_LAUNCH(k_eval_commit<T><<<n_tc_ + n_tr_, kBlock, 0, s>>>(a));
   return kStatusSuccess;
 }
@@ -551,6 +548,7 @@ int EngineImpl<T>::enqueue_block() {
       SK_LAUNCH(k_step_dual<T, false><<<n_ta_, kBlock, 0, s>>>(a));
       SK_LAUNCH(k_step_aty<T, false><<<n_tt_, kBlock, 0, s>>>(a));
     }
+    SK_LAUNCH(k_step_finalize<T><<<K_, kBlock, 0, s>>>(a));
   }
   return enqueue_eval(0);
 }

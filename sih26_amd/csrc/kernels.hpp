@@ -3,8 +3,8 @@
 // One iteration of the reference loop (src/pdhg/pdhg.cpp:383-496) is three launches:
 //   k_step_primal  columns:   x' = proj(x - tau (c + A^T y)), xbar = 2x' - x       (:392-400)
 //   k_step_dual    rows of A: y' = prox(y + sigma A xbar)                           (:402-408)
-//   k_step_aty     rows of A^T: A^T y', interaction (x'-x)^T (A^T y' - A^T y), and the
-//                  adaptive step-size decision by the last tile of each instance   (:410-496)
+//   k_step_aty     rows of A^T: A^T y', interaction (x'-x)^T (A^T y' - A^T y)      (:410-438)
+//   k_step_finalize one workgroup per instance: the adaptive step-size decision      (:440-503)
 // The carried A^T y replaces the reference's third product A (x'-x) (pdhg.cpp:432):
 // (y'-y)^T A (x'-x) = (x'-x)^T (A^T y' - A^T y), and A^T y' is the next primal step's input.
 //
@@ -12,12 +12,13 @@
 // FP64 evaluation stage (pdhg.cpp:502-643):
 //   k_eval_prepare  unscale current and average candidates; ||x - x_restart||, ||y - y_restart||
 //   k_eval_rows     A x per candidate on the unscaled A (FP64): primal residual, row terms
-//   k_eval_cols     c + A^T y per candidate (FP64): dual residual, objectives; the last tile
-//                   of an instance picks the candidate, tests termination and restarts
+//   k_eval_cols     c + A^T y per candidate (FP64): dual residual, objectives
+//   k_eval_finalize one workgroup per instance: picks the candidate, tests termination,
+//                   restarts
 //   k_eval_commit   copy the chosen candidate into `best`, reset sums on restart
 //
-// Reductions are deterministic: per-tile partials are reduced in a fixed order by the last
-// tile of the instance, so an instance gives bitwise the same trajectory alone or in a batch.
+// Reductions are deterministic: per-tile partials are reduced in a fixed order by one
+// workgroup per instance, so an instance gives bitwise the same trajectory alone or in a batch.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -96,8 +97,6 @@ struct Args {
   double* pe_r;  // eval: 2 per row chunk
   double* pe_a;  // eval: 8 per CSR(A) block
   double* pe_t;  // eval: 8 per CSR(A^T) block
-  uint32_t* arrive_step;
-  uint32_t* arrive_eval;
   uint32_t* done_count;
   int32_t n_tc;    // number of column-chunk tiles (eval kernels cover tc then tr)
   int32_t force;   // timing mode: every instance steps, nothing is decided
@@ -111,7 +110,8 @@ __device__ __forceinline__ bool stepping(const InstState& s) { return !s.done &&
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_step_primal(Args<T> a) {
   __shared__ double red[kBlock / kWave];
-  const Tile t = a.tc[blockIdx.x];
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const Tile t = a.tc[tile];
   const InstState& s = a.state[t.inst];
   if (!a.force && !stepping(s)) return;
   const int p = s.parity;
@@ -132,7 +132,7 @@ __global__ __launch_bounds__(kBlock) void k_step_primal(Args<T> a) {
     d2 += d * d;
   }
   d2 = block_sum(d2, red);
-  if (threadIdx.x == 0) a.ps_c[blockIdx.x] = d2;
+  if (threadIdx.x == 0) a.ps_c[tile] = d2;
 }
 
 template <class T, bool STAGE>
@@ -140,7 +140,8 @@ __global__ __launch_bounds__(kBlock) void k_step_dual(Args<T> a) {
   __shared__ T stage[STAGE ? kMaxTileNnz : 1];
   __shared__ T red_t[kBlock / kWave];
   __shared__ double red[kBlock / kWave];
-  const Tile t = a.ta[blockIdx.x];
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const Tile t = a.ta[tile];
   const InstState& s = a.state[t.inst];
   if (!a.force && !stepping(s)) return;
   const int p = s.parity;
@@ -165,7 +166,7 @@ __global__ __launch_bounds__(kBlock) void k_step_dual(Args<T> a) {
       },
       stage, red_t);
   d2 = block_sum(d2, red);
-  if (threadIdx.x == 0) a.ps_a[blockIdx.x] = d2;
+  if (threadIdx.x == 0) a.ps_a[tile] = d2;
 }
 
 // Reduce `count` per-tile partials starting at tile `base` with the whole workgroup in a
@@ -184,20 +185,6 @@ __device__ __forceinline__ double tile_max(const double* part, int base, int cou
   return block_max(v, red);
 }
 
-// Arrival counter of the "last tile finalises" pattern: returns true in every thread of the
-// workgroup that is the last of the instance's `tiles` to arrive.
-__device__ __forceinline__ bool arrive_last(uint32_t* counter, int tiles, int* lds_flag) {
-  __threadfence();
-  if (threadIdx.x == 0) {
-    const uint32_t prev = atomicAdd(counter, 1u);
-    *lds_flag = prev == static_cast<uint32_t>(tiles - 1);
-  }
-  __syncthreads();
-  const bool last = *lds_flag != 0;
-  if (last) __threadfence();
-  return last;
-}
-
 __device__ __forceinline__ void finish(InstState& s, int status, uint32_t* done_count) {
   s.done = 1;
   s.status = status;
@@ -205,7 +192,7 @@ __device__ __forceinline__ void finish(InstState& s, int status, uint32_t* done_
   atomicAdd(done_count, 1u);
 }
 
-// The adaptive step-size rule and acceptance (pdhg.cpp:410-503), applied by the last tile.
+// The adaptive step-size rule and acceptance (pdhg.cpp:410-503).
 __device__ void decide_step(InstState& s, double dx2, double dy2, double inter_sum, int has_rows,
                             uint32_t* done_count) {
   const double omega = s.omega;
@@ -250,9 +237,9 @@ __global__ __launch_bounds__(kBlock) void k_step_aty(Args<T> a) {
   __shared__ T stage[STAGE ? kMaxTileNnz : 1];
   __shared__ T red_t[kBlock / kWave];
   __shared__ double red[kBlock / kWave];
-  __shared__ int last_flag;
-  const Tile t = a.tt[blockIdx.x];
-  InstState& s = a.state[t.inst];
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const Tile t = a.tt[tile];
+  const InstState& s = a.state[t.inst];
   if (!a.force && !stepping(s)) return;
   const int p = s.parity;
   const T* __restrict__ x = p ? a.x1 : a.x0;
@@ -271,18 +258,22 @@ __global__ __launch_bounds__(kBlock) void k_step_aty(Args<T> a) {
       },
       stage, red_t);
   inter = block_sum(inter, red);
-  if (threadIdx.x == 0) a.ps_t[blockIdx.x] = inter;
-  if (a.force) return;
+  if (threadIdx.x == 0) a.ps_t[tile] = inter;
+}
 
-  const InstDesc d = a.desc[t.inst];
-  if (!arrive_last(&a.arrive_step[t.inst], d.tt1 - d.tt0, &last_flag)) return;
+// One workgroup per instance: reduce the step partials in tile order and decide the step.
+// A separate launch (rather than a last-arriving tile) makes the partials visible across
+// XCDs through the kernel boundary, with no per-tile device-scope fence.
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_step_finalize(Args<T> a) {
+  __shared__ double red[kBlock / kWave];
+  InstState& s = a.state[blockIdx.x];
+  if (!stepping(s)) return;
+  const InstDesc d = a.desc[blockIdx.x];
   const double dx2 = tile_sum(a.ps_c, d.tc0, d.tc1 - d.tc0, 1, 0, red);
   const double dy2 = tile_sum(a.ps_a, d.ta0, d.ta1 - d.ta0, 1, 0, red);
   const double isum = tile_sum(a.ps_t, d.tt0, d.tt1 - d.tt0, 1, 0, red);
-  if (threadIdx.x == 0) {
-    a.arrive_step[t.inst] = 0;
-    decide_step(s, dx2, dy2, isum, d.rows > 0, a.done_count);
-  }
+  if (threadIdx.x == 0) decide_step(s, dx2, dy2, isum, d.rows > 0, a.done_count);
 }
 
 // ---- evaluation kernels (FP64) ---------------------------------------------------------------
@@ -293,8 +284,9 @@ __global__ __launch_bounds__(kBlock) void k_step_aty(Args<T> a) {
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_eval_prepare(Args<T> a) {
   __shared__ double red[kBlock / kWave];
-  const bool is_col = static_cast<int>(blockIdx.x) < a.n_tc;
-  const int tile = is_col ? static_cast<int>(blockIdx.x) : static_cast<int>(blockIdx.x) - a.n_tc;
+  const int b = xcd_tile(blockIdx.x, gridDim.x);
+  const bool is_col = b < a.n_tc;
+  const int tile = is_col ? b : b - a.n_tc;
   const Tile t = is_col ? a.tc[tile] : a.tr[tile];
   const InstState& s = a.state[t.inst];
   if (!s.eval_pending) return;
@@ -332,7 +324,8 @@ template <class T>
 __global__ __launch_bounds__(kBlock) void k_eval_rows(Args<T> a) {
   __shared__ D2 red2[kBlock / kWave];
   __shared__ double red[kBlock / kWave];
-  const Tile t = a.ta[blockIdx.x];
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const Tile t = a.ta[tile];
   const InstState& s = a.state[t.inst];
   if (!(a.report ? s.report_pending : s.eval_pending)) return;
   const double* __restrict__ xa = a.report ? a.bestx : a.xu0;
@@ -376,7 +369,7 @@ __global__ __launch_bounds__(kBlock) void k_eval_rows(Args<T> a) {
       (D2*)nullptr, red2);
   for (int k = 0; k < 8; ++k) {
     const double r = (k & 3) == 3 ? block_max(acc[k], red) : block_sum(acc[k], red);
-    if (threadIdx.x == 0) a.pe_a[(size_t)blockIdx.x * 8 + k] = r;
+    if (threadIdx.x == 0) a.pe_a[(size_t)tile * 8 + k] = r;
   }
 }
 
@@ -464,17 +457,16 @@ __device__ void decide_evaluation(InstState& s, const Residuals& cur, const Resi
 }
 
 // k_eval_cols: reduced cost d = c + A^T y per candidate on the original A and every column
-// term of evaluate() (pdhg.cpp:175-199, 239-257); the last tile of an instance reduces all
-// partials, forms both Residuals and decides. Partials per tile, per candidate c:
+// term of evaluate() (pdhg.cpp:175-199, 239-257). Partials per tile, per candidate c:
 // [4c+0] dual violation from d, [4c+1] bound contribution, [4c+2] complementarity,
 // [4c+3] primal objective.
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_eval_cols(Args<T> a) {
   __shared__ D2 red2[kBlock / kWave];
   __shared__ double red[kBlock / kWave];
-  __shared__ int last_flag;
-  const Tile t = a.tt[blockIdx.x];
-  InstState& s = a.state[t.inst];
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const Tile t = a.tt[tile];
+  const InstState& s = a.state[t.inst];
   if (!(a.report ? s.report_pending : s.eval_pending)) return;
   const double* __restrict__ xa = a.report ? a.bestx : a.xu0;
   const double* __restrict__ xb = a.report ? a.bestx : a.xu1;
@@ -515,11 +507,19 @@ __global__ __launch_bounds__(kBlock) void k_eval_cols(Args<T> a) {
       (D2*)nullptr, red2);
   for (int k = 0; k < 8; ++k) {
     const double r = (k & 3) == 2 ? block_max(acc[k], red) : block_sum(acc[k], red);
-    if (threadIdx.x == 0) a.pe_t[(size_t)blockIdx.x * 8 + k] = r;
+    if (threadIdx.x == 0) a.pe_t[(size_t)tile * 8 + k] = r;
   }
 
-  const InstDesc d = a.desc[t.inst];
-  if (!arrive_last(&a.arrive_eval[t.inst], d.tt1 - d.tt0, &last_flag)) return;
+}
+
+// One workgroup per evaluated instance: reduce every evaluation partial in tile order, form
+// both Residuals and decide (or store the report).
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_eval_finalize(Args<T> a) {
+  __shared__ double red[kBlock / kWave];
+  InstState& s = a.state[blockIdx.x];
+  if (!(a.report ? s.report_pending : s.eval_pending)) return;
+  const InstDesc d = a.desc[blockIdx.x];
   double v[18];
   for (int c = 0; c < 2; ++c) {
     v[8 * c + 0] = tile_sum(a.pe_a, d.ta0, d.ta1 - d.ta0, 8, 4 * c + 0, red);  // viol^2
@@ -534,7 +534,6 @@ __global__ __launch_bounds__(kBlock) void k_eval_cols(Args<T> a) {
   v[16] = a.report ? 0.0 : tile_sum(a.pe_c, d.tc0, d.tc1 - d.tc0, 2, 0, red);
   v[17] = a.report ? 0.0 : tile_sum(a.pe_r, d.tr0, d.tr1 - d.tr0, 2, 0, red);
   if (threadIdx.x != 0) return;
-  a.arrive_eval[t.inst] = 0;
   Residuals r[2];
   for (int c = 0; c < 2; ++c) {
     const double* q = v + 8 * c;
@@ -552,8 +551,9 @@ __global__ __launch_bounds__(kBlock) void k_eval_cols(Args<T> a) {
 // Idempotent, so an instance that stopped may see it again without effect.
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_eval_commit(Args<T> a) {
-  const bool is_col = static_cast<int>(blockIdx.x) < a.n_tc;
-  const int tile = is_col ? static_cast<int>(blockIdx.x) : static_cast<int>(blockIdx.x) - a.n_tc;
+  const int b = xcd_tile(blockIdx.x, gridDim.x);
+  const bool is_col = b < a.n_tc;
+  const int tile = is_col ? b : b - a.n_tc;
   const Tile t = is_col ? a.tc[tile] : a.tr[tile];
   const InstState& s = a.state[t.inst];
   if (!s.copy_best && !s.do_restart) return;

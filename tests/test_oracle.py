@@ -1,0 +1,291 @@
+"""Pins for the FP64 CPU oracle (oracle/sk_oracle.c), CPU only.
+
+Each restated function is checked against something it does not share code with: the
+reference's own compiled scaling/sparse code (golden vectors in tests/golden/ref_scaling.npz,
+written by tests/golden/make_golden.py from oracle/_ref), the reference's known-answer tests,
+brute force, closed-form optima, published Netlib optima, and invariants.
+"""
+import ctypes as C
+import json
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from oracle import oracle
+from sih26_amd.mps import read_mps
+from tests.lp_cases import CLOSED_FORM, closed_form, csc_to_dense, dense_to_lp, kkt_lp
+
+GOLDEN = Path(__file__).resolve().parent / "golden"
+L = oracle.lib()
+
+
+def golden_cases():
+    z = np.load(GOLDEN / "ref_scaling.npz")
+    names = sorted({k.split("/")[0] for k in z.files})
+    out = {}
+    for name in names:
+        m, n, sense = (int(v) for v in z[f"{name}/in/dims"])
+        lp = {"num_rows": m, "num_cols": n, "sense": sense, "offset": 0.0}
+        for k in ("col_starts", "row_indices", "values", "cost", "col_lower", "col_upper",
+                  "row_lower", "row_upper"):
+            lp[k] = z[f"{name}/in/{k}"]
+        out[name] = (lp, {k.split("/", 1)[1]: z[k] for k in z.files if k.startswith(name + "/")})
+    return out
+
+
+GOLD = golden_cases()
+
+
+def bits(a):
+    return np.ascontiguousarray(a, np.float64).view(np.uint64)
+
+
+# ---- known-answer tests of the reference (tests/unit/test_gpu.cpp) ---------------------------
+
+def test_axpy_kat():  # test_gpu.cpp:86-97
+    x = np.array([1.0, 2.0, -3.0, 4.5])
+    y = np.array([0.5, -1.0, 2.0, 0.0])
+    L.ok_cpu_axpy(4, 2.0, x.ctypes.data, y.ctypes.data)
+    np.testing.assert_array_equal(y, [2.5, 3.0, -4.0, 9.0])
+
+
+def test_dot_kat():  # test_gpu.cpp:99-105
+    x = np.array([1.0, 2.0, -3.0, 4.0])
+    y = np.array([2.0, -1.0, 0.5, 3.0])
+    assert L.ok_cpu_dot(4, x.ctypes.data, y.ctypes.data) == 10.5
+
+
+def primal_step(n, tau, x, c, aty, lo, hi):
+    xn, xb = np.zeros(n), np.zeros(n)
+    args = [np.ascontiguousarray(v, float) for v in (x, c, aty, lo, hi)]
+    L.ok_cpu_primal_step(n, tau, *[a.ctypes.data for a in args], xn.ctypes.data, xb.ctypes.data)
+    return xn, xb
+
+
+def test_primal_step_kat():  # test_gpu.cpp:107-140
+    inf = float("inf")
+    xn, xb = primal_step(4, 0.5, [2.0, 0.0, -4.0, 0.5], [1.0, -2.0, 2.0, 3.0],
+                         [0.0, 1.0, 0.0, 0.0], [-1.0, 0.0, -inf, 0.0], [1.0, 2.0, inf, inf])
+    np.testing.assert_array_equal(xn, [1.0, 0.5, -5.0, 0.0])
+    np.testing.assert_array_equal(xb, [0.0, 1.0, -6.0, -0.5])
+
+
+def test_primal_step_cpu_dispatch_kat():  # test_gpu.cpp:222-243
+    xn, _ = primal_step(2, 0.1, [1.0, 2.0], [0.5, -0.5], [0.0, 0.0], [-10.0, -10.0], [10.0, 10.0])
+    np.testing.assert_allclose(xn, [0.95, 2.05], atol=1e-12)
+
+
+def test_projection_treats_1e30_as_infinite():  # types.hpp:32-39, pdhg.cpp:63-67
+    assert L.ok_project(5e30, -1e30, 1e30) == 5e30
+    assert L.ok_project(-5.0, -1e29, 1e30) == -5.0
+    assert L.ok_project(-5.0, -1.0, 1e30) == -1.0
+    assert L.ok_is_finite_bound(1e29) == 1 and L.ok_is_finite_bound(-1e30) == 0
+
+
+def test_spmv_csc_kat_both_orientations():  # test_gpu.cpp:142-177
+    cs = np.array([0, 1, 2, 4], np.int32)
+    ri = np.array([0, 1, 0, 1], np.int32)
+    v = np.array([1.0, 3.0, 2.0, 4.0])
+    x = np.array([1.0, 2.0, 3.0])
+    y = np.zeros(2)
+    L.ok_cpu_spmv_csc(2, 3, cs.ctypes.data, ri.ctypes.data, v.ctypes.data, x.ctypes.data, y.ctypes.data, 1.0, 0)
+    np.testing.assert_array_equal(y, [7.0, 18.0])
+    x2 = np.array([2.0, 5.0])
+    y2 = np.zeros(3)
+    L.ok_cpu_spmv_csc(2, 3, cs.ctypes.data, ri.ctypes.data, v.ctypes.data, x2.ctypes.data, y2.ctypes.data, 1.0, 1)
+    np.testing.assert_array_equal(y2, [2.0, 15.0, 24.0])
+
+
+def compare(a, b, tol):
+    a, b = np.asarray(a, float), np.asarray(b, float)
+    ma, mr, nf = C.c_double(), C.c_double(), C.c_int32()
+    p = L.ok_compare_vectors(len(a), a.ctypes.data, b.ctypes.data, tol, C.byref(ma), C.byref(mr), C.byref(nf))
+    return p, ma.value, mr.value, nf.value
+
+
+def test_compare_vectors_kats():  # test_gpu.cpp:183-216
+    p, ma, _, nf = compare([1.0, 2.0, 3.0], [1.0, 2.00000001, 2.99999999], 1e-7)
+    assert p == 1 and nf == 0 and ma <= 1e-7
+    p, ma, _, _ = compare([1.0, 2.0, 3.0], [1.0, 2.05, 3.0], 1e-4)
+    assert p == 0 and abs(ma - 0.05) < 1e-6
+    assert compare([1.0, float("nan"), 3.0], [1.0, 2.0, 3.0], 1e-4)[3] == 1
+    assert compare([1.0, float("inf"), 3.0], [1.0, 2.0, 3.0], 1e-4)[3] == 1
+    # relative branch: |diff| > tol but relative <= tol passes
+    assert compare([1e6], [1e6 + 1e-2], 1e-7)[0] == 1
+
+
+# ---- sparse products --------------------------------------------------------------------------
+
+@pytest.mark.parametrize("name", sorted(GOLD))
+def test_products_bitwise_against_reference_build(name):
+    lp, g = GOLD[name]
+    np.testing.assert_array_equal(bits(oracle.multiply(lp, g["in/x"])), bits(g["ax"]))
+    np.testing.assert_array_equal(bits(oracle.multiply(lp, g["in/y"], transpose=True)), bits(g["aty"]))
+
+
+def test_products_against_dense_brute_force():
+    rng = np.random.default_rng(9)
+    a = np.where(rng.random((30, 45)) < 0.2, rng.standard_normal((30, 45)), 0.0)
+    lp = dense_to_lp(a, np.zeros(30), np.ones(30), np.ones(45))
+    x, y = rng.standard_normal(45), rng.standard_normal(30)
+    np.testing.assert_allclose(oracle.multiply(lp, x), a @ x, rtol=1e-13, atol=1e-13)
+    np.testing.assert_allclose(oracle.multiply(lp, y, transpose=True), a.T @ y, rtol=1e-13, atol=1e-13)
+
+
+# ---- scaling and the power iteration ------------------------------------------------------------
+
+@pytest.mark.parametrize("name", sorted(GOLD))
+def test_scaling_bitwise_against_reference_build(name):
+    lp, g = GOLD[name]
+    s = oracle.build_scaling(lp)
+    for k in ("row", "column", "values", "cost", "col_lower", "col_upper", "row_lower", "row_upper"):
+        np.testing.assert_array_equal(bits(s[k]), bits(g[f"scaling/{k}"]), err_msg=k)
+    assert (s["min_abs"], s["max_abs"]) == tuple(g["scaling/minmax"])
+
+
+@pytest.mark.parametrize("name", sorted(GOLD))
+def test_spectral_norm_bitwise_against_reference_build(name):
+    lp, g = GOLD[name]
+    s = oracle.build_scaling(lp)
+    assert oracle.spectral_norm(lp, s["values"], seed=1) == g["norm_seed1"][0]
+
+
+@pytest.mark.parametrize("name", sorted(GOLD))
+def test_spectral_norm_bounds_sigma_max(name):
+    lp, _ = GOLD[name]
+    s = oracle.build_scaling(lp)
+    scaled = dict(lp, values=s["values"])
+    sigma = np.linalg.svd(csc_to_dense(scaled), compute_uv=False)[0]
+    est = oracle.spectral_norm(lp, s["values"], seed=1)
+    # power iteration from below, then rounded up by 1.01 (scaling.cpp:190-193)
+    assert est <= 1.01 * sigma * (1 + 1e-12)
+    assert est >= 0.9 * sigma
+
+
+def test_ruiz_equilibrates_rows_and_columns():
+    lp, _ = GOLD["random_b"]
+    s = oracle.build_scaling(lp, ruiz=30)
+    a = np.abs(csc_to_dense(dict(lp, values=s["values"])))
+    rows = a.max(axis=1)
+    cols = a.max(axis=0)
+    # the Pock-Chambolle pass follows Ruiz, so norms are near (not at) one; empty stay empty
+    assert np.all((rows == 0) | ((rows > 0.2) & (rows < 5)))
+    assert np.all((cols == 0) | ((cols > 0.2) & (cols < 5)))
+
+
+def test_scaled_problem_is_the_same_problem():
+    # residuals measured on the unscaled model are invariant: x = Dc xhat, y = Dr yhat
+    lp, _ = GOLD["c1_seed1"]
+    s = oracle.build_scaling(lp)
+    rng = np.random.default_rng(4)
+    xh, yh = rng.random(lp["num_cols"]), rng.standard_normal(lp["num_rows"])
+    a = csc_to_dense(lp)
+    ah = csc_to_dense(dict(lp, values=s["values"]))
+    np.testing.assert_allclose(np.diag(s["row"]) @ a @ np.diag(s["column"]), ah, rtol=1e-12, atol=1e-15)
+    np.testing.assert_allclose(ah @ xh, s["row"] * (a @ (s["column"] * xh)), rtol=1e-11, atol=1e-12)
+
+
+def test_mt19937_matches_the_standard():
+    # [rand.predef]: the 10000th output of a default-constructed mt19937 is 4123659995
+    out = np.zeros(10000, np.uint32)
+    L.ok_mt19937_raw(5489, 10000, out.ctypes.data)
+    assert int(out[-1]) == 4123659995
+
+
+# ---- evaluate() and the engine ------------------------------------------------------------------
+
+def test_evaluate_closed_form_terms():
+    # min x + 2y  s.t. x + y = 4 (E), 0 <= x, y; at x = (4, 0), y_row = -1 (engine sign):
+    # activity 4 -> primal 0; d = c + A^T y = (0, 1): bound contribution 0*0 + 1*0 = 0,
+    # support: y < 0 prices row_lower 4 -> -4; dual objective = 0 - (-4) = 4 = primal objective
+    d, _, _ = closed_form("equality")
+    ev = oracle.evaluate(d, np.array([4.0, 0.0]), np.array([-1.0]))
+    assert ev["absolute_primal"] == 0.0 and ev["absolute_dual"] == 0.0
+    assert ev["primal_objective"] == 4.0 and ev["dual_objective"] == 4.0
+    assert ev["gap"] == 0.0 and ev["complementarity"] == 0.0
+    # a violated point: x = (1, 0): violation 3 -> primal = 3 / (1 + ||b||) = 3 / 5
+    ev = oracle.evaluate(d, np.array([1.0, 0.0]), np.array([0.0]))
+    assert ev["absolute_primal"] == 3.0 and abs(ev["primal"] - 0.6) < 1e-15
+    # y > 0 on an equality prices row_upper; d_j < 0 with an infinite upper bound is a violation
+    ev = oracle.evaluate(d, np.array([4.0, 0.0]), np.array([2.0]))
+    np.testing.assert_allclose(ev["reduced"], [3.0, 4.0])
+    assert ev["absolute_dual"] == 0.0 and ev["dual_objective"] == 0.0 - 8.0
+
+
+@pytest.mark.parametrize("name", [c[0] for c in CLOSED_FORM])
+def test_oracle_solves_closed_form_lps(name):  # test_pdhg.cpp:95-146
+    d, optimum, xs = closed_form(name)
+    r = oracle.solve_pdhg(d, tolerance=1e-8, iteration_limit=200000)
+    assert r["status"] == 1
+    assert abs(r["objective"] - optimum) <= 1e-6 * max(1.0, abs(optimum))
+    if xs is not None:
+        np.testing.assert_allclose(r["col_value"], xs, atol=1e-4)
+
+
+def test_oracle_iteration_limit_is_honest():  # test_pdhg.cpp:148-157
+    d, _, _ = closed_form("sum")
+    r = oracle.solve_pdhg(d, tolerance=1e-12, iteration_limit=20)
+    assert r["status"] == 6 and np.isinf(r["dual_bound"]) and r["iterations"] == 20
+
+
+def test_oracle_step_size_survives_first_iteration():  # test_pdhg.cpp:248-263
+    d, _, _ = closed_form("sum")
+    r = oracle.solve_pdhg(d, tolerance=1e-8)
+    assert r["status"] == 1 and r["iterations"] < 500
+
+
+def test_oracle_kkt_backwards_instances():  # the idea of test_pdhg.cpp:178-222
+    rng = np.random.default_rng(20260904)
+    compared = 0
+    for _ in range(60):
+        d, opt = kkt_lp(rng, int(rng.integers(2, 7)), int(rng.integers(2, 7)))
+        r = oracle.solve_pdhg(d, tolerance=1e-9, iteration_limit=200000)
+        if r["status"] != 1:
+            continue
+        compared += 1
+        assert abs(r["objective"] - opt) <= 1e-6 * max(1.0, abs(opt))
+    assert compared > 30
+
+
+NETLIB = json.loads((GOLDEN / "netlib" / "optima.json").read_text())["instances"]
+
+
+@pytest.mark.parametrize("name", ["afiro", "sc50a", "sc50b", "blend"])
+def test_oracle_reaches_published_netlib_optima(name):
+    d = read_mps(GOLDEN / "netlib" / f"{name}.mps")
+    assert d["num_cols"] == NETLIB[name]["cols"]
+    assert d["num_rows"] == NETLIB[name]["rows"] - 1  # the published count includes the N row
+    r = oracle.solve_pdhg(d, tolerance=1e-8, iteration_limit=200000)
+    p = NETLIB[name]["published_optimal"]
+    assert r["status"] == 1
+    assert abs(r["objective"] - p) <= 2e-8 * abs(p)
+
+
+def test_stop_at_request_keeps_the_absolute_primal_clause():
+    # pdhg.cpp:119-122: with the clause a 1e-4 request still needs absolute primal <= 1e-7
+    d, _ = GOLD["c1_seed1"]
+    cheap = oracle.solve_pdhg(d, tolerance=1e-4, stop_at_request=True, absolute_primal=False)
+    honest = oracle.solve_pdhg(d, tolerance=1e-4, stop_at_request=True, iteration_limit=20000)
+    assert cheap["status"] == 5 and cheap["final"]["primal"] <= 1e-4
+    assert cheap["iterations"] < honest["iterations"]
+
+
+def test_recompute_quality_by_hand():
+    # model.cpp:228-422 on a fabricated point: x = (5, 0) for x + y >= 2 (row), d = (1, 1)
+    d, _, _ = closed_form("sum")
+    mdl, a = oracle.model(d)
+    x, yd, cd, act = np.array([5.0, -0.5]), np.array([0.25]), np.array([1.0, 1.0]), np.zeros(1)
+    sol = oracle.Solution()
+    sol.col_value, sol.row_dual, sol.col_dual, sol.row_activity = (x.ctypes.data, yd.ctypes.data,
+                                                                   cd.ctypes.data, act.ctypes.data)
+    sol.dual_bound = 2.0
+    L.ok_recompute_quality(C.byref(mdl), C.byref(sol))
+    assert act[0] == 4.5
+    assert sol.primal_infeasibility == 0.5  # column 1 below its lower bound 0 by 0.5
+    assert sol.objective == 4.5
+    # implied reduced cost c - A^T y = (0.75, 0.75) vs reported (1, 1): consistency 0.25
+    assert abs(sol.dual_infeasibility - 0.25) < 1e-15
+    # complementarity: |d0| * distance(x0, 0) = 1 * 5; row: |y| * (4.5 - 2) = 0.625
+    assert sol.complementarity_violation == 5.0
+    assert sol.absolute_gap == 2.5 and sol.relative_gap == 2.5 / 4.5
