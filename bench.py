@@ -34,10 +34,10 @@ WORKLOADS = {
 KERNEL_NAMES = {0: "k_step_primal", 1: "k_step_dual", 2: "k_step_aty"}
 
 
-def solve_options(sk, precision: str = "fp32"):
+def solve_options(sk, precision: str = "fp32", **extra):
     return sk.make_options(pdhg_tolerance=1e-4, pdhg_stop_at_request=True,
                            pdhg_absolute_primal=False, pdhg_precision=precision,
-                           compute_quality=0)
+                           compute_quality=0, **extra)
 
 
 def instances(sk, workload: str, count: int):
@@ -105,6 +105,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--kernel-reps", type=int, default=50)
+    ap.add_argument("--spmv-stage", type=int, default=-1, help="-1 auto, 0 direct, 1 LDS-staged")
+    ap.add_argument("--tile-nnz", type=int, default=0, help="row-block capacity, 0 auto")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -118,7 +120,7 @@ def main() -> int:
         return 1
 
     lps = instances(sk, args.workload, args.instances)
-    batch = sk.Batch(solve_options(sk))
+    batch = sk.Batch(solve_options(sk, spmv_stage=args.spmv_stage, tile_nnz=args.tile_nnz))
     for lp in lps:
         batch.add(lp)
     batch.upload()

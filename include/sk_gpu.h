@@ -181,6 +181,9 @@ typedef struct {
   int32_t spmv_stage;        /* SpMV schedule: -1 autotune (default), 0 direct, 1 LDS-staged */
   int32_t tile_nnz;          /* row-block capacity in nonzeros, 0 = automatic */
   int32_t compute_quality;   /* 1: fill the recompute_quality measures (default) */
+  int32_t eval_stride;       /* an evaluation stage follows every this many steps inside a
+                                captured block, so an instance due for evaluation waits at most
+                                this many steps (default 10) */
 } sk_pdhg_options;
 
 void sk_pdhg_default_options(sk_pdhg_options* options);

@@ -60,7 +60,7 @@ enum : int32_t {
 };
 
 // Mutable per-instance solver state; lives in device memory for the whole solve and is
-// written only by the single "last tile" of an instance in the step and evaluation kernels.
+// written only by the per-instance finalize kernels (and the set-up kernels).
 struct InstState {
   // step control (pdhg.cpp:368-373)
   double eta;
@@ -99,7 +99,8 @@ struct InstState {
   int32_t do_restart;       // commit stage resets sums and restart point
   int32_t report_pending;   // final evaluation of `best` still to run
   int32_t nonfinite;        // a non-finite step quantity was seen
-  int32_t pad0, pad1;
+  int32_t norm_trivial;     // no rows, columns or nonzeros: spectral norm is 1 by definition
+  int32_t pad1;
   Residuals current;
   Residuals average;
   Residuals best;

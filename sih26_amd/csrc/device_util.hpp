@@ -121,13 +121,38 @@ __device__ __forceinline__ void rowblock_product(const Tile t, const int32_t* __
   const int nnz = p1 - p0;
   if (nrows == 1 && nnz > t.cap) {
     A acc = zero_of<A>();
-    for (int p = p0 + tid; p < p1; p += kBlock) acc = acc + prod(p);
+    constexpr int U = 4;
+    for (int base = p0; base < p1; base += U * kBlock) {
+      A part[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int p = base + u * kBlock + tid;
+        part[u] = p < p1 ? prod(p) : zero_of<A>();
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) acc = acc + part[u];
+    }
     acc = block_sum_any(acc, red_lds);
     if (tid == 0) epi(r0, acc);
     return;
   }
   if (STAGE) {
-    for (int k = tid; k < nnz; k += kBlock) stage[k] = prod(p0 + k);
+    // Unrolled batches: every lane issues all its loads of a batch before any is consumed,
+    // so each wavefront keeps U gathers in flight instead of one.
+    constexpr int U = 8;
+    for (int base = 0; base < nnz; base += U * kBlock) {
+      A part[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int k = base + u * kBlock + tid;
+        if (k < nnz) part[u] = prod(p0 + k);
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int k = base + u * kBlock + tid;
+        if (k < nnz) stage[k] = part[u];
+      }
+    }
     __syncthreads();
   }
   const int g = nrows > 0 ? min(kWave, pow2_floor(kBlock / nrows)) : kWave;
@@ -138,7 +163,17 @@ __device__ __forceinline__ void rowblock_product(const Tile t, const int32_t* __
     if (STAGE) {
       for (int k = b - p0 + lane; k < e - p0; k += g) acc = acc + stage[k];
     } else {
-      for (int p = b + lane; p < e; p += g) acc = acc + prod(p);
+      constexpr int U = 4;
+      for (int base = b + lane; base < e; base += U * g) {
+        A part[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const int p = base + u * g;
+          part[u] = p < e ? prod(p) : zero_of<A>();
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) acc = acc + part[u];
+      }
     }
   }
   acc = group_sum(acc, g);

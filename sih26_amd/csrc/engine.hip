@@ -93,6 +93,7 @@ class EngineImpl final : public Engine {
   explicit EngineImpl(const EngineOptions& o) : opt_(o) {}
   ~EngineImpl() override {
     if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
+    if (setup_exec_) (void)hipGraphExecDestroy(setup_exec_);
     if (stream_) (void)hipStreamDestroy(stream_);
     if (host_done_) (void)hipHostFree(host_done_);
   }
@@ -133,6 +134,8 @@ class EngineImpl final : public Engine {
   }
 
   int setup_device();
+  int enqueue_setup();
+  int capture(int (EngineImpl<T>::*body)(), hipGraphExec_t* exec);
   int run_loop();
   int report();
   int enqueue_block();
@@ -151,9 +154,11 @@ class EngineImpl final : public Engine {
   std::vector<double> v0_;  // power-iteration start, all instances
   std::vector<uint8_t> trivial_norm_;
   int32_t n_ta_ = 0, n_tt_ = 0, n_tc_ = 0, n_tr_ = 0;
+  uint32_t initially_done_ = 0;
 
   hipStream_t stream_ = nullptr;
   hipGraphExec_t graph_exec_ = nullptr;
+  hipGraphExec_t setup_exec_ = nullptr;
   uint32_t* host_done_ = nullptr;
 
   // device buffers
@@ -165,7 +170,7 @@ class EngineImpl final : public Engine {
   DevBuf b_xsum_, b_ysum_, b_xu0_, b_xu1_, b_yu0_, b_yu1_, b_bestx_, b_besty_, b_act_, b_red_;
   DevBuf b_tc_, b_tr_, b_ta_, b_tt_, b_desc_, b_state_;
   DevBuf b_ps_c_, b_ps_a_, b_ps_t_, b_pe_c_, b_pe_r_, b_pe_a_, b_pe_t_;
-  DevBuf b_done_, b_v_, b_av_, b_len_, b_zero_;
+  DevBuf b_done_, b_v_, b_v0_, b_av_, b_len_, b_zero_;
 
   Args<T> args_{};
   double* d_sa_ = nullptr;
@@ -175,11 +180,12 @@ class EngineImpl final : public Engine {
   double* d_lhat64_ = nullptr;
   double* d_uhat64_ = nullptr;
   double* d_v_ = nullptr;
+  double* d_v0_ = nullptr;
   double* d_av_ = nullptr;
   double* d_len_ = nullptr;
   double* d_ones_m_ = nullptr;
   double* d_ones_n_ = nullptr;
-  uint8_t* d_zero_seen_ = nullptr;
+  int32_t* d_zero_seen_ = nullptr;
 
   // results
   std::vector<InstState> final_state_;
@@ -191,6 +197,7 @@ template <class T>
 int EngineImpl<T>::upload() {
   K_ = static_cast<int32_t>(lps_.size());
   m_ = n_ = nnz_ = 0;
+  initially_done_ = 0;
   for (size_t k = 0; k < lps_.size(); ++k) {
     const HostLp* lp = lps_[k];
     const std::string problem = lp->validate();
@@ -257,6 +264,7 @@ int EngineImpl<T>::upload() {
       for (int32_t j = 0; j < n; ++j) v0_[static_cast<size_t>(col0 + j)] = spread(rng);
     } else {
       trivial_norm_[static_cast<size_t>(k)] = 1;
+      init_state_[static_cast<size_t>(k)].norm_trivial = 1;
     }
     InstDesc& d = desc_[static_cast<size_t>(k)];
     d.row0 = row0;
@@ -275,6 +283,12 @@ int EngineImpl<T>::upload() {
     s.stop_at_request = opt_.stop_at_request != 0;
     s.absolute_clause = opt_.absolute_clause != 0;
     s.best.primal = s.best.dual = s.best.gap = INFINITY;
+    if (limit <= 0) {  // pdhg.cpp:384: the loop stops before its first step
+      s.done = 1;
+      s.status = kStIterationLimit;
+      s.report_pending = 1;
+      ++initially_done_;
+    }
     row0 += m;
     col0 += n;
     nz0 += static_cast<int32_t>(lp.nnz());
@@ -426,7 +440,8 @@ int EngineImpl<T>::upload() {
   SK_TRY(alloc(b_pe_a_, static_cast<size_t>(n_ta_) * 8, &a.pe_a));
   SK_TRY(alloc(b_pe_t_, static_cast<size_t>(n_tt_) * 8, &a.pe_t));
   SK_TRY(alloc(b_done_, 1, &a.done_count));
-  SK_TRY(put(b_v_, v0_, &d_v_));
+  SK_TRY(put(b_v0_, v0_, &d_v0_));
+  SK_TRY(alloc(b_v_, N, &d_v_));
   SK_TRY(alloc(b_av_, M, &d_av_));
   SK_TRY(alloc(b_len_, static_cast<size_t>(K_), &d_len_));
   SK_TRY(alloc(b_zero_, static_cast<size_t>(K_), &d_zero_seen_));
@@ -437,23 +452,26 @@ int EngineImpl<T>::upload() {
   return kStatusSuccess;
 }
 
+// Device set-up of one solve, enqueued on the engine's stream (captured once as a graph):
+// scaling (scaling.cpp:111-159), scaled vectors, the power iteration (scaling.cpp:161-194),
+// the initial step sizes and iterates (pdhg.cpp:327-379).
 template <class T>
-int EngineImpl<T>::setup_device() {
+int EngineImpl<T>::enqueue_setup() {
   Args<T>& a = args_;
   hipStream_t s = stream_;
   const auto M = static_cast<int32_t>(m_), N = static_cast<int32_t>(n_);
-  const auto Z = static_cast<size_t>(nnz_);
-  SK_HIP(hipMemcpyAsync(d_sa_, a.a_orig, Z * sizeof(double), hipMemcpyDeviceToDevice, s), kStatusCopyFailed);
-  SK_HIP(hipMemcpyAsync(d_st_, a.t_orig, Z * sizeof(double), hipMemcpyDeviceToDevice, s), kStatusCopyFailed);
-  SK_HIP(hipMemcpyAsync(const_cast<double*>(a.dr), d_ones_m_, static_cast<size_t>(M) * sizeof(double), hipMemcpyDeviceToDevice, s), kStatusCopyFailed);
-  SK_HIP(hipMemcpyAsync(const_cast<double*>(a.dc), d_ones_n_, static_cast<size_t>(N) * sizeof(double), hipMemcpyDeviceToDevice, s), kStatusCopyFailed);
+  const auto Z = static_cast<int64_t>(nnz_);
+  SK_LAUNCH(k_copy<<<blocks_for(Z), kBlock, 0, s>>>(a.a_orig, d_sa_, Z));
+  SK_LAUNCH(k_copy<<<blocks_for(Z), kBlock, 0, s>>>(a.t_orig, d_st_, Z));
+  SK_LAUNCH(k_fill<<<blocks_for(M), kBlock, 0, s>>>(const_cast<double*>(a.dr), 1.0, M));
+  SK_LAUNCH(k_fill<<<blocks_for(N), kBlock, 0, s>>>(const_cast<double*>(a.dc), 1.0, N));
   // 10 Ruiz rounds, then one Pock-Chambolle round (scaling.cpp:121-122)
   for (int pass = 0; pass < 11; ++pass) {
     const int use_sum = pass == 10;
     SK_LAUNCH(k_row_pass<<<blocks_for(M), kBlock, 0, s>>>(a.a_ptr, d_sa_, M, use_sum, d_rpass_));
     SK_LAUNCH(k_row_pass<<<blocks_for(N), kBlock, 0, s>>>(a.t_ptr, d_st_, N, use_sum, d_cpass_));
-    SK_LAUNCH(k_apply_pass<<<blocks_for(M), kBlock, 0, s>>>(a.a_ptr, a.a_idx, d_sa_, M, 1, d_rpass_, d_cpass_));
-    SK_LAUNCH(k_apply_pass<<<blocks_for(N), kBlock, 0, s>>>(a.t_ptr, a.t_idx, d_st_, N, 0, d_rpass_, d_cpass_));
+    SK_LAUNCH(k_apply_pass<<<n_ta_, kBlock, 0, s>>>(a.ta, a.a_ptr, a.a_idx, d_sa_, 1, d_rpass_, d_cpass_));
+    SK_LAUNCH(k_apply_pass<<<n_tt_, kBlock, 0, s>>>(a.tt, a.t_ptr, a.t_idx, d_st_, 0, d_rpass_, d_cpass_));
     SK_LAUNCH(k_scale_mul<<<blocks_for(M), kBlock, 0, s>>>(const_cast<double*>(a.dr), d_rpass_, M));
     SK_LAUNCH(k_scale_mul<<<blocks_for(N), kBlock, 0, s>>>(const_cast<double*>(a.dc), d_cpass_, N));
   }
@@ -475,50 +493,70 @@ int EngineImpl<T>::setup_device() {
         a64, a.xu0, d_lhat64_, d_uhat64_, a.yu0, a.yu1, N, 0));
   }
   if (sizeof(T) != sizeof(double)) {
-    SK_LAUNCH(k_convert<T><<<blocks_for(static_cast<int64_t>(Z)), kBlock, 0, s>>>(d_sa_, const_cast<T*>(a.a_val), static_cast<int64_t>(Z)));
-    SK_LAUNCH(k_convert<T><<<blocks_for(static_cast<int64_t>(Z)), kBlock, 0, s>>>(d_st_, const_cast<T*>(a.t_val), static_cast<int64_t>(Z)));
+    SK_LAUNCH(k_convert<T><<<blocks_for(Z), kBlock, 0, s>>>(d_sa_, const_cast<T*>(a.a_val), Z));
+    SK_LAUNCH(k_convert<T><<<blocks_for(Z), kBlock, 0, s>>>(d_st_, const_cast<T*>(a.t_val), Z));
   }
   // power iteration on A^T A, 30 steps (scaling.cpp:161-194)
-  SK_HIP(hipMemcpyAsync(d_v_, v0_.data(), static_cast<size_t>(N) * sizeof(double), hipMemcpyHostToDevice, s), kStatusCopyFailed);
-  SK_HIP(hipMemsetAsync(d_zero_seen_, 0, static_cast<size_t>(K_), s), kStatusCopyFailed);
-  std::vector<double> lens(static_cast<size_t>(K_) * 31);
-  for (int step = 0; step <= 30; ++step) {
+  SK_LAUNCH(k_copy<<<blocks_for(N), kBlock, 0, s>>>(d_v0_, d_v_, N));
+  SK_LAUNCH(k_fill_i32<<<blocks_for(K_), kBlock, 0, s>>>(d_zero_seen_, 0, K_));
+  for (int step = 0; step < 30; ++step) {
     SK_LAUNCH(k_chunk_sumsq<<<n_tc_, kBlock, 0, s>>>(a.tc, d_v_, a.ps_c));
-    SK_LAUNCH(k_inst_norm<<<K_, kBlock, 0, s>>>(a.desc, a.ps_c, d_len_));
-    SK_HIP(hipMemcpyAsync(lens.data() + static_cast<size_t>(step) * K_, d_len_, static_cast<size_t>(K_) * sizeof(double), hipMemcpyDeviceToHost, s), kStatusCopyFailed);
-    if (step == 30) break;
+    SK_LAUNCH(k_inst_norm<<<K_, kBlock, 0, s>>>(a.desc, a.ps_c, d_len_, d_zero_seen_));
     SK_LAUNCH(k_chunk_div<<<n_tc_, kBlock, 0, s>>>(a.tc, d_len_, d_v_));
-    SK_LAUNCH(k_spmv_serial<<<blocks_for(M), kBlock, 0, s>>>(a.a_ptr, a.a_idx, d_sa_, d_v_, d_av_, M));
-    SK_LAUNCH(k_spmv_serial<<<blocks_for(N), kBlock, 0, s>>>(a.t_ptr, a.t_idx, d_st_, d_av_, d_v_, N));
+    SK_LAUNCH(k_spmv_tiles<<<n_ta_, kBlock, 0, s>>>(a.ta, a.a_ptr, a.a_idx, d_sa_, d_v_, d_av_));
+    SK_LAUNCH(k_spmv_tiles<<<n_tt_, kBlock, 0, s>>>(a.tt, a.t_ptr, a.t_idx, d_st_, d_av_, d_v_));
   }
-  SK_HIP(hipStreamSynchronize(s), kStatusSyncFailed);
-  std::vector<InstState> st = init_state_;
-  for (int32_t k = 0; k < K_; ++k) {
-    double norm = 1.0;
-    if (!trivial_norm_[static_cast<size_t>(k)]) {
-      bool zero = false;
-      for (int step = 0; step < 30; ++step) zero = zero || lens[static_cast<size_t>(step) * K_ + k] == 0.0;
-      // lens[30] = ||A^T A v||, so sqrt of it is ||A^T A v||^(1/2)
-      const double r = std::sqrt(lens[static_cast<size_t>(30) * K_ + k]);
-      norm = zero ? 1.0 : (r > 0.0 ? r * 1.01 : 1.0);
-    }
-    InstState& x = st[static_cast<size_t>(k)];
-    x.spectral_norm = norm;
-    x.eta = norm > 0.0 ? 1.0 / norm : 1.0;
-    x.eta_ceiling = 1.0e3 / std::max(norm, 1e-12);
-    if (x.iteration_limit <= 0) {
-      x.done = 1;
-      x.status = kStIterationLimit;
-      x.report_pending = 1;
-    }
-  }
-  uint32_t initially_done = 0;
-  for (const auto& x : st) initially_done += static_cast<uint32_t>(x.done);
-  SK_HIP(hipMemcpyAsync(a.state, st.data(), st.size() * sizeof(InstState), hipMemcpyHostToDevice, s), kStatusCopyFailed);
-  host_done_[2] = initially_done;
-  SK_HIP(hipMemcpyAsync(a.done_count, &host_done_[2], sizeof(uint32_t), hipMemcpyHostToDevice, s), kStatusCopyFailed);
+  SK_LAUNCH(k_chunk_sumsq<<<n_tc_, kBlock, 0, s>>>(a.tc, d_v_, a.ps_c));
+  SK_LAUNCH(k_inst_norm<<<K_, kBlock, 0, s>>>(a.desc, a.ps_c, d_len_, nullptr));
+  SK_LAUNCH(k_power_finish<<<blocks_for(K_), kBlock, 0, s>>>(d_len_, d_zero_seen_, a.state, K_));
   SK_LAUNCH(k_init_iterates<T><<<blocks_for(MN), kBlock, 0, s>>>(a, d_lhat64_, d_uhat64_, N, M));
-  SK_HIP(hipStreamSynchronize(s), kStatusSyncFailed);
+  return kStatusSuccess;
+}
+
+template <class T>
+int EngineImpl<T>::capture(int (EngineImpl<T>::*body)(), hipGraphExec_t* exec) {
+  hipStream_t s = stream_;
+  SK_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), kStatusLaunchFailed);
+  const int st = (this->*body)();
+  hipGraph_t graph = nullptr;
+  const hipError_t e = hipStreamEndCapture(s, &graph);
+  if (st) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return st;
+  }
+  if (e != hipSuccess) {
+    set_last_error(std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    return kStatusLaunchFailed;
+  }
+  const hipError_t ei = hipGraphInstantiate(exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (ei != hipSuccess) {
+    set_last_error(std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
+    *exec = nullptr;
+    return kStatusLaunchFailed;
+  }
+  return kStatusSuccess;
+}
+
+template <class T>
+int EngineImpl<T>::setup_device() {
+  hipStream_t s = stream_;
+  // per-solve state: host-built records (norms, options, instances stopped before step 1)
+  host_done_[2] = initially_done_;
+  SK_HIP(hipMemcpyAsync(args_.state, init_state_.data(), init_state_.size() * sizeof(InstState),
+                        hipMemcpyHostToDevice, s), kStatusCopyFailed);
+  SK_HIP(hipMemcpyAsync(args_.done_count, &host_done_[2], sizeof(uint32_t), hipMemcpyHostToDevice, s),
+         kStatusCopyFailed);
+  if (opt_.use_graph) {
+    if (!setup_exec_) {
+      const int st = capture(&EngineImpl<T>::enqueue_setup, &setup_exec_);
+      if (st) return st;
+    }
+    SK_HIP(hipGraphLaunch(setup_exec_, s), kStatusLaunchFailed);
+  } else {
+    const int st = enqueue_setup();
+    if (st) return st;
+  }
   return kStatusSuccess;
 }
 
@@ -549,8 +587,13 @@ int EngineImpl<T>::enqueue_block() {
       SK_LAUNCH(k_step_aty<T, false><<<n_tt_, kBlock, 0, s>>>(a));
     }
     SK_LAUNCH(k_step_finalize<T><<<K_, kBlock, 0, s>>>(a));
+    // an instance that became due for evaluation waits at most eval_stride steps
+    if ((k + 1) % opt_.eval_stride == 0 || k + 1 == opt_.steps_per_block) {
+      const int st = enqueue_eval(0);
+      if (st) return st;
+    }
   }
-  return enqueue_eval(0);
+  return kStatusSuccess;
 }
 
 template <class T>
@@ -558,24 +601,7 @@ int EngineImpl<T>::run_loop() {
   hipStream_t s = stream_;
   int st = 0;
   if (opt_.use_graph && !graph_exec_) {
-    hipGraph_t graph = nullptr;
-    SK_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), kStatusLaunchFailed);
-    st = enqueue_block();
-    hipGraph_t captured = nullptr;
-    const hipError_t e = hipStreamEndCapture(s, &captured);
-    if (st) return st;
-    if (e != hipSuccess) {
-      set_last_error(std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
-      return kStatusLaunchFailed;
-    }
-    graph = captured;
-    const hipError_t ei = hipGraphInstantiate(&graph_exec_, graph, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(graph);
-    if (ei != hipSuccess) {
-      set_last_error(std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
-      graph_exec_ = nullptr;
-      return kStatusLaunchFailed;
-    }
+    if ((st = capture(&EngineImpl<T>::enqueue_block, &graph_exec_))) return st;
   }
   const auto t0 = std::chrono::steady_clock::now();
   const uint32_t target = static_cast<uint32_t>(K_);

@@ -121,7 +121,11 @@ __global__ __launch_bounds__(kBlock) void k_step_primal(Args<T> a) {
   T* __restrict__ xn = p ? a.x0 : a.x1;
   const T* __restrict__ aty = p ? a.aty1 : a.aty0;
   double d2 = 0.0;
-  for (int j = t.begin + threadIdx.x; j < t.end; j += kBlock) {
+  constexpr int U = kChunk / kBlock;
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const int j = t.begin + u * kBlock + threadIdx.x;
+    if (j >= t.end) continue;
     const T xj = x[j];
     if (acc) a.xsum[j] += static_cast<double>(xj);
     const T g = a.chat[j] + aty[j];
@@ -593,17 +597,23 @@ __global__ __launch_bounds__(kBlock) void k_row_pass(const int32_t* __restrict__
   pass[r] = acc > 0.0 ? 1.0 / sqrt(acc) : 1.0;
 }
 
-// value * row_pass[i] * column_pass[j] in both layouts (scaling.cpp:14-40). `row_major`:
-// rows of this CSR are rows of A (index array holds columns); otherwise rows are columns.
-__global__ __launch_bounds__(kBlock) void k_apply_pass(const int32_t* __restrict__ ptr,
+// value * row_pass[i] * column_pass[j] in both layouts (scaling.cpp:14-40), one workgroup per
+// row-block tile with G lanes per row. `row_major`: rows of this CSR are rows of A (the index
+// array holds columns); otherwise its rows are columns of A.
+__global__ __launch_bounds__(kBlock) void k_apply_pass(const Tile* __restrict__ tiles,
+                                                        const int32_t* __restrict__ ptr,
                                                         const int32_t* __restrict__ idx,
-                                                        double* __restrict__ val, int32_t rows,
-                                                        int32_t row_major,
+                                                        double* __restrict__ val, int32_t row_major,
                                                         const double* __restrict__ row_pass,
                                                         const double* __restrict__ col_pass) {
-  const int r = blockIdx.x * kBlock + threadIdx.x;
-  if (r >= rows) return;
-  for (int p = ptr[r]; p < ptr[r + 1]; ++p) {
+  const Tile t = tiles[blockIdx.x];
+  const int nrows = t.end - t.begin;
+  if (nrows <= 0) return;
+  const int g = nrows == 1 ? kBlock : min(kWave, pow2_floor(kBlock / nrows));
+  const int row = threadIdx.x / g, lane = threadIdx.x % g;
+  if (row >= nrows) return;
+  const int r = t.begin + row;
+  for (int p = ptr[r] + lane; p < ptr[r + 1]; p += g) {
     const double ri = row_major ? row_pass[r] : row_pass[idx[p]];
     const double cj = row_major ? col_pass[idx[p]] : col_pass[r];
     val[p] = val[p] * ri * cj;
@@ -646,18 +656,19 @@ __global__ __launch_bounds__(kBlock) void k_convert(const double* __restrict__ s
 
 // ---- power iteration (scaling.cpp:161-194) ---------------------------------------------------
 
-// y_r = sum_p val[p] x[idx[p]] in stored order, one lane per row (the reference's per-row
-// accumulation order for both A x and A^T y).
-__global__ __launch_bounds__(kBlock) void k_spmv_serial(const int32_t* __restrict__ ptr,
-                                                         const int32_t* __restrict__ idx,
-                                                         const double* __restrict__ val,
-                                                         const double* __restrict__ x,
-                                                         double* __restrict__ y, int32_t rows) {
-  const int r = blockIdx.x * kBlock + threadIdx.x;
-  if (r >= rows) return;
-  double acc = 0.0;
-  for (int p = ptr[r]; p < ptr[r + 1]; ++p) acc += val[p] * x[idx[p]];
-  y[r] = acc;
+// y = A x over the row-block tiles of a CSR matrix in FP64 (any tile schedule; the power
+// iteration only sets the starting step size).
+__global__ __launch_bounds__(kBlock) void k_spmv_tiles(const Tile* __restrict__ tiles,
+                                                        const int32_t* __restrict__ ptr,
+                                                        const int32_t* __restrict__ idx,
+                                                        const double* __restrict__ val,
+                                                        const double* __restrict__ x,
+                                                        double* __restrict__ y) {
+  __shared__ double red[kBlock / kWave];
+  const Tile t = tiles[xcd_tile(blockIdx.x, gridDim.x)];
+  rowblock_product<double, false>(
+      t, ptr, [&](int q) { return val[q] * x[idx[q]]; }, [&](int r, double acc) { y[r] = acc; },
+      (double*)nullptr, red);
 }
 
 // Per column chunk: sum of v^2 (slot 0 of pe_c).
@@ -672,17 +683,57 @@ __global__ __launch_bounds__(kBlock) void k_chunk_sumsq(const Tile* __restrict__
   if (threadIdx.x == 0) part[blockIdx.x] = acc;
 }
 
-// One workgroup per instance: length = sqrt(sum of its chunk partials).
+// One workgroup per instance: length = sqrt(sum of its chunk partials); a zero length is
+// remembered (the reference then returns 1, scaling.cpp:176-179).
 __global__ __launch_bounds__(kBlock) void k_inst_norm(const InstDesc* __restrict__ desc,
                                                        const double* __restrict__ part,
-                                                       double* __restrict__ out) {
+                                                       double* __restrict__ out,
+                                                       int32_t* __restrict__ zero_seen) {
   __shared__ double red[kBlock / kWave];
   const InstDesc d = desc[blockIdx.x];
   const double s = tile_sum(part, d.tc0, d.tc1 - d.tc0, 1, 0, red);
-  if (threadIdx.x == 0) out[blockIdx.x] = sqrt(s);
+  if (threadIdx.x == 0) {
+    out[blockIdx.x] = sqrt(s);
+    if (zero_seen && s == 0.0) zero_seen[blockIdx.x] = 1;
+  }
 }
 
-// v /= length of the instance (a zero length leaves v as is; the host then reports 1).
+// Spectral norm estimate and the initial step sizes (scaling.cpp:187-193, pdhg.cpp:368,
+// 495). `len` holds ||A^T A v|| after the last step, so the estimate is sqrt(len) * 1.01.
+__global__ __launch_bounds__(kBlock) void k_power_finish(const double* __restrict__ len,
+                                                          const int32_t* __restrict__ zero_seen,
+                                                          InstState* __restrict__ state,
+                                                          int32_t count) {
+  const int k = blockIdx.x * kBlock + threadIdx.x;
+  if (k >= count) return;
+  InstState& s = state[k];
+  double norm = 1.0;
+  if (!s.norm_trivial && !zero_seen[k]) {
+    const double r = sqrt(len[k]);
+    norm = r > 0.0 ? r * 1.01 : 1.0;
+  }
+  s.spectral_norm = norm;
+  s.eta = norm > 0.0 ? 1.0 / norm : 1.0;
+  s.eta_ceiling = 1.0e3 / dev_max(norm, 1e-12);
+}
+
+__global__ __launch_bounds__(kBlock) void k_fill(double* __restrict__ dst, double value, int64_t n) {
+  const int64_t k = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+  if (k < n) dst[k] = value;
+}
+
+__global__ __launch_bounds__(kBlock) void k_fill_i32(int32_t* __restrict__ dst, int32_t value, int64_t n) {
+  const int64_t k = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+  if (k < n) dst[k] = value;
+}
+
+__global__ __launch_bounds__(kBlock) void k_copy(const double* __restrict__ src, double* __restrict__ dst,
+                                                  int64_t n) {
+  const int64_t k = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+  if (k < n) dst[k] = src[k];
+}
+
+// v /= length of the instance (a zero length leaves v as is; the estimate is then 1).
 __global__ __launch_bounds__(kBlock) void k_chunk_div(const Tile* __restrict__ tc,
                                                        const double* __restrict__ length,
                                                        double* __restrict__ v) {
@@ -692,7 +743,7 @@ __global__ __launch_bounds__(kBlock) void k_chunk_div(const Tile* __restrict__ t
   for (int k = t.begin + threadIdx.x; k < t.end; k += kBlock) v[k] /= len;
 }
 
-#pragma clang fp contract(on)
+#pragma clang fp contract(fast)
 
 // ---- initial state (pdhg.cpp:327-379) ----------------------------------------------------------
 
