@@ -104,22 +104,25 @@ __device__ __forceinline__ int pow2_floor(int v) {
 
 // ---- the row-block product ---------------------------------------------------------------
 // Computes acc_r = sum_p prod(p) for every row r of tile t of a CSR matrix and calls
-// epi(r, acc_r) exactly once per row, on one lane.
+// epi(r, acc_r) exactly once per row, on one lane; that lane calls pre(r) first, before any
+// product is loaded.
 //
 // Stream tile (rows > 1 or nnz <= cap): when STAGE, all lanes first write prod(p) for the
 // tile's nonzeros into LDS with coalesced loads (CSR-Stream; Greathouse & Daga, SC'14), then
 // G = 256 / rows lanes (power of two, <= 64) reduce each row from LDS. Without STAGE the
 // same G-lane groups read their row straight from memory.
 // Vector tile (one row with nnz > cap): the workgroup strides over the row and reduces.
-template <class A, bool STAGE, class Prod, class Epi>
+template <class A, bool STAGE, class Prod, class Pre, class Epi>
 __device__ __forceinline__ void rowblock_product(const Tile t, const int32_t* __restrict__ ptr,
-                                                 Prod prod, Epi epi, A* stage, A* red_lds) {
+                                                 Prod prod, Pre pre, Epi epi, A* stage,
+                                                 A* red_lds) {
   const int tid = threadIdx.x;
   const int r0 = t.begin, nrows = t.end - t.begin;
   const int p0 = ptr[r0];
   const int p1 = ptr[t.end];
   const int nnz = p1 - p0;
   if (nrows == 1 && nnz > t.cap) {
+    if (tid == 0) pre(r0);
     A acc = zero_of<A>();
     constexpr int U = 4;
     for (int base = p0; base < p1; base += U * kBlock) {
@@ -136,6 +139,13 @@ __device__ __forceinline__ void rowblock_product(const Tile t, const int32_t* __
     if (tid == 0) epi(r0, acc);
     return;
   }
+  const int g = nrows > 0 ? min(kWave, pow2_floor(kBlock / nrows)) : kWave;
+  const int row = tid / g, lane = tid % g;
+  // the lane that finalises a row loads its epilogue operands first, so those loads are in
+  // flight together with the product loads below
+  if (row < nrows && lane == 0) pre(r0 + row);
+  const int b = row < nrows ? ptr[r0 + row] : 0;
+  const int e = row < nrows ? ptr[r0 + row + 1] : 0;
   if (STAGE) {
     // Unrolled batches: every lane issues all its loads of a batch before any is consumed,
     // so each wavefront keeps U gathers in flight instead of one.
@@ -155,11 +165,8 @@ __device__ __forceinline__ void rowblock_product(const Tile t, const int32_t* __
     }
     __syncthreads();
   }
-  const int g = nrows > 0 ? min(kWave, pow2_floor(kBlock / nrows)) : kWave;
-  const int row = tid / g, lane = tid % g;
   A acc = zero_of<A>();
   if (row < nrows) {
-    const int b = ptr[r0 + row], e = ptr[r0 + row + 1];
     if (STAGE) {
       for (int k = b - p0 + lane; k < e - p0; k += g) acc = acc + stage[k];
     } else {

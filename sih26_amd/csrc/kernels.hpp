@@ -157,13 +157,20 @@ __global__ __launch_bounds__(kBlock) void k_step_dual(Args<T> a) {
   const T* __restrict__ val = a.a_val;
   const int32_t* __restrict__ idx = a.a_idx;
   double d2 = 0.0;
+  T yi = 0, lo = 0, hi = 0;
+  double sum = 0.0;
   rowblock_product<T, STAGE>(
       t, a.a_ptr, [&](int q) { return val[q] * xbar[idx[q]]; },
+      [&](int i) {
+        yi = y[i];
+        lo = a.rlhat[i];
+        hi = a.ruhat[i];
+        if (acc) sum = a.ysum[i];
+      },
       [&](int i, T r) {
-        const T yi = y[i];
-        if (acc) a.ysum[i] += static_cast<double>(yi);
+        if (acc) a.ysum[i] = sum + static_cast<double>(yi);
         const T v = yi + sigma * r;
-        const T ynew = v - sigma * dev_project<T>(v / sigma, a.rlhat[i], a.ruhat[i]);
+        const T ynew = v - sigma * dev_project<T>(v / sigma, lo, hi);
         yn[i] = ynew;
         const double d = static_cast<double>(ynew - yi);
         d2 += d * d;
@@ -254,11 +261,17 @@ __global__ __launch_bounds__(kBlock) void k_step_aty(Args<T> a) {
   const T* __restrict__ val = a.t_val;
   const int32_t* __restrict__ idx = a.t_idx;
   double inter = 0.0;
+  T xj = 0, xnj = 0, atyj = 0;
   rowblock_product<T, STAGE>(
       t, a.t_ptr, [&](int q) { return val[q] * yn[idx[q]]; },
+      [&](int j) {
+        xj = x[j];
+        xnj = xn[j];
+        atyj = aty[j];
+      },
       [&](int j, T r) {
         atyn[j] = r;
-        inter += static_cast<double>(xn[j] - x[j]) * static_cast<double>(r - aty[j]);
+        inter += static_cast<double>(xnj - xj) * static_cast<double>(r - atyj);
       },
       stage, red_t);
   inter = block_sum(inter, red);
@@ -345,6 +358,7 @@ __global__ __launch_bounds__(kBlock) void k_eval_rows(Args<T> a) {
         const int c = idx[q];
         return D2{val[q] * xa[c], val[q] * xb[c]};
       },
+      [](int) {},
       [&](int i, D2 act) {
         const double lo = a.rl[i], hi = a.ru[i];
         const bool lo_f = dev_finite_bound(lo), hi_f = dev_finite_bound(hi);
@@ -485,6 +499,7 @@ __global__ __launch_bounds__(kBlock) void k_eval_cols(Args<T> a) {
         const int r = idx[q];
         return D2{val[q] * ya[r], val[q] * yb[r]};
       },
+      [](int) {},
       [&](int j, D2 dot) {
         const double lo = a.cl[j], hi = a.cu[j];
         const bool lo_f = dev_finite_bound(lo), hi_f = dev_finite_bound(hi);
@@ -667,8 +682,8 @@ __global__ __launch_bounds__(kBlock) void k_spmv_tiles(const Tile* __restrict__ 
   __shared__ double red[kBlock / kWave];
   const Tile t = tiles[xcd_tile(blockIdx.x, gridDim.x)];
   rowblock_product<double, false>(
-      t, ptr, [&](int q) { return val[q] * x[idx[q]]; }, [&](int r, double acc) { y[r] = acc; },
-      (double*)nullptr, red);
+      t, ptr, [&](int q) { return val[q] * x[idx[q]]; }, [](int) {},
+      [&](int r, double acc) { y[r] = acc; }, (double*)nullptr, red);
 }
 
 // Per column chunk: sum of v^2 (slot 0 of pe_c).
