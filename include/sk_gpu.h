@@ -270,6 +270,72 @@ int32_t sk_batch_time_kernel(sk_batch* batch, int32_t kernel, int32_t reps, doub
                              double* bytes);
 void sk_batch_destroy(sk_batch* batch);
 
+/* ---- serving: a continuous-batching request loop -------------------------------------------
+ * No counterpart in the reference, which solves one LP per call (src/core/solve.cpp:385-399).
+ * Requests queue in arrival order; `slots` worker threads, each owning a batch engine with its
+ * own stream, take every waiting request that fits the batch and device-memory limits as one
+ * block-diagonal batch, so batches overlap on the device. Each request completes (retires) as
+ * soon as its own instance stops, while the rest of its batch keeps running. A request's
+ * answer is bitwise the same as sk_pdhg_solve's with the same options. */
+typedef struct sk_server sk_server;
+
+typedef struct {
+  int32_t slots;          /* concurrent batches (default 4) */
+  int32_t max_batch;      /* instances per batch (default 256) */
+  int64_t memory_budget;  /* device bytes all slots may hold; 0: 90% of free memory at create */
+  int32_t keep_solutions; /* 1: keep the primal/dual vectors for sk_server_result (default 1) */
+  int32_t reserved;
+} sk_server_options;
+
+typedef struct {
+  int64_t id;
+  double arrival;   /* seconds on the server clock (sk_server_now) */
+  double admitted;  /* its batch was formed */
+  double finished;  /* its answer was read back */
+  double latency;   /* finished - arrival */
+  int32_t slot;
+  int32_t batch_size;
+  int32_t error;    /* SK_GPU_* status when its batch failed, else SK_GPU_SUCCESS */
+  int32_t num_rows;
+  int32_t num_cols;
+  int32_t reserved;
+} sk_server_record;
+
+typedef struct {
+  int64_t submitted;
+  int64_t completed;
+  int64_t rejected;       /* larger than the memory budget */
+  int64_t failed;         /* completed with an error */
+  int64_t batches;
+  int64_t max_queue;
+  int64_t max_batch_seen;
+  int64_t memory_budget;
+  int64_t peak_held;      /* largest device memory held by the slots together */
+  int64_t releases;       /* idle slots that gave their memory back to admit a request */
+} sk_server_stats;
+
+void sk_server_default_options(sk_server_options* out);
+/* Starts the slot threads; fails with SK_GPU_NO_DEVICE without a device. */
+int32_t sk_server_create(const sk_pdhg_options* options, const sk_server_options* server_options,
+                         sk_server** out);
+int32_t sk_server_now(const sk_server* server, double* seconds);
+/* On success the server owns `lp` (do not use or free it again); `arrival` < 0 means now.
+ * An LP whose estimated device footprint exceeds the budget is refused with
+ * SK_GPU_OUT_OF_MEMORY, and the caller keeps it. */
+int32_t sk_server_submit(sk_server* server, sk_lp* lp, double arrival, int64_t* id);
+/* Ids completed since the previous call, oldest first, at most `capacity` (non-blocking). */
+int32_t sk_server_completed(sk_server* server, int64_t* ids, int32_t capacity, int32_t* count);
+/* Blocks until every submitted request has completed or `timeout` seconds passed. */
+int32_t sk_server_wait(sk_server* server, double timeout, int32_t* idle);
+/* The answer of a completed request, once: the server forgets it afterwards. Arrays as
+ * sk_pdhg_solve's (NULL skips one); they must be NULL when keep_solutions is 0. */
+int32_t sk_server_result(sk_server* server, int64_t id, sk_server_record* record,
+                         sk_pdhg_result* result, double* col_value, double* row_dual,
+                         double* col_dual, double* row_activity);
+int32_t sk_server_stats_get(const sk_server* server, sk_server_stats* out);
+/* Finishes the running batches, drops queued requests and frees everything. */
+void sk_server_destroy(sk_server* server);
+
 #ifdef __cplusplus
 }
 #endif

@@ -66,6 +66,23 @@ class BatchStats(C.Structure):
                 ("total_nnz", i64), ("instances", i32), ("spmv_stage", i32)]
 
 
+class ServerOptions(C.Structure):
+    _fields_ = [("slots", i32), ("max_batch", i32), ("memory_budget", i64),
+                ("keep_solutions", i32), ("reserved", i32)]
+
+
+class ServerRecord(C.Structure):
+    _fields_ = [("id", i64), ("arrival", f64), ("admitted", f64), ("finished", f64),
+                ("latency", f64), ("slot", i32), ("batch_size", i32), ("error", i32),
+                ("num_rows", i32), ("num_cols", i32), ("reserved", i32)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("submitted", i64), ("completed", i64), ("rejected", i64), ("failed", i64),
+                ("batches", i64), ("max_queue", i64), ("max_batch_seen", i64),
+                ("memory_budget", i64), ("peak_held", i64), ("releases", i64)]
+
+
 # name -> (restype, argtypes)
 SIGNATURES = {
     "sk_gpu_status_string": (cstr, [i32]),
@@ -104,6 +121,15 @@ SIGNATURES = {
     "sk_batch_stats_get": (i32, [vp, P(BatchStats)]),
     "sk_batch_time_kernel": (i32, [vp, i32, i32, P(f64), P(f64)]),
     "sk_batch_destroy": (None, [vp]),
+    "sk_server_default_options": (None, [P(ServerOptions)]),
+    "sk_server_create": (i32, [P(PdhgOptions), P(ServerOptions), P(vp)]),
+    "sk_server_now": (i32, [vp, P(f64)]),
+    "sk_server_submit": (i32, [vp, vp, f64, P(i64)]),
+    "sk_server_completed": (i32, [vp, P(i64), i32, P(i32)]),
+    "sk_server_wait": (i32, [vp, f64, P(i32)]),
+    "sk_server_result": (i32, [vp, i64, P(ServerRecord), P(PdhgResult), vp, vp, vp, vp]),
+    "sk_server_stats_get": (i32, [vp, P(ServerStats)]),
+    "sk_server_destroy": (None, [vp]),
 }
 
 

@@ -246,3 +246,74 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+class Server:
+    """Continuous-batching request loop on the GPU (sk_server_* of include/sk_gpu.h)."""
+
+    def __init__(self, options: _ffi.PdhgOptions | None = None, *, slots: int = 4,
+                 max_batch: int = 256, memory_budget: int = 0, keep_solutions: bool = True,
+                 **kw):
+        self.options = options if options is not None else make_options(**kw)
+        so = _ffi.ServerOptions()
+        lib().sk_server_default_options(C.byref(so))
+        so.slots, so.max_batch = slots, max_batch
+        so.memory_budget, so.keep_solutions = memory_budget, int(bool(keep_solutions))
+        self.server_options = so
+        h = C.c_void_p()
+        check(lib().sk_server_create(C.byref(self.options), C.byref(so), C.byref(h)),
+              "sk_server_create")
+        self._h = h
+
+    def now(self) -> float:
+        t = C.c_double()
+        check(lib().sk_server_now(self._h, C.byref(t)), "sk_server_now")
+        return t.value
+
+    def submit(self, lp: Lp, arrival: float = -1.0) -> int:
+        """Hands `lp` to the server (the Lp object is spent afterwards); returns the id."""
+        rid = C.c_int64()
+        check(lib().sk_server_submit(self._h, lp._h, arrival, C.byref(rid)), "sk_server_submit")
+        lp._h = None
+        return rid.value
+
+    def completed(self, capacity: int = 4096) -> list[int]:
+        ids = (C.c_int64 * capacity)()
+        cnt = C.c_int32()
+        check(lib().sk_server_completed(self._h, ids, capacity, C.byref(cnt)), "sk_server_completed")
+        return list(ids[:cnt.value])
+
+    def wait(self, timeout: float = 1e9) -> bool:
+        idle = C.c_int32()
+        check(lib().sk_server_wait(self._h, timeout, C.byref(idle)), "sk_server_wait")
+        return bool(idle.value)
+
+    def result(self, rid: int, dims: tuple[int, int] | None = None) -> tuple[SolveResult, dict]:
+        """The answer of a completed request (once). `dims` = (rows, cols) sizes the vectors;
+        without it, or with keep_solutions off, only the record and scalars come back."""
+        rec, res = _ffi.ServerRecord(), _ffi.PdhgResult()
+        if dims is not None and self.server_options.keep_solutions:
+            m, n = dims
+            x, d, yd, act = np.zeros(n), np.zeros(n), np.zeros(m), np.zeros(m)
+        else:
+            x = d = yd = act = None
+        check(lib().sk_server_result(self._h, rid, C.byref(rec), C.byref(res), _ptr(x), _ptr(yd),
+                                     _ptr(d), _ptr(act)), "sk_server_result")
+        record = {f: getattr(rec, f) for f, _ in rec._fields_}
+        return _result(res, x, yd, d, act), record
+
+    def stats(self) -> dict[str, Any]:
+        s = _ffi.ServerStats()
+        check(lib().sk_server_stats_get(self._h, C.byref(s)), "sk_server_stats_get")
+        return {f: getattr(s, f) for f, _ in s._fields_}
+
+    def close(self) -> None:
+        if self._h:
+            lib().sk_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

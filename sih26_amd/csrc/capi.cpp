@@ -9,15 +9,19 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <deque>
 #include <memory>
+#include <mutex>
 #include <new>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/sk_gpu.h"
 #include "engine.hpp"
 #include "lp.hpp"
 #include "quality.hpp"
+#include "server.hpp"
 #include "status.hpp"
 
 struct sk_lp {
@@ -524,5 +528,200 @@ int32_t sk_batch_time_kernel(sk_batch* b, int32_t kernel, int32_t reps, double* 
 }
 
 void sk_batch_destroy(sk_batch* b) { delete b; }
+
+}  // extern "C"
+
+// A completed request as the C ABI hands it out.
+struct ServedAnswer {
+  sk_server_record record{};
+  sk_pdhg_result result{};
+  std::vector<double> col_value, row_dual, col_dual, row_activity;
+};
+
+struct sk_server {
+  sk_pdhg_options options{};
+  sk_server_options server_options{};
+  std::mutex mu;  // answers and completed ids (written by the slot threads)
+  std::unordered_map<int64_t, ServedAnswer> answers;
+  std::deque<int64_t> completed;
+  std::unique_ptr<sk::Server> server;  // last member: its slot threads stop first
+};
+
+namespace {
+
+// Runs on a slot thread while the request's LP is still alive.
+void store_answer(sk_server* s, sk::Request& r) {
+  ServedAnswer a;
+  const sk::HostLp& lp = *r.lp;
+  a.record.id = r.id;
+  a.record.arrival = r.arrival;
+  a.record.admitted = r.admitted;
+  a.record.finished = r.finished;
+  a.record.latency = r.finished - r.arrival;
+  a.record.slot = r.slot;
+  a.record.batch_size = r.batch_size;
+  a.record.error = r.error;
+  a.record.num_rows = lp.num_rows;
+  a.record.num_cols = lp.num_cols;
+  if (r.error) {
+    a.result.status = SK_SOLVE_NOT_SOLVED;
+    std::snprintf(a.result.algorithm, sizeof(a.result.algorithm), "pdhg-gpu");
+    std::snprintf(a.result.message, sizeof(a.result.message), "GPU PDHG failed: %s: %s",
+                  sk_gpu_status_string(r.error), r.error_message.c_str());
+  } else {
+    if (s->server_options.keep_solutions) {
+      a.col_value.resize(static_cast<size_t>(lp.num_cols));
+      a.col_dual.resize(static_cast<size_t>(lp.num_cols));
+      a.row_dual.resize(static_cast<size_t>(lp.num_rows));
+      a.row_activity.resize(static_cast<size_t>(lp.num_rows));
+    }
+    const bool keep = s->server_options.keep_solutions != 0;
+    const int32_t st = fill_result(lp, r.result, s->options, r.finished - r.admitted, &a.result,
+                                   keep ? a.col_value.data() : nullptr,
+                                   keep ? a.row_dual.data() : nullptr,
+                                   keep ? a.col_dual.data() : nullptr,
+                                   keep ? a.row_activity.data() : nullptr);
+    if (st) a.record.error = st;
+  }
+  std::lock_guard<std::mutex> lk(s->mu);
+  s->answers.emplace(r.id, std::move(a));
+  s->completed.push_back(r.id);
+}
+
+}  // namespace
+
+extern "C" {
+
+void sk_server_default_options(sk_server_options* out) {
+  if (!out) return;
+  std::memset(out, 0, sizeof(*out));
+  out->slots = 4;
+  out->max_batch = 256;
+  out->memory_budget = 0;
+  out->keep_solutions = 1;
+}
+
+int32_t sk_server_create(const sk_pdhg_options* options, const sk_server_options* server_options,
+                         sk_server** out) {
+  if (!out) return SK_GPU_INVALID_ARGUMENT;
+  *out = nullptr;
+  return guarded([&]() -> int32_t {
+    auto s = std::make_unique<sk_server>();
+    if (options) s->options = *options;
+    else sk_pdhg_default_options(&s->options);
+    if (server_options) s->server_options = *server_options;
+    else sk_server_default_options(&s->server_options);
+    const sk_server_options& so = s->server_options;
+    if (so.slots < 1 || so.slots > 64 || so.max_batch < 1 || so.memory_budget < 0) {
+      sk::set_last_error("sk_server_create: slots must be in [1, 64], max_batch >= 1, memory_budget >= 0");
+      return SK_GPU_INVALID_ARGUMENT;
+    }
+    sk::ServerOptions o;
+    o.engine = to_engine(s->options);
+    o.slots = so.slots;
+    o.max_batch = so.max_batch;
+    o.memory_budget = so.memory_budget;
+    sk_server* raw = s.get();
+    s->server = std::make_unique<sk::Server>(o, [raw](sk::Request& r) { store_answer(raw, r); });
+    const int32_t st = s->server->open();
+    if (st) return st;
+    *out = s.release();
+    return SK_GPU_SUCCESS;
+  });
+}
+
+int32_t sk_server_now(const sk_server* s, double* seconds) {
+  if (!s || !seconds) return SK_GPU_INVALID_ARGUMENT;
+  *seconds = s->server->now();
+  return SK_GPU_SUCCESS;
+}
+
+int32_t sk_server_submit(sk_server* s, sk_lp* lp, double arrival, int64_t* id) {
+  if (!s || !lp) return SK_GPU_INVALID_ARGUMENT;
+  return guarded([&]() -> int32_t {
+    // the server takes the model out of the handle only once it accepts the request
+    auto owned = std::make_unique<sk::HostLp>(std::move(lp->lp));
+    const int32_t st = s->server->submit(owned, arrival, id);
+    if (st) {
+      lp->lp = std::move(*owned);  // refused: the caller keeps the model
+      return st;
+    }
+    delete lp;
+    return SK_GPU_SUCCESS;
+  });
+}
+
+int32_t sk_server_completed(sk_server* s, int64_t* ids, int32_t capacity, int32_t* count) {
+  if (!s || !count || capacity < 0 || (capacity > 0 && !ids)) return SK_GPU_INVALID_ARGUMENT;
+  std::lock_guard<std::mutex> lk(s->mu);
+  int32_t n = 0;
+  while (n < capacity && !s->completed.empty()) {
+    ids[n++] = s->completed.front();
+    s->completed.pop_front();
+  }
+  *count = n;
+  return SK_GPU_SUCCESS;
+}
+
+int32_t sk_server_wait(sk_server* s, double timeout, int32_t* idle) {
+  if (!s) return SK_GPU_INVALID_ARGUMENT;
+  return guarded([&]() -> int32_t {
+    const bool done = s->server->wait_idle(timeout);
+    if (idle) *idle = done ? 1 : 0;
+    return SK_GPU_SUCCESS;
+  });
+}
+
+int32_t sk_server_result(sk_server* s, int64_t id, sk_server_record* record, sk_pdhg_result* result,
+                         double* col_value, double* row_dual, double* col_dual,
+                         double* row_activity) {
+  if (!s) return SK_GPU_INVALID_ARGUMENT;
+  if (!s->server_options.keep_solutions && (col_value || row_dual || col_dual || row_activity)) {
+    sk::set_last_error("sk_server_result: the server keeps no solution vectors (keep_solutions = 0)");
+    return SK_GPU_INVALID_ARGUMENT;
+  }
+  return guarded([&]() -> int32_t {
+    ServedAnswer a;
+    {
+      std::lock_guard<std::mutex> lk(s->mu);
+      auto it = s->answers.find(id);
+      if (it == s->answers.end()) {
+        sk::set_last_error("sk_server_result: no completed request with id " + std::to_string(id));
+        return SK_GPU_INVALID_ARGUMENT;
+      }
+      a = std::move(it->second);
+      s->answers.erase(it);
+    }
+    if (record) *record = a.record;
+    if (result) *result = a.result;
+    if (col_value) std::copy(a.col_value.begin(), a.col_value.end(), col_value);
+    if (row_dual) std::copy(a.row_dual.begin(), a.row_dual.end(), row_dual);
+    if (col_dual) std::copy(a.col_dual.begin(), a.col_dual.end(), col_dual);
+    if (row_activity) std::copy(a.row_activity.begin(), a.row_activity.end(), row_activity);
+    return SK_GPU_SUCCESS;
+  });
+}
+
+int32_t sk_server_stats_get(const sk_server* s, sk_server_stats* out) {
+  if (!s || !out) return SK_GPU_INVALID_ARGUMENT;
+  const sk::ServerStats t = s->server->stats();
+  out->submitted = t.submitted;
+  out->completed = t.completed;
+  out->rejected = t.rejected;
+  out->failed = t.failed;
+  out->batches = t.batches;
+  out->max_queue = t.max_queue;
+  out->max_batch_seen = t.max_batch_seen;
+  out->memory_budget = t.memory_budget;
+  out->peak_held = t.peak_held;
+  out->releases = t.releases;
+  return SK_GPU_SUCCESS;
+}
+
+void sk_server_destroy(sk_server* s) {
+  if (!s) return;
+  s->server.reset();  // joins the slot threads before the answers go
+  delete s;
+}
 
 }  // extern "C"

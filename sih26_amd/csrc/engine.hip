@@ -95,13 +95,18 @@ class EngineImpl final : public Engine {
     if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
     if (setup_exec_) (void)hipGraphExecDestroy(setup_exec_);
     if (stream_) (void)hipStreamDestroy(stream_);
+    if (copy_stream_) (void)hipStreamDestroy(copy_stream_);
     if (host_done_) (void)hipHostFree(host_done_);
+    if (host_state_) (void)hipHostFree(host_state_);
+    for (hipEvent_t e : block_ev_)
+      if (e) (void)hipEventDestroy(e);
   }
 
   int32_t add(const HostLp* lp) override {
     lps_.push_back(lp);
     return static_cast<int32_t>(lps_.size()) - 1;
   }
+  void clear() override { lps_.clear(); }
   int32_t size() const override { return static_cast<int32_t>(lps_.size()); }
   const EngineTimings& timings() const override { return timings_; }
   int64_t total_nnz() const override { return nnz_; }
@@ -111,15 +116,29 @@ class EngineImpl final : public Engine {
 
   int upload() override;
   int solve() override;
+  int start() override;
+  int poll(bool wait, bool* finished, std::vector<int32_t>* newly_done) override;
+  int64_t device_bytes() const override;
+  void release() override;
   int result(int32_t index, InstanceResult* out) const override;
   int time_kernel(int kernel, int reps, double* mean_ms) override;
   double kernel_bytes(int kernel) const override;
 
  private:
+  // Grow-only device buffers: a reloaded engine (a serving slot taking its next batch) keeps
+  // every allocation that is already large enough.
   template <class U>
   int alloc(DevBuf& b, size_t count, U** typed) {
-    b.bytes = std::max<size_t>(1, count) * sizeof(U);
-    SK_HIP(hipMalloc(&b.ptr, b.bytes), kStatusOutOfMemory);
+    const size_t need = std::max<size_t>(1, count) * sizeof(U);
+    if (b.ptr && b.bytes < need) {
+      (void)hipFree(b.ptr);
+      b.ptr = nullptr;
+      b.bytes = 0;
+    }
+    if (!b.ptr) {
+      SK_HIP(hipMalloc(&b.ptr, need), kStatusOutOfMemory);
+      b.bytes = need;
+    }
     *typed = static_cast<U*>(b.ptr);
     return kStatusSuccess;
   }
@@ -128,7 +147,8 @@ class EngineImpl final : public Engine {
     int st = alloc(b, host.size(), typed);
     if (st) return st;
     if (!host.empty())
-      SK_HIP(hipMemcpy(b.ptr, host.data(), host.size() * sizeof(U), hipMemcpyHostToDevice),
+      SK_HIP(hipMemcpyAsync(b.ptr, host.data(), host.size() * sizeof(U), hipMemcpyHostToDevice,
+                            stream_),
              kStatusCopyFailed);
     return kStatusSuccess;
   }
@@ -140,6 +160,9 @@ class EngineImpl final : public Engine {
   int report();
   int enqueue_block();
   int enqueue_eval(int report_mode);
+  int launch_block();
+  int extract_finished(const InstState* snapshot, std::vector<int32_t>* newly_done);
+  int stop_on_time_limit(std::vector<int32_t>* newly_done);
 
   EngineOptions opt_;
   std::vector<const HostLp*> lps_;
@@ -165,12 +188,26 @@ class EngineImpl final : public Engine {
   DevBuf b_a_ptr_, b_a_idx_, b_a_orig_, b_t_ptr_, b_t_idx_, b_t_orig_;
   DevBuf b_sa_, b_st_, b_a_val_, b_t_val_;
   DevBuf b_cmin_, b_cl_, b_cu_, b_rl_, b_ru_, b_dc_, b_dr_, b_rpass_, b_cpass_;
-  DevBuf b_chat_, b_lhat_, b_uhat_, b_rlhat_, b_ruhat_, b_lhat64_, b_uhat64_, b_scratch64_[3];
+  DevBuf b_chat_, b_lhat_, b_uhat_, b_rlhat_, b_ruhat_, b_lhat64_, b_uhat64_;
   DevBuf b_x0_, b_x1_, b_y0_, b_y1_, b_aty0_, b_aty1_, b_xbar_, b_xrst_, b_yrst_;
   DevBuf b_xsum_, b_ysum_, b_xu0_, b_xu1_, b_yu0_, b_yu1_, b_bestx_, b_besty_, b_act_, b_red_;
   DevBuf b_tc_, b_tr_, b_ta_, b_tt_, b_desc_, b_state_;
   DevBuf b_ps_c_, b_ps_a_, b_ps_t_, b_pe_c_, b_pe_r_, b_pe_a_, b_pe_t_;
   DevBuf b_done_, b_v_, b_v0_, b_av_, b_len_, b_zero_;
+  std::vector<DevBuf*> buffers() {
+    return {&b_a_ptr_, &b_a_idx_, &b_a_orig_, &b_t_ptr_, &b_t_idx_, &b_t_orig_, &b_sa_, &b_st_,
+            &b_a_val_, &b_t_val_, &b_cmin_, &b_cl_, &b_cu_, &b_rl_, &b_ru_, &b_dc_, &b_dr_,
+            &b_rpass_, &b_cpass_, &b_chat_, &b_lhat_, &b_uhat_, &b_rlhat_, &b_ruhat_,
+            &b_lhat64_, &b_uhat64_, &b_x0_, &b_x1_, &b_y0_, &b_y1_, &b_aty0_, &b_aty1_,
+            &b_xbar_, &b_xrst_, &b_yrst_, &b_xsum_, &b_ysum_, &b_xu0_, &b_xu1_, &b_yu0_,
+            &b_yu1_, &b_bestx_, &b_besty_, &b_act_, &b_red_, &b_tc_, &b_tr_, &b_ta_, &b_tt_,
+            &b_desc_, &b_state_, &b_ps_c_, &b_ps_a_, &b_ps_t_, &b_pe_c_, &b_pe_r_, &b_pe_a_,
+            &b_pe_t_, &b_done_, &b_v_, &b_v0_, &b_av_, &b_len_, &b_zero_};
+  }
+  std::vector<const DevBuf*> buffers() const {
+    auto all = const_cast<EngineImpl<T>*>(this)->buffers();
+    return std::vector<const DevBuf*>(all.begin(), all.end());
+  }
 
   Args<T> args_{};
   double* d_sa_ = nullptr;
@@ -183,9 +220,19 @@ class EngineImpl final : public Engine {
   double* d_v0_ = nullptr;
   double* d_av_ = nullptr;
   double* d_len_ = nullptr;
-  double* d_ones_m_ = nullptr;
-  double* d_ones_n_ = nullptr;
   int32_t* d_zero_seen_ = nullptr;
+
+  // serving: blocks in flight (a ring of 2: event, finished count and state snapshot per
+  // block), instances already read back, and a second stream for those read-backs
+  hipStream_t copy_stream_ = nullptr;
+  InstState* host_state_ = nullptr;  // pinned, 2 * capacity records
+  size_t host_state_cap_ = 0;
+  hipEvent_t block_ev_[2] = {nullptr, nullptr};
+  int64_t launched_ = 0;
+  int inflight_ = 0;
+  uint32_t extracted_count_ = 0;
+  std::chrono::steady_clock::time_point started_;
+  std::vector<uint8_t> extracted_;
 
   // results
   std::vector<InstState> final_state_;
@@ -195,6 +242,11 @@ class EngineImpl final : public Engine {
 
 template <class T>
 int EngineImpl<T>::upload() {
+  // a reload (new instance set) invalidates the captured graphs: grids and tables change
+  if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
+  if (setup_exec_) (void)hipGraphExecDestroy(setup_exec_);
+  graph_exec_ = setup_exec_ = nullptr;
+  inflight_ = 0;
   K_ = static_cast<int32_t>(lps_.size());
   m_ = n_ = nnz_ = 0;
   initially_done_ = 0;
@@ -333,7 +385,18 @@ int EngineImpl<T>::upload() {
     return kStatusNoDevice;
   }
   if (!stream_) SK_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), kStatusInitFailed);
+  if (!copy_stream_) SK_HIP(hipStreamCreateWithFlags(&copy_stream_, hipStreamNonBlocking), kStatusInitFailed);
   if (!host_done_) SK_HIP(hipHostMalloc(reinterpret_cast<void**>(&host_done_), 4 * sizeof(uint32_t)), kStatusOutOfMemory);
+  if (host_state_cap_ < static_cast<size_t>(K_)) {
+    if (host_state_) (void)hipHostFree(host_state_);
+    host_state_ = nullptr;
+    host_state_cap_ = 0;
+    SK_HIP(hipHostMalloc(reinterpret_cast<void**>(&host_state_), 2 * static_cast<size_t>(K_) * sizeof(InstState)),
+           kStatusOutOfMemory);
+    host_state_cap_ = static_cast<size_t>(K_);
+  }
+  for (hipEvent_t& e : block_ev_)
+    if (!e) SK_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming), kStatusInitFailed);
 
   Args<T>& a = args_;
   int st = 0;
@@ -382,11 +445,6 @@ int EngineImpl<T>::upload() {
   a.dr = pdr;
   SK_TRY(alloc(b_rpass_, M, &d_rpass_));
   SK_TRY(alloc(b_cpass_, N, &d_cpass_));
-  {
-    std::vector<double> ones_m(M, 1.0), ones_n(N, 1.0);
-    SK_TRY(put(b_scratch64_[0], ones_m, &d_ones_m_));
-    SK_TRY(put(b_scratch64_[1], ones_n, &d_ones_n_));
-  }
   T *chat, *lhat, *uhat, *rlhat, *ruhat;
   SK_TRY(alloc(b_chat_, N, &chat));
   SK_TRY(alloc(b_lhat_, N, &lhat));
@@ -449,6 +507,14 @@ int EngineImpl<T>::upload() {
   a.force = 0;
   a.report = 0;
 #undef SK_TRY
+  // the host vectors above are sources of asynchronous copies
+  SK_HIP(hipStreamSynchronize(stream_), kStatusSyncFailed);
+  final_state_.assign(static_cast<size_t>(K_), InstState{});
+  extracted_.assign(static_cast<size_t>(K_), 0);
+  bestx_.assign(static_cast<size_t>(n_), 0.0);
+  besty_.assign(static_cast<size_t>(m_), 0.0);
+  act_.assign(static_cast<size_t>(m_), 0.0);
+  red_.assign(static_cast<size_t>(n_), 0.0);
   return kStatusSuccess;
 }
 
@@ -593,7 +659,9 @@ int EngineImpl<T>::enqueue_block() {
       if (st) return st;
     }
   }
-  return kStatusSuccess;
+  // the final evaluation of every instance that stopped in this block, so its answer can be
+  // read back as soon as the block completes
+  return enqueue_eval(1);
 }
 
 template <class T>
@@ -669,6 +737,153 @@ int EngineImpl<T>::run_loop() {
     SK_HIP(hipMemcpy(args_.state, cur.data(), cur.size() * sizeof(InstState), hipMemcpyHostToDevice), kStatusCopyFailed);
   }
   return kStatusSuccess;
+}
+
+// ---- serving: non-blocking solve ------------------------------------------------------------
+
+template <class T>
+int EngineImpl<T>::launch_block() {
+  // ring slot of this block; poll() has consumed the block that used it before
+  const size_t slot = static_cast<size_t>(launched_ & 1);
+  hipStream_t s = stream_;
+  if (graph_exec_) {
+    SK_HIP(hipGraphLaunch(graph_exec_, s), kStatusLaunchFailed);
+  } else {
+    const int st = enqueue_block();
+    if (st) return st;
+  }
+  SK_HIP(hipMemcpyAsync(&host_done_[slot], args_.done_count, sizeof(uint32_t), hipMemcpyDeviceToHost, s),
+         kStatusCopyFailed);
+  SK_HIP(hipMemcpyAsync(host_state_ + slot * static_cast<size_t>(K_), args_.state,
+                        static_cast<size_t>(K_) * sizeof(InstState), hipMemcpyDeviceToHost, s),
+         kStatusCopyFailed);
+  SK_HIP(hipEventRecord(block_ev_[slot], s), kStatusLaunchFailed);
+  ++launched_;
+  ++inflight_;
+  return kStatusSuccess;
+}
+
+template <class T>
+int EngineImpl<T>::start() {
+  if (!stream_) {
+    set_last_error("start() before upload()");
+    return kStatusInvalidState;
+  }
+  if (inflight_ > 0) {
+    set_last_error("start() while the previous solve is still running");
+    return kStatusInvalidState;
+  }
+  started_ = std::chrono::steady_clock::now();
+  int st = setup_device();
+  if (st) return st;
+  if (opt_.use_graph && !graph_exec_) {
+    if ((st = capture(&EngineImpl<T>::enqueue_block, &graph_exec_))) return st;
+  }
+  launched_ = 0;
+  inflight_ = 0;
+  extracted_count_ = 0;
+  std::fill(extracted_.begin(), extracted_.end(), uint8_t{0});
+  timings_ = EngineTimings{};
+  // two blocks in flight, so the device never waits for the host's read-back; one is enough
+  // when every instance stopped before its first step (it runs their final evaluation)
+  if ((st = launch_block())) return st;
+  if (initially_done_ < static_cast<uint32_t>(K_)) st = launch_block();
+  return st;
+}
+
+// Reads back the answers of the instances that a block's snapshot shows as stopped and that
+// were not read yet. Their report is complete (the block ends with the final evaluation of
+// every instance that stopped in it) and no later kernel writes their slices, so the copies
+// run on the second stream while the next block computes.
+template <class T>
+int EngineImpl<T>::extract_finished(const InstState* snapshot, std::vector<int32_t>* newly_done) {
+  hipStream_t c = copy_stream_;
+  std::vector<int32_t> fresh;
+  for (int32_t k = 0; k < K_; ++k) {
+    if (snapshot[k].done && !extracted_[static_cast<size_t>(k)] && !snapshot[k].report_pending)
+      fresh.push_back(k);
+  }
+  for (const int32_t k : fresh) {
+    final_state_[static_cast<size_t>(k)] = snapshot[k];
+    const InstDesc& d = desc_[static_cast<size_t>(k)];
+    const auto c0 = static_cast<size_t>(d.col0), r0 = static_cast<size_t>(d.row0);
+    const auto nc = static_cast<size_t>(d.cols), nr = static_cast<size_t>(d.rows);
+    if (nc) {
+      SK_HIP(hipMemcpyAsync(bestx_.data() + c0, args_.bestx + c0, nc * sizeof(double), hipMemcpyDeviceToHost, c), kStatusCopyFailed);
+      SK_HIP(hipMemcpyAsync(red_.data() + c0, args_.reduced + c0, nc * sizeof(double), hipMemcpyDeviceToHost, c), kStatusCopyFailed);
+    }
+    if (nr) {
+      SK_HIP(hipMemcpyAsync(besty_.data() + r0, args_.besty + r0, nr * sizeof(double), hipMemcpyDeviceToHost, c), kStatusCopyFailed);
+      SK_HIP(hipMemcpyAsync(act_.data() + r0, args_.activity + r0, nr * sizeof(double), hipMemcpyDeviceToHost, c), kStatusCopyFailed);
+    }
+  }
+  if (!fresh.empty()) SK_HIP(hipStreamSynchronize(c), kStatusSyncFailed);
+  for (const int32_t k : fresh) {
+    extracted_[static_cast<size_t>(k)] = 1;
+    if (newly_done) newly_done->push_back(k);
+  }
+  extracted_count_ += static_cast<uint32_t>(fresh.size());
+  return kStatusSuccess;
+}
+
+// Time limit of a non-blocking solve: drain the device, stop every running instance as the
+// blocking loop does, run their final evaluation and read everything back.
+template <class T>
+int EngineImpl<T>::stop_on_time_limit(std::vector<int32_t>* newly_done) {
+  hipStream_t s = stream_;
+  SK_HIP(hipStreamSynchronize(s), kStatusSyncFailed);
+  inflight_ = 0;
+  std::vector<InstState> cur(static_cast<size_t>(K_));
+  SK_HIP(hipMemcpy(cur.data(), args_.state, cur.size() * sizeof(InstState), hipMemcpyDeviceToHost), kStatusCopyFailed);
+  for (auto& x : cur) {
+    if (!x.done) {
+      x.done = 1;
+      x.status = kStTimeLimit;
+      x.report_pending = 1;
+      x.eval_pending = 0;
+    }
+  }
+  SK_HIP(hipMemcpy(args_.state, cur.data(), cur.size() * sizeof(InstState), hipMemcpyHostToDevice), kStatusCopyFailed);
+  int st = enqueue_eval(1);
+  if (st) return st;
+  SK_HIP(hipMemcpyAsync(host_state_, args_.state, cur.size() * sizeof(InstState), hipMemcpyDeviceToHost, s),
+         kStatusCopyFailed);
+  SK_HIP(hipStreamSynchronize(s), kStatusSyncFailed);
+  return extract_finished(host_state_, newly_done);
+}
+
+template <class T>
+int EngineImpl<T>::poll(bool wait, bool* finished, std::vector<int32_t>* newly_done) {
+  *finished = false;
+  if (inflight_ == 0) {
+    *finished = extracted_count_ >= static_cast<uint32_t>(K_);
+    return kStatusSuccess;
+  }
+  const size_t slot = static_cast<size_t>((launched_ - inflight_) & 1);  // the oldest block
+  const hipError_t q = wait ? hipEventSynchronize(block_ev_[slot]) : hipEventQuery(block_ev_[slot]);
+  if (q == hipErrorNotReady) return kStatusSuccess;
+  if (q != hipSuccess) {
+    set_last_error(std::string("hipEventQuery: ") + hipGetErrorString(q));
+    return kStatusSyncFailed;
+  }
+  --inflight_;
+  int st = kStatusSuccess;
+  if (host_done_[slot] > extracted_count_) {
+    st = extract_finished(host_state_ + slot * static_cast<size_t>(K_), newly_done);
+    if (st) return st;
+  }
+  timings_.graph_launches = launched_;
+  if (extracted_count_ >= static_cast<uint32_t>(K_)) {
+    *finished = inflight_ == 0;  // else the block still in flight is drained by the next poll
+    return kStatusSuccess;
+  }
+  const double elapsed = std::chrono::duration<double>(std::chrono::steady_clock::now() - started_).count();
+  if (elapsed > opt_.time_limit) {
+    st = stop_on_time_limit(newly_done);
+    *finished = !st;
+    return st;
+  }
+  return launch_block();
 }
 
 template <class T>
@@ -802,7 +1017,65 @@ double EngineImpl<T>::kernel_bytes(int kernel) const {
   return (w + 4.0) * z + 4.0 * (n + 1.0) + w * m + n * 4.0 * w;
 }
 
+template <class T>
+int64_t EngineImpl<T>::device_bytes() const {
+  int64_t total = 0;
+  for (const DevBuf* b : buffers()) total += static_cast<int64_t>(b->bytes);
+  return total;
+}
+
+template <class T>
+void EngineImpl<T>::release() {
+  if (stream_) (void)hipStreamSynchronize(stream_);
+  if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
+  if (setup_exec_) (void)hipGraphExecDestroy(setup_exec_);
+  graph_exec_ = setup_exec_ = nullptr;
+  for (DevBuf* b : buffers()) {
+    if (b->ptr) (void)hipFree(b->ptr);
+    b->ptr = nullptr;
+    b->bytes = 0;
+  }
+  args_ = Args<T>{};
+  inflight_ = 0;
+  K_ = 0;
+}
+
 }  // namespace
+
+// Per-element byte counts of the buffers upload() allocates (w = iterate bytes):
+//   nonzero: two index arrays, the FP64 originals and FP64 scaled values in both layouts,
+//            plus the iterate-precision copies when w = 4; about 1 B of tile tables
+//   row:     pointer, bounds, scaling, iterates (y, y', restart point: w each, 2 scaled
+//            bounds), sums and candidates, best, activity, power-iteration image
+//   column:  pointer, cost, bounds, scaling, 3 scaled vectors + FP64 bounds, iterates (x, x',
+//            A^T y twice, xbar, restart point), sums, candidates, best, reduced costs, power
+//            iteration vectors
+//   instance: state record, descriptor and minimum tile entries
+// plus a fixed allowance for the allocation granularity of some 60 buffers.
+int query_device_memory(int64_t* free_bytes, int64_t* total_bytes) {
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count == 0) {
+    set_last_error("no HIP device is available");
+    return kStatusNoDevice;
+  }
+  size_t f = 0, t = 0;
+  SK_HIP(hipMemGetInfo(&f, &t), kStatusInitFailed);
+  *free_bytes = static_cast<int64_t>(f);
+  *total_bytes = static_cast<int64_t>(t);
+  return kStatusSuccess;
+}
+
+int64_t estimate_device_bytes(int64_t rows, int64_t cols, int64_t nnz, int64_t instances,
+                              int32_t precision) {
+  const double w = precision == 1 ? 8.0 : 4.0;
+  const double per_nnz = 40.0 + (precision == 1 ? 0.0 : 8.0) + 1.1;
+  const double per_row = 84.0 + 5.0 * w + 0.5;
+  const double per_col = 116.0 + 9.0 * w + 0.5;
+  const double bytes = per_nnz * static_cast<double>(nnz) + per_row * static_cast<double>(rows) +
+                       per_col * static_cast<double>(cols) + 1024.0 * static_cast<double>(instances) +
+                       262144.0;
+  return static_cast<int64_t>(std::ceil(bytes));
+}
 
 void set_last_error(const std::string& message) { g_last_error = message; }
 const char* last_error() { return g_last_error.c_str(); }

@@ -63,6 +63,20 @@ class Engine {
   virtual int upload() = 0;  // host CSR/tiles + device allocation + H2D (not part of a solve)
   virtual int solve() = 0;   // everything on the device, from the uploaded originals
   virtual int result(int32_t index, InstanceResult* out) const = 0;
+  // Forget the instances added so far (a serving slot taking its next batch); device buffers
+  // are kept and reused by the next upload() when large enough.
+  virtual void clear() = 0;
+  // Non-blocking solve for a serving loop: start() enqueues the set-up and the first blocks
+  // and returns; each poll() returns at once, appends the instances that stopped since the
+  // last call to `newly_done` (result() answers for them from then on, while the rest of the
+  // batch keeps running) and sets `finished` once every instance is done and the device is
+  // idle. solve() is start() followed by poll() until finished, minus the early read-backs.
+  // With `wait`, poll() first blocks until the oldest block in flight has completed.
+  virtual int start() = 0;
+  virtual int poll(bool wait, bool* finished, std::vector<int32_t>* newly_done) = 0;
+  // Device memory the engine holds, and a way to give it back (an idle serving slot).
+  virtual int64_t device_bytes() const = 0;
+  virtual void release() = 0;
   virtual const EngineTimings& timings() const = 0;
   virtual int32_t size() const = 0;
   // Launch one of the step kernels `reps` times in timing mode between two HIP events on the
@@ -77,6 +91,14 @@ class Engine {
 };
 
 std::unique_ptr<Engine> make_engine(const EngineOptions& options);
+
+// Device bytes an engine allocates for a batch with these totals (rows, columns, nonzeros
+// and instances summed over the batch), rounded up; used for admission by memory footprint.
+int64_t estimate_device_bytes(int64_t rows, int64_t cols, int64_t nnz, int64_t instances,
+                              int32_t precision);
+
+// Free and total memory of the current HIP device.
+int query_device_memory(int64_t* free_bytes, int64_t* total_bytes);
 
 // Thread-local description of the last failure inside the library.
 void set_last_error(const std::string& message);

@@ -561,6 +561,10 @@ __global__ __launch_bounds__(kBlock) void k_eval_finalize(Args<T> a) {
   if (a.report) {
     s.report = r[0];
     s.report_pending = 0;
+    // nothing may rewrite a reported instance's answer: later commit stages skip it, so the
+    // host can read it back while the rest of the batch keeps running
+    s.copy_best = 0;
+    s.do_restart = 0;
     return;
   }
   decide_evaluation(s, r[0], r[1], v[16], v[17], a.done_count);

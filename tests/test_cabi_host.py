@@ -150,3 +150,20 @@ def test_power_law_preset_hits_exact_nnz():
     assert (m, n, z) == (2000, 5000, 25000)
     deg = np.diff(lp.export()["col_starts"])
     assert deg.max() <= 2000 and deg.min() >= 1
+
+
+def test_server_options_and_missing_device():
+    so = _ffi.ServerOptions()
+    sk.lib().sk_server_default_options(C.byref(so))
+    assert (so.slots, so.max_batch, so.memory_budget, so.keep_solutions) == (4, 256, 0, 1)
+    if sk.gpu_available():
+        pytest.skip("a device is visible; the missing-device path is for CPU-only hosts")
+    h = C.c_void_p()
+    st = sk.lib().sk_server_create(None, C.byref(so), C.byref(h))
+    assert st == 2 and not h.value  # SK_GPU_NO_DEVICE, nothing created
+    bad = _ffi.ServerOptions()
+    sk.lib().sk_server_default_options(C.byref(bad))
+    bad.slots = 0
+    assert sk.lib().sk_server_create(None, C.byref(bad), C.byref(h)) == 10  # invalid argument
+    # null handles are argument errors, not crashes
+    assert sk.lib().sk_server_submit(None, None, -1.0, None) == 10
