@@ -97,11 +97,10 @@ def test_memory_budget_admission_and_rejection(gpu):
     for lp in small:
         m, n, _ = lp.dims
         dims[srv.submit(lp)] = (m, n)
-    big = gpu.Lp.generate(1, 1)
+    big = gpu.Lp.generate(4, 2)  # 34688 x 69376, 346880 nonzeros: far above the budget
     big_dims = big.dims
-    for _ in range(1):
-        with pytest.raises(gpu.SkError):
-            srv.submit(big)  # far larger than the budget
+    with pytest.raises(gpu.SkError):
+        srv.submit(big)
     assert big.dims == big_dims  # refused: the caller still owns a whole model
     got = _drain(srv, dims)
     st = srv.stats()
