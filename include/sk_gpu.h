@@ -178,7 +178,7 @@ typedef struct {
                                 default), 1 "fp64" */
   int32_t steps_per_block;   /* PDHG steps per captured graph, default 40 */
   int32_t use_graph;         /* 1: hipGraph-captured step blocks (default) */
-  int32_t spmv_stage;        /* SpMV schedule: -1 autotune (default), 0 direct, 1 LDS-staged */
+  int32_t spmv_stage;        /* SpMV staging order: -1 default (1), 0 CSR order, 1 sorted by gather index */
   int32_t tile_nnz;          /* row-block capacity in nonzeros, 0 = automatic */
   int32_t compute_quality;   /* 1: fill the recompute_quality measures (default) */
   int32_t eval_stride;       /* an evaluation stage follows every this many steps inside a

@@ -111,11 +111,16 @@ __device__ __forceinline__ int pow2_floor(int v) {
 // tile's nonzeros into LDS with coalesced loads (CSR-Stream; Greathouse & Daga, SC'14), then
 // G = 256 / rows lanes (power of two, <= 64) reduce each row from LDS. Without STAGE the
 // same G-lane groups read their row straight from memory.
-// Vector tile (one row with nnz > cap): the workgroup strides over the row and reduces.
+// Staging order: with `pos`, the tile's nonzeros are stored sorted by gather index, so the
+// lanes of one load instruction gather from a few cache lines, and pos[p] is the CSR-order
+// slot (within the tile) that product p is staged to. The rows are then reduced in CSR order
+// exactly as without `pos`. Without `pos` the staging order is the CSR order.
+// Vector tile (one row with nnz > cap): the workgroup strides over the row and reduces; its
+// staging order is always the CSR order.
 template <class A, bool STAGE, class Prod, class Pre, class Epi>
 __device__ __forceinline__ void rowblock_product(const Tile t, const int32_t* __restrict__ ptr,
-                                                 Prod prod, Pre pre, Epi epi, A* stage,
-                                                 A* red_lds) {
+                                                 const uint16_t* __restrict__ pos, Prod prod,
+                                                 Pre pre, Epi epi, A* stage, A* red_lds) {
   const int tid = threadIdx.x;
   const int r0 = t.begin, nrows = t.end - t.begin;
   const int p0 = ptr[r0];
@@ -152,15 +157,19 @@ __device__ __forceinline__ void rowblock_product(const Tile t, const int32_t* __
     constexpr int U = 8;
     for (int base = 0; base < nnz; base += U * kBlock) {
       A part[U];
+      int slot[U];
 #pragma unroll
       for (int u = 0; u < U; ++u) {
         const int k = base + u * kBlock + tid;
-        if (k < nnz) part[u] = prod(p0 + k);
+        if (k < nnz) {
+          part[u] = prod(p0 + k);
+          slot[u] = pos ? static_cast<int>(pos[p0 + k]) : k;
+        }
       }
 #pragma unroll
       for (int u = 0; u < U; ++u) {
         const int k = base + u * kBlock + tid;
-        if (k < nnz) stage[k] = part[u];
+        if (k < nnz) stage[slot[u]] = part[u];
       }
     }
     __syncthreads();

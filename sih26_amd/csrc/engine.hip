@@ -186,7 +186,7 @@ class EngineImpl final : public Engine {
 
   // device buffers
   DevBuf b_a_ptr_, b_a_idx_, b_a_orig_, b_t_ptr_, b_t_idx_, b_t_orig_;
-  DevBuf b_sa_, b_st_, b_a_val_, b_t_val_;
+  DevBuf b_sa_, b_st_, b_a_val_, b_t_val_, b_a_sidx_, b_t_sidx_, b_a_pos_, b_t_pos_;
   DevBuf b_cmin_, b_cl_, b_cu_, b_rl_, b_ru_, b_dc_, b_dr_, b_rpass_, b_cpass_;
   DevBuf b_chat_, b_lhat_, b_uhat_, b_rlhat_, b_ruhat_, b_lhat64_, b_uhat64_;
   DevBuf b_x0_, b_x1_, b_y0_, b_y1_, b_aty0_, b_aty1_, b_xbar_, b_xrst_, b_yrst_;
@@ -196,7 +196,7 @@ class EngineImpl final : public Engine {
   DevBuf b_done_, b_v_, b_v0_, b_av_, b_len_, b_zero_;
   std::vector<DevBuf*> buffers() {
     return {&b_a_ptr_, &b_a_idx_, &b_a_orig_, &b_t_ptr_, &b_t_idx_, &b_t_orig_, &b_sa_, &b_st_,
-            &b_a_val_, &b_t_val_, &b_cmin_, &b_cl_, &b_cu_, &b_rl_, &b_ru_, &b_dc_, &b_dr_,
+            &b_a_val_, &b_t_val_, &b_a_sidx_, &b_t_sidx_, &b_a_pos_, &b_t_pos_, &b_cmin_, &b_cl_, &b_cu_, &b_rl_, &b_ru_, &b_dc_, &b_dr_,
             &b_rpass_, &b_cpass_, &b_chat_, &b_lhat_, &b_uhat_, &b_rlhat_, &b_ruhat_,
             &b_lhat64_, &b_uhat64_, &b_x0_, &b_x1_, &b_y0_, &b_y1_, &b_aty0_, &b_aty1_,
             &b_xbar_, &b_xrst_, &b_yrst_, &b_xsum_, &b_ysum_, &b_xu0_, &b_xu1_, &b_yu0_,
@@ -356,7 +356,7 @@ int EngineImpl<T>::upload() {
       }
     }
   }
-  // SpMV schedule: LDS-staged row blocks unless told otherwise.
+  // Staging order of the SpMV tiles: sorted by gather index unless told otherwise.
   stage_ = opt_.stage < 0 ? 1 : opt_.stage;
   for (int32_t k = 0; k < K_; ++k) {
     InstDesc& d = desc_[static_cast<size_t>(k)];
@@ -419,13 +419,8 @@ int EngineImpl<T>::upload() {
   SK_TRY(alloc(b_sa_, Z, &d_sa_));
   SK_TRY(alloc(b_st_, Z, &d_st_));
   T *av, *tv;
-  if (sizeof(T) == sizeof(double)) {
-    av = reinterpret_cast<T*>(d_sa_);
-    tv = reinterpret_cast<T*>(d_st_);
-  } else {
-    SK_TRY(alloc(b_a_val_, Z, &av));
-    SK_TRY(alloc(b_t_val_, Z, &tv));
-  }
+  SK_TRY(alloc(b_a_val_, Z, &av));
+  SK_TRY(alloc(b_t_val_, Z, &tv));
   a.a_val = av;
   a.t_val = tv;
   double *pcm, *pcl, *pcu, *prl, *pru, *pdc, *pdr;
@@ -486,6 +481,22 @@ int EngineImpl<T>::upload() {
   a.tr = ptr_;
   a.ta = pta;
   a.tt = ptt;
+  {
+    // staging order of the step kernels' tiles (structure only, so once per upload)
+    int32_t *asx, *tsx;
+    uint16_t *apos, *tpos;
+    SK_TRY(alloc(b_a_sidx_, Z, &asx));
+    SK_TRY(alloc(b_t_sidx_, Z, &tsx));
+    SK_TRY(alloc(b_a_pos_, Z, &apos));
+    SK_TRY(alloc(b_t_pos_, Z, &tpos));
+    const int32_t sorted = stage_ != 0;
+    SK_LAUNCH(k_sort_tiles<<<n_ta_, kBlock, 0, stream_>>>(pta, ap, ai, sorted, asx, apos));
+    SK_LAUNCH(k_sort_tiles<<<n_tt_, kBlock, 0, stream_>>>(ptt, tp, ti, sorted, tsx, tpos));
+    a.a_sidx = asx;
+    a.t_sidx = tsx;
+    a.a_pos = apos;
+    a.t_pos = tpos;
+  }
   InstDesc* pdesc;
   SK_TRY(put(b_desc_, desc_, &pdesc));
   a.desc = pdesc;
@@ -558,10 +569,8 @@ int EngineImpl<T>::enqueue_setup() {
     SK_LAUNCH(k_scaled_vectors<double><<<blocks_for(N), kBlock, 0, s>>>(
         a64, a.xu0, d_lhat64_, d_uhat64_, a.yu0, a.yu1, N, 0));
   }
-  if (sizeof(T) != sizeof(double)) {
-    SK_LAUNCH(k_convert<T><<<blocks_for(Z), kBlock, 0, s>>>(d_sa_, const_cast<T*>(a.a_val), Z));
-    SK_LAUNCH(k_convert<T><<<blocks_for(Z), kBlock, 0, s>>>(d_st_, const_cast<T*>(a.t_val), Z));
-  }
+  SK_LAUNCH(k_stage_values<T><<<n_ta_, kBlock, 0, s>>>(a.ta, a.a_ptr, a.a_pos, d_sa_, const_cast<T*>(a.a_val)));
+  SK_LAUNCH(k_stage_values<T><<<n_tt_, kBlock, 0, s>>>(a.tt, a.t_ptr, a.t_pos, d_st_, const_cast<T*>(a.t_val)));
   // power iteration on A^T A, 30 steps (scaling.cpp:161-194)
   SK_LAUNCH(k_copy<<<blocks_for(N), kBlock, 0, s>>>(d_v0_, d_v_, N));
   SK_LAUNCH(k_fill_i32<<<blocks_for(K_), kBlock, 0, s>>>(d_zero_seen_, 0, K_));
@@ -645,13 +654,8 @@ int EngineImpl<T>::enqueue_block() {
   hipStream_t s = stream_;
   for (int k = 0; k < opt_.steps_per_block; ++k) {
     SK_LAUNCH(k_step_primal<T><<<n_tc_, kBlock, 0, s>>>(a));
-    if (stage_) {
-      SK_LAUNCH(k_step_dual<T, true><<<n_ta_, kBlock, 0, s>>>(a));
-      SK_LAUNCH(k_step_aty<T, true><<<n_tt_, kBlock, 0, s>>>(a));
-    } else {
-      SK_LAUNCH(k_step_dual<T, false><<<n_ta_, kBlock, 0, s>>>(a));
-      SK_LAUNCH(k_step_aty<T, false><<<n_tt_, kBlock, 0, s>>>(a));
-    }
+    SK_LAUNCH(k_step_dual<T><<<n_ta_, kBlock, 0, s>>>(a));
+    SK_LAUNCH(k_step_aty<T><<<n_tt_, kBlock, 0, s>>>(a));
     SK_LAUNCH(k_step_finalize<T><<<K_, kBlock, 0, s>>>(a));
     // an instance that became due for evaluation waits at most eval_stride steps
     if ((k + 1) % opt_.eval_stride == 0 || k + 1 == opt_.steps_per_block) {
@@ -980,10 +984,8 @@ int EngineImpl<T>::time_kernel(int kernel, int reps, double* mean_ms) {
   SK_HIP(hipEventCreate(&e1), kStatusInitFailed);
   auto one = [&]() -> int {
     if (kernel == 0) SK_LAUNCH(k_step_primal<T><<<n_tc_, kBlock, 0, s>>>(a));
-    else if (kernel == 1 && stage_) SK_LAUNCH(k_step_dual<T, true><<<n_ta_, kBlock, 0, s>>>(a));
-    else if (kernel == 1) SK_LAUNCH(k_step_dual<T, false><<<n_ta_, kBlock, 0, s>>>(a));
-    else if (stage_) SK_LAUNCH(k_step_aty<T, true><<<n_tt_, kBlock, 0, s>>>(a));
-    else SK_LAUNCH(k_step_aty<T, false><<<n_tt_, kBlock, 0, s>>>(a));
+    else if (kernel == 1) SK_LAUNCH(k_step_dual<T><<<n_ta_, kBlock, 0, s>>>(a));
+    else SK_LAUNCH(k_step_aty<T><<<n_tt_, kBlock, 0, s>>>(a));
     return kStatusSuccess;
   };
   int st = one();  // warm
@@ -1043,8 +1045,8 @@ void EngineImpl<T>::release() {
 }  // namespace
 
 // Per-element byte counts of the buffers upload() allocates (w = iterate bytes):
-//   nonzero: two index arrays, the FP64 originals and FP64 scaled values in both layouts,
-//            plus the iterate-precision copies when w = 4; about 1 B of tile tables
+//   nonzero: in both layouts, the CSR-order index, the FP64 original and FP64 scaled value,
+//            and the staging-order index, value (w) and 16-bit slot; about 1 B of tile tables
 //   row:     pointer, bounds, scaling, iterates (y, y', restart point: w each, 2 scaled
 //            bounds), sums and candidates, best, activity, power-iteration image
 //   column:  pointer, cost, bounds, scaling, 3 scaled vectors + FP64 bounds, iterates (x, x',
@@ -1068,7 +1070,7 @@ int query_device_memory(int64_t* free_bytes, int64_t* total_bytes) {
 int64_t estimate_device_bytes(int64_t rows, int64_t cols, int64_t nnz, int64_t instances,
                               int32_t precision) {
   const double w = precision == 1 ? 8.0 : 4.0;
-  const double per_nnz = 40.0 + (precision == 1 ? 0.0 : 8.0) + 1.1;
+  const double per_nnz = 40.0 + 2.0 * w + 12.0 + 1.1;
   const double per_row = 84.0 + 5.0 * w + 0.5;
   const double per_col = 116.0 + 9.0 * w + 0.5;
   const double bytes = per_nnz * static_cast<double>(nnz) + per_row * static_cast<double>(rows) +

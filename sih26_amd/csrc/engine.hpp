@@ -29,7 +29,7 @@ struct EngineOptions {
   int32_t steps_per_block = 40;   // PDHG steps per captured graph
   int32_t eval_stride = 10;       // evaluation stage after every this many steps
   int32_t use_graph = 1;
-  int32_t stage = -1;             // SpMV schedule: -1 autotune, 0 direct, 1 LDS-staged
+  int32_t stage = -1;             // staging order of SpMV tiles: -1 default (1), 0 CSR, 1 sorted
   int32_t tile_nnz = 0;           // row-block capacity; 0 picks per instance
 };
 

@@ -37,13 +37,20 @@ namespace sk {
 // (float for the FP32/FP64 split engine, double for the plain FP64 engine).
 template <class T>
 struct Args {
-  // CSR(A-hat) and CSR(A-hat^T) in the iterate precision (the scaled matrix)
+  // CSR(A) and CSR(A^T): row pointers and CSR-order column indices
   const int32_t* __restrict__ a_ptr;
   const int32_t* __restrict__ a_idx;
-  const T* __restrict__ a_val;
   const int32_t* __restrict__ t_ptr;
   const int32_t* __restrict__ t_idx;
+  // the scaled matrix A-hat for the step kernels, in each tile's staging order (sorted by
+  // gather index; see rowblock_product): gather index, value in the iterate precision, and
+  // the entry's CSR-order slot within its tile
+  const int32_t* __restrict__ a_sidx;
+  const T* __restrict__ a_val;
+  const uint16_t* __restrict__ a_pos;
+  const int32_t* __restrict__ t_sidx;
   const T* __restrict__ t_val;
+  const uint16_t* __restrict__ t_pos;
   // the original A in the same two layouts, FP64 (evaluation)
   const double* __restrict__ a_orig;
   const double* __restrict__ t_orig;
@@ -139,9 +146,9 @@ __global__ __launch_bounds__(kBlock) void k_step_primal(Args<T> a) {
   if (threadIdx.x == 0) a.ps_c[tile] = d2;
 }
 
-template <class T, bool STAGE>
+template <class T>
 __global__ __launch_bounds__(kBlock) void k_step_dual(Args<T> a) {
-  __shared__ T stage[STAGE ? kMaxTileNnz : 1];
+  __shared__ T stage[kMaxTileNnz];
   __shared__ T red_t[kBlock / kWave];
   __shared__ double red[kBlock / kWave];
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
@@ -155,12 +162,12 @@ __global__ __launch_bounds__(kBlock) void k_step_dual(Args<T> a) {
   T* __restrict__ yn = p ? a.y0 : a.y1;
   const T* __restrict__ xbar = a.xbar;
   const T* __restrict__ val = a.a_val;
-  const int32_t* __restrict__ idx = a.a_idx;
+  const int32_t* __restrict__ idx = a.a_sidx;
   double d2 = 0.0;
   T yi = 0, lo = 0, hi = 0;
   double sum = 0.0;
-  rowblock_product<T, STAGE>(
-      t, a.a_ptr, [&](int q) { return val[q] * xbar[idx[q]]; },
+  rowblock_product<T, true>(
+      t, a.a_ptr, a.a_pos, [&](int q) { return val[q] * xbar[idx[q]]; },
       [&](int i) {
         yi = y[i];
         lo = a.rlhat[i];
@@ -243,9 +250,9 @@ __device__ void decide_step(InstState& s, double dx2, double dy2, double inter_s
   }
 }
 
-template <class T, bool STAGE>
+template <class T>
 __global__ __launch_bounds__(kBlock) void k_step_aty(Args<T> a) {
-  __shared__ T stage[STAGE ? kMaxTileNnz : 1];
+  __shared__ T stage[kMaxTileNnz];
   __shared__ T red_t[kBlock / kWave];
   __shared__ double red[kBlock / kWave];
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
@@ -259,11 +266,11 @@ __global__ __launch_bounds__(kBlock) void k_step_aty(Args<T> a) {
   const T* __restrict__ aty = p ? a.aty1 : a.aty0;
   T* __restrict__ atyn = p ? a.aty0 : a.aty1;
   const T* __restrict__ val = a.t_val;
-  const int32_t* __restrict__ idx = a.t_idx;
+  const int32_t* __restrict__ idx = a.t_sidx;
   double inter = 0.0;
   T xj = 0, xnj = 0, atyj = 0;
-  rowblock_product<T, STAGE>(
-      t, a.t_ptr, [&](int q) { return val[q] * yn[idx[q]]; },
+  rowblock_product<T, true>(
+      t, a.t_ptr, a.t_pos, [&](int q) { return val[q] * yn[idx[q]]; },
       [&](int j) {
         xj = x[j];
         xnj = xn[j];
@@ -353,7 +360,7 @@ __global__ __launch_bounds__(kBlock) void k_eval_rows(Args<T> a) {
   const int32_t* __restrict__ idx = a.a_idx;
   double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   rowblock_product<D2, false>(
-      t, a.a_ptr,
+      t, a.a_ptr, nullptr,
       [&](int q) {
         const int c = idx[q];
         return D2{val[q] * xa[c], val[q] * xb[c]};
@@ -494,7 +501,7 @@ __global__ __launch_bounds__(kBlock) void k_eval_cols(Args<T> a) {
   const int32_t* __restrict__ idx = a.t_idx;
   double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   rowblock_product<D2, false>(
-      t, a.t_ptr,
+      t, a.t_ptr, nullptr,
       [&](int q) {
         const int r = idx[q];
         return D2{val[q] * ya[r], val[q] * yb[r]};
@@ -673,6 +680,71 @@ __global__ __launch_bounds__(kBlock) void k_convert(const double* __restrict__ s
   if (k < n) dst[k] = static_cast<T>(src[k]);
 }
 
+// Staging order of every row-block tile (structure only; run once per upload). A stream tile's
+// nonzeros are sorted by gather index (column of A for CSR(A), row of A for CSR(A^T)), ties
+// broken by CSR position, with a bitonic sort of (index << 16 | slot) keys in LDS; sidx gets
+// the sorted indices and pos the CSR-order slot of each. The keys are distinct, so the order
+// is unique. With `sorted` = 0, or for a vector tile, the staging order is the CSR order.
+__global__ __launch_bounds__(kBlock) void k_sort_tiles(const Tile* __restrict__ tiles,
+                                                        const int32_t* __restrict__ ptr,
+                                                        const int32_t* __restrict__ idx,
+                                                        int32_t sorted, int32_t* __restrict__ sidx,
+                                                        uint16_t* __restrict__ pos) {
+  __shared__ uint64_t keys[kMaxTileNnz];
+  const Tile t = tiles[blockIdx.x];
+  const int tid = threadIdx.x;
+  const int p0 = ptr[t.begin], nnz = ptr[t.end] - p0;
+  const bool vector = t.end - t.begin == 1 && nnz > t.cap;
+  if (!sorted || vector) {
+    for (int k = tid; k < nnz; k += kBlock) {
+      sidx[p0 + k] = idx[p0 + k];
+      pos[p0 + k] = static_cast<uint16_t>(k);
+    }
+    return;
+  }
+  // a stream tile holds at most cap <= kMaxTileNnz nonzeros, so slots fit 16 bits
+  int n2 = 1;
+  while (n2 < nnz) n2 <<= 1;
+  for (int k = tid; k < n2; k += kBlock)
+    keys[k] = k < nnz ? (static_cast<uint64_t>(static_cast<uint32_t>(idx[p0 + k])) << 16) |
+                            static_cast<uint64_t>(k)
+                      : ~0ull;
+  __syncthreads();
+  for (int size = 2; size <= n2; size <<= 1) {
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      for (int i = tid; i < n2 / 2; i += kBlock) {
+        const int lo = 2 * stride * (i / stride) + (i % stride), hi = lo + stride;
+        const bool ascending = (lo & size) == 0;
+        const uint64_t u = keys[lo], v = keys[hi];
+        if ((u > v) == ascending) {
+          keys[lo] = v;
+          keys[hi] = u;
+        }
+      }
+      __syncthreads();
+    }
+  }
+  for (int k = tid; k < nnz; k += kBlock) {
+    sidx[p0 + k] = static_cast<int32_t>(keys[k] >> 16);
+    pos[p0 + k] = static_cast<uint16_t>(keys[k] & 0xFFFFu);
+  }
+}
+
+// The scaled values in each tile's staging order, in the iterate precision (a vector tile keeps
+// the CSR order; every other tile reads through its slot map).
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_stage_values(const Tile* __restrict__ tiles,
+                                                          const int32_t* __restrict__ ptr,
+                                                          const uint16_t* __restrict__ pos,
+                                                          const double* __restrict__ src,
+                                                          T* __restrict__ dst) {
+  const Tile t = tiles[blockIdx.x];
+  const int p0 = ptr[t.begin], nnz = ptr[t.end] - p0;
+  const bool vector = t.end - t.begin == 1 && nnz > t.cap;
+  for (int k = threadIdx.x; k < nnz; k += kBlock)
+    dst[p0 + k] = static_cast<T>(src[p0 + (vector ? k : static_cast<int>(pos[p0 + k]))]);
+}
+
 // ---- power iteration (scaling.cpp:161-194) ---------------------------------------------------
 
 // y = A x over the row-block tiles of a CSR matrix in FP64 (any tile schedule; the power
@@ -686,7 +758,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv_tiles(const Tile* __restrict__ 
   __shared__ double red[kBlock / kWave];
   const Tile t = tiles[xcd_tile(blockIdx.x, gridDim.x)];
   rowblock_product<double, false>(
-      t, ptr, [&](int q) { return val[q] * x[idx[q]]; }, [](int) {},
+      t, ptr, nullptr, [&](int q) { return val[q] * x[idx[q]]; }, [](int) {},
       [&](int r, double acc) { y[r] = acc; }, (double*)nullptr, red);
 }
 

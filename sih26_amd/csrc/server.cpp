@@ -157,6 +157,11 @@ bool Server::take_batch(int index, std::vector<std::unique_ptr<Request>>* batch)
         me.extent.cols = std::max(me.extent.cols, t.cols);
         me.extent.nnz = std::max(me.extent.nnz, t.nnz);
         me.extent.instances = std::max(me.extent.instances, t.instances);
+        // reserve the projection now, so slots admitting batches meanwhile count it; the
+        // slot's real holding (never larger) replaces it after the upload
+        me.held = std::max(me.held, estimate_device_bytes(me.extent.rows, me.extent.cols,
+                                                          me.extent.nnz, me.extent.instances,
+                                                          opt_.engine.precision));
         const double when = now();
         for (size_t k = 0; k < take; ++k) {
           std::unique_ptr<Request> r = std::move(queue_.front());

@@ -91,7 +91,7 @@ def test_memory_budget_admission_and_rejection(gpu):
     # a budget that holds about one C3 member at a time, so batches are single instances
     sizes = [lp.dims for lp in small]
     biggest = max(sizes, key=lambda t: t[2])
-    per = 50 * biggest[2] + 110 * biggest[0] + 160 * biggest[1] + (1 << 19)
+    per = 62 * biggest[2] + 110 * biggest[0] + 160 * biggest[1] + (1 << 19)
     srv = gpu.Server(slots=2, max_batch=8, memory_budget=int(1.5 * per), **OPTS)
     dims = {}
     for lp in small:
