@@ -188,8 +188,11 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--kernel-reps", type=int, default=50)
-    ap.add_argument("--spmv-stage", type=int, default=-1, help="-1 auto, 0 direct, 1 LDS-staged")
+    ap.add_argument("--spmv-stage", type=int, default=-1,
+                    help="staging order: -1 default, 0 CSR order, 1 sorted by gather index")
     ap.add_argument("--tile-nnz", type=int, default=0, help="row-block capacity, 0 auto")
+    ap.add_argument("--spmv-groups", type=int, default=-1,
+                    help="SpMV workgroups per CU walking the tiles (0: one per tile, -1 auto)")
     ap.add_argument("--requests", type=int, default=0,
                     help="serving stream length (0: 10000 for --workload c4, 1000 otherwise)")
     ap.add_argument("--rate", type=float, default=200.0, help="serving arrivals per second")
@@ -211,7 +214,8 @@ def main() -> int:
     if args.workload == "c4":
         return main_serving(sk, args)
     lps = instances(sk, args.workload, args.instances)
-    batch = sk.Batch(solve_options(sk, spmv_stage=args.spmv_stage, tile_nnz=args.tile_nnz))
+    batch = sk.Batch(solve_options(sk, spmv_stage=args.spmv_stage, tile_nnz=args.tile_nnz,
+                                   spmv_groups=args.spmv_groups))
     for lp in lps:
         batch.add(lp)
     batch.upload()

@@ -96,6 +96,24 @@ __device__ __forceinline__ int xcd_tile(int b, int n) {
   return x * per + min(x, extra) + k;
 }
 
+// Calls f(tile) for the tiles of an n-tile table this workgroup owns. With one workgroup per
+// tile (gridDim.x == n) that is xcd_tile(blockIdx.x, n). With a smaller grid, the workgroups
+// on XCD x (those with blockIdx.x % 8 == x) walk the x-th contiguous eighth of the table with
+// a stride of their count, so the XCD slices are the same as with one workgroup per tile.
+// f must leave LDS reusable (end on a barrier) since the next tile follows at once.
+template <bool WALK, class F>
+__device__ __forceinline__ void for_each_tile(int n, F f) {
+  if (!WALK) {
+    f(xcd_tile(blockIdx.x, gridDim.x));
+    return;
+  }
+  const int g = gridDim.x, b = blockIdx.x;
+  const int x = b % kXcds, per = n / kXcds, extra = n % kXcds;
+  const int first = x * per + min(x, extra), count = per + (x < extra ? 1 : 0);
+  const int groups = g / kXcds + (x < g % kXcds ? 1 : 0);
+  for (int i = b / kXcds; i < count; i += groups) f(first + i);
+}
+
 __device__ __forceinline__ int pow2_floor(int v) {
   int p = 1;
   while (p * 2 <= v) p *= 2;

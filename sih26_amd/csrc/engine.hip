@@ -161,6 +161,30 @@ class EngineImpl final : public Engine {
   int enqueue_block();
   int enqueue_eval(int report_mode);
   int launch_block();
+  // One step SpMV launch (1: A xbar + dual step, 2: A^T y' + interaction), instantiated for
+  // the staging capacity the tile tables were built with.
+  int launch_spmv(int kernel, const Args<T>& a) {
+    hipStream_t s = stream_;
+    const int32_t tiles = kernel == 1 ? n_ta_ : n_tt_;
+    const int32_t groups = opt_.spmv_groups < 0 ? 0 : opt_.spmv_groups;
+    const unsigned grid = static_cast<unsigned>(
+        groups > 0 ? std::max(1, std::min(tiles, groups * cus_)) : tiles);
+    auto go = [&](auto cap) -> int {
+      constexpr int C = decltype(cap)::value;
+      if (groups > 0) {
+        if (kernel == 1) SK_LAUNCH(k_step_dual<T, C, true><<<grid, kBlock, 0, s>>>(a));
+        else SK_LAUNCH(k_step_aty<T, C, true><<<grid, kBlock, 0, s>>>(a));
+      } else {
+        if (kernel == 1) SK_LAUNCH(k_step_dual<T, C, false><<<grid, kBlock, 0, s>>>(a));
+        else SK_LAUNCH(k_step_aty<T, C, false><<<grid, kBlock, 0, s>>>(a));
+      }
+      return kStatusSuccess;
+    };
+    if (cap_ <= 512) return go(std::integral_constant<int, 512>{});
+    if (cap_ <= 1024) return go(std::integral_constant<int, 1024>{});
+    if (cap_ <= 2048) return go(std::integral_constant<int, 2048>{});
+    return go(std::integral_constant<int, kMaxTileNnz>{});
+  }
   int extract_finished(const InstState* snapshot, std::vector<int32_t>* newly_done);
   int stop_on_time_limit(std::vector<int32_t>* newly_done);
 
@@ -169,6 +193,8 @@ class EngineImpl final : public Engine {
   int64_t m_ = 0, n_ = 0, nnz_ = 0;
   int32_t K_ = 0;
   int stage_ = 1;
+  int32_t cap_ = 2048;  // row-block tile capacity (nonzeros)
+  int32_t cus_ = 256;   // compute units of the device
 
   // host copies needed after upload
   std::vector<InstDesc> desc_;
@@ -360,7 +386,7 @@ int EngineImpl<T>::upload() {
   stage_ = opt_.stage < 0 ? 1 : opt_.stage;
   for (int32_t k = 0; k < K_; ++k) {
     InstDesc& d = desc_[static_cast<size_t>(k)];
-    const int32_t cap = opt_.tile_nnz > 0 ? std::min(opt_.tile_nnz, kMaxTileNnz) : 2048;
+    const int32_t cap = cap_ = opt_.tile_nnz > 0 ? std::min(opt_.tile_nnz, kMaxTileNnz) : 2048;
     d.ta0 = static_cast<int32_t>(ta.size());
     build_row_tiles(a_ptr, d.row0, d.rows, k, cap, &ta);
     d.ta1 = static_cast<int32_t>(ta.size());
@@ -384,7 +410,13 @@ int EngineImpl<T>::upload() {
     set_last_error("no HIP device is available");
     return kStatusNoDevice;
   }
-  if (!stream_) SK_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), kStatusInitFailed);
+  if (!stream_) {
+    SK_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), kStatusInitFailed);
+    int dev = 0, cus = 0;
+    SK_HIP(hipGetDevice(&dev), kStatusInitFailed);
+    SK_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev), kStatusInitFailed);
+    cus_ = std::max(1, cus);
+  }
   if (!copy_stream_) SK_HIP(hipStreamCreateWithFlags(&copy_stream_, hipStreamNonBlocking), kStatusInitFailed);
   if (!host_done_) SK_HIP(hipHostMalloc(reinterpret_cast<void**>(&host_done_), 4 * sizeof(uint32_t)), kStatusOutOfMemory);
   if (host_state_cap_ < static_cast<size_t>(K_)) {
@@ -515,6 +547,8 @@ int EngineImpl<T>::upload() {
   SK_TRY(alloc(b_len_, static_cast<size_t>(K_), &d_len_));
   SK_TRY(alloc(b_zero_, static_cast<size_t>(K_), &d_zero_seen_));
   a.n_tc = n_tc_;
+  a.n_ta = n_ta_;
+  a.n_tt = n_tt_;
   a.force = 0;
   a.report = 0;
 #undef SK_TRY
@@ -654,8 +688,9 @@ int EngineImpl<T>::enqueue_block() {
   hipStream_t s = stream_;
   for (int k = 0; k < opt_.steps_per_block; ++k) {
     SK_LAUNCH(k_step_primal<T><<<n_tc_, kBlock, 0, s>>>(a));
-    SK_LAUNCH(k_step_dual<T><<<n_ta_, kBlock, 0, s>>>(a));
-    SK_LAUNCH(k_step_aty<T><<<n_tt_, kBlock, 0, s>>>(a));
+    int st = launch_spmv(1, a);
+    if (st) return st;
+    if ((st = launch_spmv(2, a))) return st;
     SK_LAUNCH(k_step_finalize<T><<<K_, kBlock, 0, s>>>(a));
     // an instance that became due for evaluation waits at most eval_stride steps
     if ((k + 1) % opt_.eval_stride == 0 || k + 1 == opt_.steps_per_block) {
@@ -984,8 +1019,7 @@ int EngineImpl<T>::time_kernel(int kernel, int reps, double* mean_ms) {
   SK_HIP(hipEventCreate(&e1), kStatusInitFailed);
   auto one = [&]() -> int {
     if (kernel == 0) SK_LAUNCH(k_step_primal<T><<<n_tc_, kBlock, 0, s>>>(a));
-    else if (kernel == 1) SK_LAUNCH(k_step_dual<T><<<n_ta_, kBlock, 0, s>>>(a));
-    else SK_LAUNCH(k_step_aty<T><<<n_tt_, kBlock, 0, s>>>(a));
+    else return launch_spmv(kernel, a);
     return kStatusSuccess;
   };
   int st = one();  // warm

@@ -106,6 +106,8 @@ struct Args {
   double* pe_t;  // eval: 8 per CSR(A^T) block
   uint32_t* done_count;
   int32_t n_tc;    // number of column-chunk tiles (eval kernels cover tc then tr)
+  int32_t n_ta;    // row blocks of CSR(A)
+  int32_t n_tt;    // row blocks of CSR(A^T)
   int32_t force;   // timing mode: every instance steps, nothing is decided
   int32_t report;  // evaluation kernels: 0 in-loop candidates, 1 final report of best
 };
@@ -146,45 +148,49 @@ __global__ __launch_bounds__(kBlock) void k_step_primal(Args<T> a) {
   if (threadIdx.x == 0) a.ps_c[tile] = d2;
 }
 
-template <class T>
+// CAP: the tile capacity the tables were built for, which sizes the LDS staging buffer (and
+// with it how many workgroups a CU holds). WALK: fewer workgroups than tiles, each walking
+// its share (see for_each_tile); otherwise one workgroup per tile.
+template <class T, int CAP, bool WALK>
 __global__ __launch_bounds__(kBlock) void k_step_dual(Args<T> a) {
-  __shared__ T stage[kMaxTileNnz];
+  __shared__ T stage[CAP];
   __shared__ T red_t[kBlock / kWave];
   __shared__ double red[kBlock / kWave];
-  const int tile = xcd_tile(blockIdx.x, gridDim.x);
-  const Tile t = a.ta[tile];
-  const InstState& s = a.state[t.inst];
-  if (!a.force && !stepping(s)) return;
-  const int p = s.parity;
-  const bool acc = a.force || s.pending_accept;
-  const T sigma = static_cast<T>(s.eta * s.omega);
-  const T* __restrict__ y = p ? a.y1 : a.y0;
-  T* __restrict__ yn = p ? a.y0 : a.y1;
-  const T* __restrict__ xbar = a.xbar;
-  const T* __restrict__ val = a.a_val;
-  const int32_t* __restrict__ idx = a.a_sidx;
-  double d2 = 0.0;
-  T yi = 0, lo = 0, hi = 0;
-  double sum = 0.0;
-  rowblock_product<T, true>(
-      t, a.a_ptr, a.a_pos, [&](int q) { return val[q] * xbar[idx[q]]; },
-      [&](int i) {
-        yi = y[i];
-        lo = a.rlhat[i];
-        hi = a.ruhat[i];
-        if (acc) sum = a.ysum[i];
-      },
-      [&](int i, T r) {
-        if (acc) a.ysum[i] = sum + static_cast<double>(yi);
-        const T v = yi + sigma * r;
-        const T ynew = v - sigma * dev_project<T>(v / sigma, lo, hi);
-        yn[i] = ynew;
-        const double d = static_cast<double>(ynew - yi);
-        d2 += d * d;
-      },
-      stage, red_t);
-  d2 = block_sum(d2, red);
-  if (threadIdx.x == 0) a.ps_a[tile] = d2;
+  for_each_tile<WALK>(a.n_ta, [&](int tile) {
+    const Tile t = a.ta[tile];
+    const InstState& s = a.state[t.inst];
+    if (!a.force && !stepping(s)) return;
+    const int p = s.parity;
+    const bool acc = a.force || s.pending_accept;
+    const T sigma = static_cast<T>(s.eta * s.omega);
+    const T* __restrict__ y = p ? a.y1 : a.y0;
+    T* __restrict__ yn = p ? a.y0 : a.y1;
+    const T* __restrict__ xbar = a.xbar;
+    const T* __restrict__ val = a.a_val;
+    const int32_t* __restrict__ idx = a.a_sidx;
+    double d2 = 0.0;
+    T yi = 0, lo = 0, hi = 0;
+    double sum = 0.0;
+    rowblock_product<T, true>(
+        t, a.a_ptr, a.a_pos, [&](int q) { return val[q] * xbar[idx[q]]; },
+        [&](int i) {
+          yi = y[i];
+          lo = a.rlhat[i];
+          hi = a.ruhat[i];
+          if (acc) sum = a.ysum[i];
+        },
+        [&](int i, T r) {
+          if (acc) a.ysum[i] = sum + static_cast<double>(yi);
+          const T v = yi + sigma * r;
+          const T ynew = v - sigma * dev_project<T>(v / sigma, lo, hi);
+          yn[i] = ynew;
+          const double d = static_cast<double>(ynew - yi);
+          d2 += d * d;
+        },
+        stage, red_t);
+    d2 = block_sum(d2, red);
+    if (threadIdx.x == 0) a.ps_a[tile] = d2;
+  });
 }
 
 // Reduce `count` per-tile partials starting at tile `base` with the whole workgroup in a
@@ -250,39 +256,40 @@ __device__ void decide_step(InstState& s, double dx2, double dy2, double inter_s
   }
 }
 
-template <class T>
+template <class T, int CAP, bool WALK>
 __global__ __launch_bounds__(kBlock) void k_step_aty(Args<T> a) {
-  __shared__ T stage[kMaxTileNnz];
+  __shared__ T stage[CAP];
   __shared__ T red_t[kBlock / kWave];
   __shared__ double red[kBlock / kWave];
-  const int tile = xcd_tile(blockIdx.x, gridDim.x);
-  const Tile t = a.tt[tile];
-  const InstState& s = a.state[t.inst];
-  if (!a.force && !stepping(s)) return;
-  const int p = s.parity;
-  const T* __restrict__ x = p ? a.x1 : a.x0;
-  const T* __restrict__ xn = p ? a.x0 : a.x1;
-  const T* __restrict__ yn = p ? a.y0 : a.y1;
-  const T* __restrict__ aty = p ? a.aty1 : a.aty0;
-  T* __restrict__ atyn = p ? a.aty0 : a.aty1;
-  const T* __restrict__ val = a.t_val;
-  const int32_t* __restrict__ idx = a.t_sidx;
-  double inter = 0.0;
-  T xj = 0, xnj = 0, atyj = 0;
-  rowblock_product<T, true>(
-      t, a.t_ptr, a.t_pos, [&](int q) { return val[q] * yn[idx[q]]; },
-      [&](int j) {
-        xj = x[j];
-        xnj = xn[j];
-        atyj = aty[j];
-      },
-      [&](int j, T r) {
-        atyn[j] = r;
-        inter += static_cast<double>(xnj - xj) * static_cast<double>(r - atyj);
-      },
-      stage, red_t);
-  inter = block_sum(inter, red);
-  if (threadIdx.x == 0) a.ps_t[tile] = inter;
+  for_each_tile<WALK>(a.n_tt, [&](int tile) {
+    const Tile t = a.tt[tile];
+    const InstState& s = a.state[t.inst];
+    if (!a.force && !stepping(s)) return;
+    const int p = s.parity;
+    const T* __restrict__ x = p ? a.x1 : a.x0;
+    const T* __restrict__ xn = p ? a.x0 : a.x1;
+    const T* __restrict__ yn = p ? a.y0 : a.y1;
+    const T* __restrict__ aty = p ? a.aty1 : a.aty0;
+    T* __restrict__ atyn = p ? a.aty0 : a.aty1;
+    const T* __restrict__ val = a.t_val;
+    const int32_t* __restrict__ idx = a.t_sidx;
+    double inter = 0.0;
+    T xj = 0, xnj = 0, atyj = 0;
+    rowblock_product<T, true>(
+        t, a.t_ptr, a.t_pos, [&](int q) { return val[q] * yn[idx[q]]; },
+        [&](int j) {
+          xj = x[j];
+          xnj = xn[j];
+          atyj = aty[j];
+        },
+        [&](int j, T r) {
+          atyn[j] = r;
+          inter += static_cast<double>(xnj - xj) * static_cast<double>(r - atyj);
+        },
+        stage, red_t);
+    inter = block_sum(inter, red);
+    if (threadIdx.x == 0) a.ps_t[tile] = inter;
+  });
 }
 
 // One workgroup per instance: reduce the step partials in tile order and decide the step.
