@@ -244,6 +244,9 @@ def main() -> int:
         ms, nbytes = batch.time_kernel(k, args.kernel_reps)
         kern[k] = {"name": KERNEL_NAMES[k], "ms": ms, "bytes": nbytes,
                    "gbs": nbytes / (ms * 1e-3) / 1e9}
+        # the same launch after the solve in normal mode: every instance has stopped, so this
+        # is the floor cost of a launch whose tiles all exit at once
+        kern[k]["stopped_ms"] = batch.time_kernel(k + 3, args.kernel_reps)[0]
     dom = max((1, 2), key=lambda k: kern[k]["ms"])
     traffic = None
     tf = ROOT / "profiles" / f"traffic_{args.workload}.json"
@@ -275,7 +278,8 @@ def main() -> int:
                            "p99": 1e3 * sorted(times)[max(0, int(round(0.99 * len(times))) - 1)]},
         "phase_ms": {"setup": stats["setup_ms"], "loop": stats["loop_ms"],
                      "report": stats["report_ms"], "graph_launches": stats["graph_launches"]},
-        "kernels": {v["name"]: {"ms": v["ms"], "GB/s": v["gbs"], "bytes": v["bytes"]}
+        "kernels": {v["name"]: {"ms": v["ms"], "GB/s": v["gbs"], "bytes": v["bytes"],
+                                "all_stopped_ms": v["stopped_ms"]}
                     for v in kern.values()},
         "roofline": {"bound": "hbm", "kernel": kern[dom]["name"], "achieved": kern[dom]["gbs"],
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -522,7 +522,7 @@ int32_t sk_batch_stats_get(const sk_batch* b, sk_batch_stats* out) {
 
 int32_t sk_batch_time_kernel(sk_batch* b, int32_t kernel, int32_t reps, double* mean_ms,
                              double* bytes) {
-  if (!b || !mean_ms || kernel < 0 || kernel > 2 || !b->uploaded) return SK_GPU_INVALID_ARGUMENT;
+  if (!b || !mean_ms || kernel < 0 || kernel > 5 || !b->uploaded) return SK_GPU_INVALID_ARGUMENT;
   return guarded([&]() -> int32_t {
     const int32_t st = b->engine->time_kernel(kernel, reps, mean_ms);
     if (bytes) *bytes = b->engine->kernel_bytes(kernel);

@@ -18,12 +18,27 @@ constexpr int kMaxTileNnz = 4096;
 
 // A tile is a half-open range [begin, end) of rows (CSR tables) or of elements (chunk
 // tables) that belongs to exactly one instance. `cap` is the staging capacity the tile was
-// built for (row-block tables) so a kernel knows whether it is a stream or a vector tile.
+// built for (row-block tables) so a kernel knows whether it is a stream or a vector tile, and
+// [p0, p1) is its nonzero range (row-block tables; zero for chunks), so a kernel can issue
+// its matrix loads as soon as the tile record arrives.
 struct Tile {
   int32_t begin;
   int32_t end;
   int32_t inst;
   int32_t cap;
+  int32_t p0;
+  int32_t p1;
+  int32_t pad0;
+  int32_t pad1;
+};
+
+// Per-tile step flags, rewritten for every tile of an instance whenever its state changes
+// (set-up and the finalize kernels), so a step kernel learns whether its tile steps, and
+// which iterate buffers it uses, from a load that does not wait for the instance record.
+enum : uint32_t {
+  kFlagStepping = 1u,  // not stopped and no evaluation pending
+  kFlagParity = 2u,    // the current iterate is buffer 1
+  kFlagAccept = 4u     // the last step was accepted and is not yet in the running sums
 };
 
 // pdhg.cpp:81-132 (Residuals)

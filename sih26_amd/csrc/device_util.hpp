@@ -141,8 +141,8 @@ __device__ __forceinline__ void rowblock_product(const Tile t, const int32_t* __
                                                  Pre pre, Epi epi, A* stage, A* red_lds) {
   const int tid = threadIdx.x;
   const int r0 = t.begin, nrows = t.end - t.begin;
-  const int p0 = ptr[r0];
-  const int p1 = ptr[t.end];
+  const int p0 = t.p0;
+  const int p1 = t.p1;
   const int nnz = p1 - p0;
   if (nrows == 1 && nnz > t.cap) {
     if (tid == 0) pre(r0);

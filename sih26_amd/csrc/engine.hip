@@ -57,27 +57,30 @@ struct DevBuf {
 void build_row_tiles(const std::vector<int32_t>& ptr, int32_t row0, int32_t rows, int32_t inst,
                      int32_t cap, std::vector<Tile>* out) {
   const size_t before = out->size();
+  auto emit = [&](int32_t b, int32_t e) {
+    out->push_back(Tile{b, e, inst, cap, ptr[static_cast<size_t>(b)], ptr[static_cast<size_t>(e)], 0, 0});
+  };
   int32_t start = row0;
   int64_t acc = 0;
   const int32_t end = row0 + rows;
   for (int32_t r = row0; r < end; ++r) {
     const int64_t len = ptr[static_cast<size_t>(r) + 1] - ptr[static_cast<size_t>(r)];
     if (len > cap) {
-      if (start < r) out->push_back(Tile{start, r, inst, cap});
-      out->push_back(Tile{r, r + 1, inst, cap});
+      if (start < r) emit(start, r);
+      emit(r, r + 1);
       start = r + 1;
       acc = 0;
       continue;
     }
     if (acc + len > cap || r - start == kBlock) {
-      out->push_back(Tile{start, r, inst, cap});
+      emit(start, r);
       start = r;
       acc = 0;
     }
     acc += len;
   }
-  if (start < end) out->push_back(Tile{start, end, inst, cap});
-  if (out->size() == before) out->push_back(Tile{row0, row0, inst, cap});
+  if (start < end) emit(start, end);
+  if (out->size() == before) emit(row0, row0);
 }
 
 void build_chunks(int32_t begin, int32_t count, int32_t inst, std::vector<Tile>* out) {
@@ -219,7 +222,7 @@ class EngineImpl final : public Engine {
   DevBuf b_xsum_, b_ysum_, b_xu0_, b_xu1_, b_yu0_, b_yu1_, b_bestx_, b_besty_, b_act_, b_red_;
   DevBuf b_tc_, b_tr_, b_ta_, b_tt_, b_desc_, b_state_;
   DevBuf b_ps_c_, b_ps_a_, b_ps_t_, b_pe_c_, b_pe_r_, b_pe_a_, b_pe_t_;
-  DevBuf b_done_, b_v_, b_v0_, b_av_, b_len_, b_zero_;
+  DevBuf b_done_, b_v_, b_v0_, b_av_, b_len_, b_zero_, b_fa_, b_ft_, b_fc_;
   std::vector<DevBuf*> buffers() {
     return {&b_a_ptr_, &b_a_idx_, &b_a_orig_, &b_t_ptr_, &b_t_idx_, &b_t_orig_, &b_sa_, &b_st_,
             &b_a_val_, &b_t_val_, &b_a_sidx_, &b_t_sidx_, &b_a_pos_, &b_t_pos_, &b_cmin_, &b_cl_, &b_cu_, &b_rl_, &b_ru_, &b_dc_, &b_dr_,
@@ -228,7 +231,8 @@ class EngineImpl final : public Engine {
             &b_xbar_, &b_xrst_, &b_yrst_, &b_xsum_, &b_ysum_, &b_xu0_, &b_xu1_, &b_yu0_,
             &b_yu1_, &b_bestx_, &b_besty_, &b_act_, &b_red_, &b_tc_, &b_tr_, &b_ta_, &b_tt_,
             &b_desc_, &b_state_, &b_ps_c_, &b_ps_a_, &b_ps_t_, &b_pe_c_, &b_pe_r_, &b_pe_a_,
-            &b_pe_t_, &b_done_, &b_v_, &b_v0_, &b_av_, &b_len_, &b_zero_};
+            &b_pe_t_, &b_done_, &b_v_, &b_v0_, &b_av_, &b_len_, &b_zero_, &b_fa_, &b_ft_,
+            &b_fc_};
   }
   std::vector<const DevBuf*> buffers() const {
     auto all = const_cast<EngineImpl<T>*>(this)->buffers();
@@ -546,6 +550,9 @@ int EngineImpl<T>::upload() {
   SK_TRY(alloc(b_av_, M, &d_av_));
   SK_TRY(alloc(b_len_, static_cast<size_t>(K_), &d_len_));
   SK_TRY(alloc(b_zero_, static_cast<size_t>(K_), &d_zero_seen_));
+  SK_TRY(alloc(b_fa_, static_cast<size_t>(n_ta_), &a.fa));
+  SK_TRY(alloc(b_ft_, static_cast<size_t>(n_tt_), &a.ft));
+  SK_TRY(alloc(b_fc_, static_cast<size_t>(n_tc_), &a.fc));
   a.n_tc = n_tc_;
   a.n_ta = n_ta_;
   a.n_tt = n_tt_;
@@ -619,6 +626,7 @@ int EngineImpl<T>::enqueue_setup() {
   SK_LAUNCH(k_inst_norm<<<K_, kBlock, 0, s>>>(a.desc, a.ps_c, d_len_, nullptr));
   SK_LAUNCH(k_power_finish<<<blocks_for(K_), kBlock, 0, s>>>(d_len_, d_zero_seen_, a.state, K_));
   SK_LAUNCH(k_init_iterates<T><<<blocks_for(MN), kBlock, 0, s>>>(a, d_lhat64_, d_uhat64_, N, M));
+  if (K_ > 0) SK_LAUNCH(k_publish_flags<T><<<K_, kBlock, 0, s>>>(a));
   return kStatusSuccess;
 }
 
@@ -1012,7 +1020,9 @@ int EngineImpl<T>::time_kernel(int kernel, int reps, double* mean_ms) {
     return kStatusInvalidArgument;
   }
   Args<T> a = args_;
-  a.force = 1;
+  // kernels 3..5 are 0..2 in normal mode: instances that stopped skip their tiles
+  a.force = kernel < 3 ? 1 : 0;
+  kernel %= 3;
   hipStream_t s = stream_;
   hipEvent_t e0, e1;
   SK_HIP(hipEventCreate(&e0), kStatusInitFailed);
@@ -1046,6 +1056,7 @@ int EngineImpl<T>::time_kernel(int kernel, int reps, double* mean_ms) {
 //             per column read x, x', A^T y (3 words), write A^T y' (1 word)
 template <class T>
 double EngineImpl<T>::kernel_bytes(int kernel) const {
+  kernel %= 3;
   const double w = sizeof(T), z = static_cast<double>(nnz_);
   const double m = static_cast<double>(m_), n = static_cast<double>(n_);
   if (kernel == 0) return n * (7.0 * w + 16.0);

@@ -81,7 +81,8 @@ class Engine {
   virtual const EngineTimings& timings() const = 0;
   virtual int32_t size() const = 0;
   // Launch one of the step kernels `reps` times in timing mode between two HIP events on the
-  // engine's stream; returns the mean duration in ms. kernel: 0 primal, 1 dual (A), 2 A^T.
+  // engine's stream; returns the mean duration in ms. kernel: 0 primal, 1 dual (A), 2 A^T;
+  // 3, 4, 5: the same in normal mode (tiles of stopped instances exit at once).
   virtual int time_kernel(int kernel, int reps, double* mean_ms) = 0;
   // Algorithmic bytes of one launch of that kernel over the whole batch.
   virtual double kernel_bytes(int kernel) const = 0;

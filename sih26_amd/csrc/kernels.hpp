@@ -96,6 +96,10 @@ struct Args {
   const Tile* __restrict__ tt;  // row blocks of CSR(A^T)
   const InstDesc* __restrict__ desc;
   InstState* state;
+  // per-tile step flags (kFlag*) of ta, tt and tc
+  uint32_t* fa;
+  uint32_t* ft;
+  uint32_t* fc;
   // per-tile partials
   double* ps_c;  // step: ||x'-x||^2 per column chunk
   double* ps_a;  // step: ||y'-y||^2 per CSR(A) block
@@ -114,6 +118,25 @@ struct Args {
 
 __device__ __forceinline__ bool stepping(const InstState& s) { return !s.done && !s.eval_pending; }
 
+__device__ __forceinline__ uint32_t step_flags(const InstState& s) {
+  return (stepping(s) ? kFlagStepping : 0u) | (s.parity ? kFlagParity : 0u) |
+         (s.pending_accept ? kFlagAccept : 0u);
+}
+
+// Writes `f` into the step flags of every tile of the instance (whole workgroup).
+template <class T>
+__device__ __forceinline__ void publish_flags(const Args<T>& a, const InstDesc& d, uint32_t f) {
+  for (int k = d.ta0 + threadIdx.x; k < d.ta1; k += kBlock) a.fa[k] = f;
+  for (int k = d.tt0 + threadIdx.x; k < d.tt1; k += kBlock) a.ft[k] = f;
+  for (int k = d.tc0 + threadIdx.x; k < d.tc1; k += kBlock) a.fc[k] = f;
+}
+
+// Set-up: the first flags of every instance, one workgroup per instance.
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_publish_flags(Args<T> a) {
+  publish_flags(a, a.desc[blockIdx.x], step_flags(a.state[blockIdx.x]));
+}
+
 // ---- step kernels ---------------------------------------------------------------------------
 
 template <class T>
@@ -121,10 +144,11 @@ __global__ __launch_bounds__(kBlock) void k_step_primal(Args<T> a) {
   __shared__ double red[kBlock / kWave];
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
   const Tile t = a.tc[tile];
+  const uint32_t f = a.fc[tile];
+  if (!a.force && !(f & kFlagStepping)) return;
+  const int p = (f & kFlagParity) ? 1 : 0;
+  const bool acc = a.force || (f & kFlagAccept);
   const InstState& s = a.state[t.inst];
-  if (!a.force && !stepping(s)) return;
-  const int p = s.parity;
-  const bool acc = a.force || s.pending_accept;
   const T tau = static_cast<T>(s.eta / s.omega);
   const T* __restrict__ x = p ? a.x1 : a.x0;
   T* __restrict__ xn = p ? a.x0 : a.x1;
@@ -158,10 +182,11 @@ __global__ __launch_bounds__(kBlock) void k_step_dual(Args<T> a) {
   __shared__ double red[kBlock / kWave];
   for_each_tile<WALK>(a.n_ta, [&](int tile) {
     const Tile t = a.ta[tile];
+    const uint32_t f = a.fa[tile];
+    if (!a.force && !(f & kFlagStepping)) return;
+    const int p = (f & kFlagParity) ? 1 : 0;
+    const bool acc = a.force || (f & kFlagAccept);
     const InstState& s = a.state[t.inst];
-    if (!a.force && !stepping(s)) return;
-    const int p = s.parity;
-    const bool acc = a.force || s.pending_accept;
     const T sigma = static_cast<T>(s.eta * s.omega);
     const T* __restrict__ y = p ? a.y1 : a.y0;
     T* __restrict__ yn = p ? a.y0 : a.y1;
@@ -263,9 +288,9 @@ __global__ __launch_bounds__(kBlock) void k_step_aty(Args<T> a) {
   __shared__ double red[kBlock / kWave];
   for_each_tile<WALK>(a.n_tt, [&](int tile) {
     const Tile t = a.tt[tile];
-    const InstState& s = a.state[t.inst];
-    if (!a.force && !stepping(s)) return;
-    const int p = s.parity;
+    const uint32_t f = a.ft[tile];
+    if (!a.force && !(f & kFlagStepping)) return;
+    const int p = (f & kFlagParity) ? 1 : 0;
     const T* __restrict__ x = p ? a.x1 : a.x0;
     const T* __restrict__ xn = p ? a.x0 : a.x1;
     const T* __restrict__ yn = p ? a.y0 : a.y1;
@@ -304,7 +329,13 @@ __global__ __launch_bounds__(kBlock) void k_step_finalize(Args<T> a) {
   const double dx2 = tile_sum(a.ps_c, d.tc0, d.tc1 - d.tc0, 1, 0, red);
   const double dy2 = tile_sum(a.ps_a, d.ta0, d.ta1 - d.ta0, 1, 0, red);
   const double isum = tile_sum(a.ps_t, d.tt0, d.tt1 - d.tt0, 1, 0, red);
-  if (threadIdx.x == 0) decide_step(s, dx2, dy2, isum, d.rows > 0, a.done_count);
+  __shared__ uint32_t flags;
+  if (threadIdx.x == 0) {
+    decide_step(s, dx2, dy2, isum, d.rows > 0, a.done_count);
+    flags = step_flags(s);
+  }
+  __syncthreads();
+  publish_flags(a, d, flags);
 }
 
 // ---- evaluation kernels (FP64) ---------------------------------------------------------------
@@ -566,22 +597,27 @@ __global__ __launch_bounds__(kBlock) void k_eval_finalize(Args<T> a) {
   }
   v[16] = a.report ? 0.0 : tile_sum(a.pe_c, d.tc0, d.tc1 - d.tc0, 2, 0, red);
   v[17] = a.report ? 0.0 : tile_sum(a.pe_r, d.tr0, d.tr1 - d.tr0, 2, 0, red);
-  if (threadIdx.x != 0) return;
-  Residuals r[2];
-  for (int c = 0; c < 2; ++c) {
-    const double* q = v + 8 * c;
-    r[c] = make_residuals(s, q[0], q[4] + q[2], q[1], q[5], dev_max(q[3], q[6]), q[7]);
+  __shared__ uint32_t flags;
+  if (threadIdx.x == 0) {
+    Residuals r[2];
+    for (int c = 0; c < 2; ++c) {
+      const double* q = v + 8 * c;
+      r[c] = make_residuals(s, q[0], q[4] + q[2], q[1], q[5], dev_max(q[3], q[6]), q[7]);
+    }
+    if (a.report) {
+      s.report = r[0];
+      s.report_pending = 0;
+      // nothing may rewrite a reported instance's answer: later commit stages skip it, so the
+      // host can read it back while the rest of the batch keeps running
+      s.copy_best = 0;
+      s.do_restart = 0;
+    } else {
+      decide_evaluation(s, r[0], r[1], v[16], v[17], a.done_count);
+    }
+    flags = step_flags(s);
   }
-  if (a.report) {
-    s.report = r[0];
-    s.report_pending = 0;
-    // nothing may rewrite a reported instance's answer: later commit stages skip it, so the
-    // host can read it back while the rest of the batch keeps running
-    s.copy_best = 0;
-    s.do_restart = 0;
-    return;
-  }
-  decide_evaluation(s, r[0], r[1], v[16], v[17], a.done_count);
+  __syncthreads();
+  if (!a.report) publish_flags(a, d, flags);
 }
 
 // k_eval_commit: apply the decisions of k_eval_cols (pdhg.cpp:541-545, 592-594, 632-638).
