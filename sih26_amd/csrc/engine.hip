@@ -619,8 +619,8 @@ int EngineImpl<T>::enqueue_setup() {
     SK_LAUNCH(k_chunk_sumsq<<<n_tc_, kBlock, 0, s>>>(a.tc, d_v_, a.ps_c));
     SK_LAUNCH(k_inst_norm<<<K_, kBlock, 0, s>>>(a.desc, a.ps_c, d_len_, d_zero_seen_));
     SK_LAUNCH(k_chunk_div<<<n_tc_, kBlock, 0, s>>>(a.tc, d_len_, d_v_));
-    SK_LAUNCH(k_spmv_tiles<<<n_ta_, kBlock, 0, s>>>(a.ta, a.a_ptr, a.a_idx, d_sa_, d_v_, d_av_));
-    SK_LAUNCH(k_spmv_tiles<<<n_tt_, kBlock, 0, s>>>(a.tt, a.t_ptr, a.t_idx, d_st_, d_av_, d_v_));
+    SK_LAUNCH(k_spmv_tiles<<<n_ta_, kBlock, 0, s>>>(a.ta, a.a_ptr, a.a_sidx, a.a_pos, d_sa_, d_v_, d_av_));
+    SK_LAUNCH(k_spmv_tiles<<<n_tt_, kBlock, 0, s>>>(a.tt, a.t_ptr, a.t_sidx, a.t_pos, d_st_, d_av_, d_v_));
   }
   SK_LAUNCH(k_chunk_sumsq<<<n_tc_, kBlock, 0, s>>>(a.tc, d_v_, a.ps_c));
   SK_LAUNCH(k_inst_norm<<<K_, kBlock, 0, s>>>(a.desc, a.ps_c, d_len_, nullptr));

@@ -667,22 +667,32 @@ __global__ __launch_bounds__(kBlock) void k_row_pass(const int32_t* __restrict__
 }
 
 // value * row_pass[i] * column_pass[j] in both layouts (scaling.cpp:14-40), one workgroup per
-// row-block tile with G lanes per row. `row_major`: rows of this CSR are rows of A (the index
-// array holds columns); otherwise its rows are columns of A.
+// row-block tile, nonzero-parallel: a tile of several rows first writes each nonzero's row
+// (within the tile) into LDS, then consecutive lanes scale consecutive nonzeros, so every load
+// and store is coalesced. `row_major`: rows of this CSR are rows of A (the index array holds
+// columns); otherwise its rows are columns of A.
 __global__ __launch_bounds__(kBlock) void k_apply_pass(const Tile* __restrict__ tiles,
                                                         const int32_t* __restrict__ ptr,
                                                         const int32_t* __restrict__ idx,
                                                         double* __restrict__ val, int32_t row_major,
                                                         const double* __restrict__ row_pass,
                                                         const double* __restrict__ col_pass) {
+  __shared__ uint8_t row_of[kMaxTileNnz];
   const Tile t = tiles[blockIdx.x];
-  const int nrows = t.end - t.begin;
-  if (nrows <= 0) return;
-  const int g = nrows == 1 ? kBlock : min(kWave, pow2_floor(kBlock / nrows));
-  const int row = threadIdx.x / g, lane = threadIdx.x % g;
-  if (row >= nrows) return;
-  const int r = t.begin + row;
-  for (int p = ptr[r] + lane; p < ptr[r + 1]; p += g) {
+  const int nrows = t.end - t.begin, nnz = t.p1 - t.p0;
+  if (nrows <= 0 || nnz <= 0) return;
+  const bool one_row = nrows == 1;  // also every vector tile (more than kMaxTileNnz nonzeros)
+  if (!one_row) {
+    // at most kBlock rows, so one lane per row and row numbers fit 8 bits
+    if (threadIdx.x < nrows) {
+      const int r = t.begin + threadIdx.x;
+      for (int p = ptr[r]; p < ptr[r + 1]; ++p) row_of[p - t.p0] = static_cast<uint8_t>(threadIdx.x);
+    }
+    __syncthreads();
+  }
+  for (int k = threadIdx.x; k < nnz; k += kBlock) {
+    const int p = t.p0 + k;
+    const int r = t.begin + (one_row ? 0 : static_cast<int>(row_of[k]));
     const double ri = row_major ? row_pass[r] : row_pass[idx[p]];
     const double cj = row_major ? col_pass[idx[p]] : col_pass[r];
     val[p] = val[p] * ri * cj;
@@ -790,19 +800,25 @@ __global__ __launch_bounds__(kBlock) void k_stage_values(const Tile* __restrict_
 
 // ---- power iteration (scaling.cpp:161-194) ---------------------------------------------------
 
-// y = A x over the row-block tiles of a CSR matrix in FP64 (any tile schedule; the power
-// iteration only sets the starting step size).
+// y = A x over the row-block tiles of a CSR matrix in FP64 with the step kernels' staged
+// schedule (gathers in each tile's staging order, rows reduced in CSR order); `val` is in CSR
+// order and read through the slot map. The power iteration only sets the starting step size.
 __global__ __launch_bounds__(kBlock) void k_spmv_tiles(const Tile* __restrict__ tiles,
                                                         const int32_t* __restrict__ ptr,
-                                                        const int32_t* __restrict__ idx,
+                                                        const int32_t* __restrict__ sidx,
+                                                        const uint16_t* __restrict__ pos,
                                                         const double* __restrict__ val,
                                                         const double* __restrict__ x,
                                                         double* __restrict__ y) {
+  __shared__ double stage[kMaxTileNnz];
   __shared__ double red[kBlock / kWave];
   const Tile t = tiles[xcd_tile(blockIdx.x, gridDim.x)];
-  rowblock_product<double, false>(
-      t, ptr, nullptr, [&](int q) { return val[q] * x[idx[q]]; }, [](int) {},
-      [&](int r, double acc) { y[r] = acc; }, (double*)nullptr, red);
+  // a vector tile keeps the CSR order, so its value index is the position itself
+  const bool vector = t.end - t.begin == 1 && t.p1 - t.p0 > t.cap;
+  rowblock_product<double, true>(
+      t, ptr, pos,
+      [&](int q) { return val[vector ? q : t.p0 + static_cast<int>(pos[q])] * x[sidx[q]]; },
+      [](int) {}, [&](int r, double acc) { y[r] = acc; }, stage, red);
 }
 
 // Per column chunk: sum of v^2 (slot 0 of pe_c).
