@@ -191,8 +191,6 @@ def main() -> int:
     ap.add_argument("--spmv-stage", type=int, default=-1,
                     help="staging order: -1 default, 0 CSR order, 1 sorted by gather index")
     ap.add_argument("--tile-nnz", type=int, default=0, help="row-block capacity, 0 auto")
-    ap.add_argument("--spmv-groups", type=int, default=-1,
-                    help="SpMV workgroups per CU walking the tiles (0: one per tile, -1 auto)")
     ap.add_argument("--requests", type=int, default=0,
                     help="serving stream length (0: 10000 for --workload c4, 1000 otherwise)")
     ap.add_argument("--rate", type=float, default=200.0, help="serving arrivals per second")
@@ -214,8 +212,7 @@ def main() -> int:
     if args.workload == "c4":
         return main_serving(sk, args)
     lps = instances(sk, args.workload, args.instances)
-    batch = sk.Batch(solve_options(sk, spmv_stage=args.spmv_stage, tile_nnz=args.tile_nnz,
-                                   spmv_groups=args.spmv_groups))
+    batch = sk.Batch(solve_options(sk, spmv_stage=args.spmv_stage, tile_nnz=args.tile_nnz))
     for lp in lps:
         batch.add(lp)
     batch.upload()

@@ -184,10 +184,8 @@ typedef struct {
   int32_t eval_stride;       /* an evaluation stage follows every this many steps inside a
                                 captured block, so an instance due for evaluation waits at most
                                 this many steps (default 10) */
-  int32_t spmv_groups;       /* SpMV launches: 0 = one workgroup per tile; k > 0 = k resident
-                                workgroups per compute unit, each walking its share of the
-                                tiles; -1 = automatic (default) */
-  int32_t reserved;
+  int32_t reserved0;
+  int32_t reserved1;
 } sk_pdhg_options;
 
 void sk_pdhg_default_options(sk_pdhg_options* options);

@@ -43,8 +43,8 @@ class PdhgOptions(C.Structure):
                 ("restart", i32), ("stop_at_request", i32), ("absolute_primal", i32),
                 ("random_seed", i64), ("precision", i32), ("steps_per_block", i32),
                 ("use_graph", i32), ("spmv_stage", i32), ("tile_nnz", i32),
-                ("compute_quality", i32), ("eval_stride", i32), ("spmv_groups", i32),
-                ("reserved", i32)]
+                ("compute_quality", i32), ("eval_stride", i32), ("reserved0", i32),
+                ("reserved1", i32)]
 
 
 class PdhgResult(C.Structure):

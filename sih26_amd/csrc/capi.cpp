@@ -69,7 +69,6 @@ sk::EngineOptions to_engine(const sk_pdhg_options& o) {
   e.use_graph = o.use_graph;
   e.stage = o.spmv_stage;
   e.tile_nnz = o.tile_nnz;
-  e.spmv_groups = o.spmv_groups;
   return e;
 }
 
@@ -345,8 +344,8 @@ void sk_pdhg_default_options(sk_pdhg_options* o) {
   o->tile_nnz = 0;
   o->compute_quality = 1;
   o->eval_stride = 10;
-  o->spmv_groups = -1;
-  o->reserved = 0;
+  o->reserved0 = 0;
+  o->reserved1 = 0;
 }
 
 int32_t sk_pdhg_set_option(sk_pdhg_options* o, const char* key_in, const char* value_in) {

@@ -20,7 +20,9 @@ constexpr int kMaxTileNnz = 4096;
 // tables) that belongs to exactly one instance. `cap` is the staging capacity the tile was
 // built for (row-block tables) so a kernel knows whether it is a stream or a vector tile, and
 // [p0, p1) is its nonzero range (row-block tables; zero for chunks), so a kernel can issue
-// its matrix loads as soon as the tile record arrives.
+// its matrix loads as soon as the tile record arrives. `base` is the first gather index of
+// the tile's instance (its first column for CSR(A), first row for CSR(A^T)): packed index
+// words hold gather indices relative to it.
 struct Tile {
   int32_t begin;
   int32_t end;
@@ -28,8 +30,8 @@ struct Tile {
   int32_t cap;
   int32_t p0;
   int32_t p1;
-  int32_t pad0;
-  int32_t pad1;
+  int32_t base;
+  int32_t pad;
 };
 
 // Per-tile step flags, rewritten for every tile of an instance whenever its state changes

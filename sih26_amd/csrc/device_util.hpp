@@ -96,23 +96,14 @@ __device__ __forceinline__ int xcd_tile(int b, int n) {
   return x * per + min(x, extra) + k;
 }
 
-// Calls f(tile) for the tiles of an n-tile table this workgroup owns. With one workgroup per
-// tile (gridDim.x == n) that is xcd_tile(blockIdx.x, n). With a smaller grid, the workgroups
-// on XCD x (those with blockIdx.x % 8 == x) walk the x-th contiguous eighth of the table with
-// a stride of their count, so the XCD slices are the same as with one workgroup per tile.
-// f must leave LDS reusable (end on a barrier) since the next tile follows at once.
-template <bool WALK, class F>
-__device__ __forceinline__ void for_each_tile(int n, F f) {
-  if (!WALK) {
-    f(xcd_tile(blockIdx.x, gridDim.x));
-    return;
-  }
-  const int g = gridDim.x, b = blockIdx.x;
-  const int x = b % kXcds, per = n / kXcds, extra = n % kXcds;
-  const int first = x * per + min(x, extra), count = per + (x < extra ? 1 : 0);
-  const int groups = g / kXcds + (x < g % kXcds ? 1 : 0);
-  for (int i = b / kXcds; i < count; i += groups) f(first + i);
-}
+// The identity slot map of rowblock_product (staging order = CSR order).
+struct SlotIdentity {
+  __device__ __forceinline__ int operator()(int, int k) const { return k; }
+};
+// Packed index words: (gather index - tile base) << 12 | CSR slot. Usable when every stream
+// tile holds at most 4096 nonzeros and every instance has fewer than 2^20 rows and columns.
+constexpr int kPackShift = 12;
+constexpr int32_t kPackMaxLocal = 1 << 20;
 
 __device__ __forceinline__ int pow2_floor(int v) {
   int p = 1;
@@ -129,16 +120,17 @@ __device__ __forceinline__ int pow2_floor(int v) {
 // tile's nonzeros into LDS with coalesced loads (CSR-Stream; Greathouse & Daga, SC'14), then
 // G = 256 / rows lanes (power of two, <= 64) reduce each row from LDS. Without STAGE the
 // same G-lane groups read their row straight from memory.
-// Staging order: with `pos`, the tile's nonzeros are stored sorted by gather index, so the
-// lanes of one load instruction gather from a few cache lines, and pos[p] is the CSR-order
-// slot (within the tile) that product p is staged to. The rows are then reduced in CSR order
-// exactly as without `pos`. Without `pos` the staging order is the CSR order.
+// Staging order: prod(q) is called for staging positions q in [p0, p1), and slot(q, k) (k =
+// q - p0) names the CSR-order slot within the tile that product q is staged to. The step
+// kernels store a tile's nonzeros sorted by gather index, so the lanes of one load instruction
+// gather from a few cache lines; the rows are still reduced in CSR order, so the sums are the
+// same as with the identity slot map.
 // Vector tile (one row with nnz > cap): the workgroup strides over the row and reduces; its
 // staging order is always the CSR order.
-template <class A, bool STAGE, class Prod, class Pre, class Epi>
+template <class A, bool STAGE, class Prod, class Slot, class Pre, class Epi>
 __device__ __forceinline__ void rowblock_product(const Tile t, const int32_t* __restrict__ ptr,
-                                                 const uint16_t* __restrict__ pos, Prod prod,
-                                                 Pre pre, Epi epi, A* stage, A* red_lds) {
+                                                 Prod prod, Slot slot, Pre pre, Epi epi, A* stage,
+                                                 A* red_lds) {
   const int tid = threadIdx.x;
   const int r0 = t.begin, nrows = t.end - t.begin;
   const int p0 = t.p0;
@@ -175,19 +167,19 @@ __device__ __forceinline__ void rowblock_product(const Tile t, const int32_t* __
     constexpr int U = 8;
     for (int base = 0; base < nnz; base += U * kBlock) {
       A part[U];
-      int slot[U];
+      int to[U];
 #pragma unroll
       for (int u = 0; u < U; ++u) {
         const int k = base + u * kBlock + tid;
         if (k < nnz) {
           part[u] = prod(p0 + k);
-          slot[u] = pos ? static_cast<int>(pos[p0 + k]) : k;
+          to[u] = slot(p0 + k, k);
         }
       }
 #pragma unroll
       for (int u = 0; u < U; ++u) {
         const int k = base + u * kBlock + tid;
-        if (k < nnz) stage[slot[u]] = part[u];
+        if (k < nnz) stage[to[u]] = part[u];
       }
     }
     __syncthreads();

@@ -55,10 +55,10 @@ struct DevBuf {
 // tile while their nonzeros fit `cap` and there are at most kBlock of them; a row with more
 // than `cap` nonzeros gets a tile of its own. Every instance gets at least one tile.
 void build_row_tiles(const std::vector<int32_t>& ptr, int32_t row0, int32_t rows, int32_t inst,
-                     int32_t cap, std::vector<Tile>* out) {
+                     int32_t cap, int32_t base, std::vector<Tile>* out) {
   const size_t before = out->size();
   auto emit = [&](int32_t b, int32_t e) {
-    out->push_back(Tile{b, e, inst, cap, ptr[static_cast<size_t>(b)], ptr[static_cast<size_t>(e)], 0, 0});
+    out->push_back(Tile{b, e, inst, cap, ptr[static_cast<size_t>(b)], ptr[static_cast<size_t>(e)], base, 0});
   };
   int32_t start = row0;
   int64_t acc = 0;
@@ -168,13 +168,10 @@ class EngineImpl final : public Engine {
   // the staging capacity the tile tables were built with.
   int launch_spmv(int kernel, const Args<T>& a) {
     hipStream_t s = stream_;
-    const int32_t tiles = kernel == 1 ? n_ta_ : n_tt_;
-    const int32_t groups = opt_.spmv_groups < 0 ? 0 : opt_.spmv_groups;
-    const unsigned grid = static_cast<unsigned>(
-        groups > 0 ? std::max(1, std::min(tiles, groups * cus_)) : tiles);
+    const unsigned grid = static_cast<unsigned>(kernel == 1 ? n_ta_ : n_tt_);
     auto go = [&](auto cap) -> int {
       constexpr int C = decltype(cap)::value;
-      if (groups > 0) {
+      if (pack_) {
         if (kernel == 1) SK_LAUNCH(k_step_dual<T, C, true><<<grid, kBlock, 0, s>>>(a));
         else SK_LAUNCH(k_step_aty<T, C, true><<<grid, kBlock, 0, s>>>(a));
       } else {
@@ -183,7 +180,6 @@ class EngineImpl final : public Engine {
       }
       return kStatusSuccess;
     };
-    if (cap_ <= 512) return go(std::integral_constant<int, 512>{});
     if (cap_ <= 1024) return go(std::integral_constant<int, 1024>{});
     if (cap_ <= 2048) return go(std::integral_constant<int, 2048>{});
     return go(std::integral_constant<int, kMaxTileNnz>{});
@@ -197,6 +193,7 @@ class EngineImpl final : public Engine {
   int32_t K_ = 0;
   int stage_ = 1;
   int32_t cap_ = 2048;  // row-block tile capacity (nonzeros)
+  bool pack_ = true;    // packed index words in the staging order
   int32_t cus_ = 256;   // compute units of the device
 
   // host copies needed after upload
@@ -392,10 +389,10 @@ int EngineImpl<T>::upload() {
     InstDesc& d = desc_[static_cast<size_t>(k)];
     const int32_t cap = cap_ = opt_.tile_nnz > 0 ? std::min(opt_.tile_nnz, kMaxTileNnz) : 2048;
     d.ta0 = static_cast<int32_t>(ta.size());
-    build_row_tiles(a_ptr, d.row0, d.rows, k, cap, &ta);
+    build_row_tiles(a_ptr, d.row0, d.rows, k, cap, d.col0, &ta);
     d.ta1 = static_cast<int32_t>(ta.size());
     d.tt0 = static_cast<int32_t>(tt.size());
-    build_row_tiles(t_ptr, d.col0, d.cols, k, cap, &tt);
+    build_row_tiles(t_ptr, d.col0, d.cols, k, cap, d.row0, &tt);
     d.tt1 = static_cast<int32_t>(tt.size());
     d.tc0 = static_cast<int32_t>(tc.size());
     build_chunks(d.col0, d.cols, k, &tc);
@@ -519,15 +516,19 @@ int EngineImpl<T>::upload() {
   a.tt = ptt;
   {
     // staging order of the step kernels' tiles (structure only, so once per upload)
+    // Packed index words (4 B per nonzero instead of 6) when every instance's local indices
+    // fit 20 bits; the slot arrays are then not needed.
+    pack_ = cap_ <= kMaxTileNnz;
+    for (const InstDesc& d : desc_) pack_ = pack_ && d.rows < kPackMaxLocal && d.cols < kPackMaxLocal;
     int32_t *asx, *tsx;
     uint16_t *apos, *tpos;
     SK_TRY(alloc(b_a_sidx_, Z, &asx));
     SK_TRY(alloc(b_t_sidx_, Z, &tsx));
-    SK_TRY(alloc(b_a_pos_, Z, &apos));
-    SK_TRY(alloc(b_t_pos_, Z, &tpos));
-    const int32_t sorted = stage_ != 0;
-    SK_LAUNCH(k_sort_tiles<<<n_ta_, kBlock, 0, stream_>>>(pta, ap, ai, sorted, asx, apos));
-    SK_LAUNCH(k_sort_tiles<<<n_tt_, kBlock, 0, stream_>>>(ptt, tp, ti, sorted, tsx, tpos));
+    SK_TRY(alloc(b_a_pos_, pack_ ? 0 : Z, &apos));
+    SK_TRY(alloc(b_t_pos_, pack_ ? 0 : Z, &tpos));
+    const int32_t sorted = stage_ != 0, pack = pack_ ? 1 : 0;
+    SK_LAUNCH(k_sort_tiles<<<n_ta_, kBlock, 0, stream_>>>(pta, ap, ai, sorted, pack, asx, apos));
+    SK_LAUNCH(k_sort_tiles<<<n_tt_, kBlock, 0, stream_>>>(ptt, tp, ti, sorted, pack, tsx, tpos));
     a.a_sidx = asx;
     a.t_sidx = tsx;
     a.a_pos = apos;
@@ -610,8 +611,11 @@ int EngineImpl<T>::enqueue_setup() {
     SK_LAUNCH(k_scaled_vectors<double><<<blocks_for(N), kBlock, 0, s>>>(
         a64, a.xu0, d_lhat64_, d_uhat64_, a.yu0, a.yu1, N, 0));
   }
-  SK_LAUNCH(k_stage_values<T><<<n_ta_, kBlock, 0, s>>>(a.ta, a.a_ptr, a.a_pos, d_sa_, const_cast<T*>(a.a_val)));
-  SK_LAUNCH(k_stage_values<T><<<n_tt_, kBlock, 0, s>>>(a.tt, a.t_ptr, a.t_pos, d_st_, const_cast<T*>(a.t_val)));
+  const int32_t pack = pack_ ? 1 : 0;
+  SK_LAUNCH(k_stage_values<T><<<n_ta_, kBlock, 0, s>>>(a.ta, a.a_ptr, a.a_sidx, a.a_pos, pack, d_sa_,
+                                                       const_cast<T*>(a.a_val)));
+  SK_LAUNCH(k_stage_values<T><<<n_tt_, kBlock, 0, s>>>(a.tt, a.t_ptr, a.t_sidx, a.t_pos, pack, d_st_,
+                                                       const_cast<T*>(a.t_val)));
   // power iteration on A^T A, 30 steps (scaling.cpp:161-194)
   SK_LAUNCH(k_copy<<<blocks_for(N), kBlock, 0, s>>>(d_v0_, d_v_, N));
   SK_LAUNCH(k_fill_i32<<<blocks_for(K_), kBlock, 0, s>>>(d_zero_seen_, 0, K_));
@@ -619,8 +623,13 @@ int EngineImpl<T>::enqueue_setup() {
     SK_LAUNCH(k_chunk_sumsq<<<n_tc_, kBlock, 0, s>>>(a.tc, d_v_, a.ps_c));
     SK_LAUNCH(k_inst_norm<<<K_, kBlock, 0, s>>>(a.desc, a.ps_c, d_len_, d_zero_seen_));
     SK_LAUNCH(k_chunk_div<<<n_tc_, kBlock, 0, s>>>(a.tc, d_len_, d_v_));
-    SK_LAUNCH(k_spmv_tiles<<<n_ta_, kBlock, 0, s>>>(a.ta, a.a_ptr, a.a_sidx, a.a_pos, d_sa_, d_v_, d_av_));
-    SK_LAUNCH(k_spmv_tiles<<<n_tt_, kBlock, 0, s>>>(a.tt, a.t_ptr, a.t_sidx, a.t_pos, d_st_, d_av_, d_v_));
+    if (pack_) {
+      SK_LAUNCH(k_spmv_tiles<true><<<n_ta_, kBlock, 0, s>>>(a.ta, a.a_ptr, a.a_sidx, a.a_pos, d_sa_, d_v_, d_av_));
+      SK_LAUNCH(k_spmv_tiles<true><<<n_tt_, kBlock, 0, s>>>(a.tt, a.t_ptr, a.t_sidx, a.t_pos, d_st_, d_av_, d_v_));
+    } else {
+      SK_LAUNCH(k_spmv_tiles<false><<<n_ta_, kBlock, 0, s>>>(a.ta, a.a_ptr, a.a_sidx, a.a_pos, d_sa_, d_v_, d_av_));
+      SK_LAUNCH(k_spmv_tiles<false><<<n_tt_, kBlock, 0, s>>>(a.tt, a.t_ptr, a.t_sidx, a.t_pos, d_st_, d_av_, d_v_));
+    }
   }
   SK_LAUNCH(k_chunk_sumsq<<<n_tc_, kBlock, 0, s>>>(a.tc, d_v_, a.ps_c));
   SK_LAUNCH(k_inst_norm<<<K_, kBlock, 0, s>>>(a.desc, a.ps_c, d_len_, nullptr));

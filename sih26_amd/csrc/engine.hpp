@@ -31,7 +31,6 @@ struct EngineOptions {
   int32_t use_graph = 1;
   int32_t stage = -1;             // staging order of SpMV tiles: -1 default (1), 0 CSR, 1 sorted
   int32_t tile_nnz = 0;           // row-block capacity; 0 picks per instance
-  int32_t spmv_groups = -1;       // SpMV workgroups per CU walking the tiles; 0 one per tile
 };
 
 struct InstanceResult {

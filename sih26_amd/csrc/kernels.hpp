@@ -172,15 +172,27 @@ __global__ __launch_bounds__(kBlock) void k_step_primal(Args<T> a) {
   if (threadIdx.x == 0) a.ps_c[tile] = d2;
 }
 
+// Gather index and CSR slot of staging position q: a packed word holds (local index << 12 |
+// slot), relative to the tile's base; otherwise separate index and 16-bit slot arrays.
+template <bool PACK>
+__device__ __forceinline__ int gather_index(const Tile& t, const int32_t* __restrict__ sidx, int q) {
+  return PACK ? t.base + static_cast<int>(static_cast<uint32_t>(sidx[q]) >> kPackShift) : sidx[q];
+}
+template <bool PACK>
+__device__ __forceinline__ int staging_slot(const int32_t* __restrict__ sidx,
+                                            const uint16_t* __restrict__ pos, int q) {
+  return PACK ? static_cast<int>(static_cast<uint32_t>(sidx[q]) & 0xFFFu) : static_cast<int>(pos[q]);
+}
+
 // CAP: the tile capacity the tables were built for, which sizes the LDS staging buffer (and
-// with it how many workgroups a CU holds). WALK: fewer workgroups than tiles, each walking
-// its share (see for_each_tile); otherwise one workgroup per tile.
-template <class T, int CAP, bool WALK>
+// with it how many workgroups a CU holds). PACK: index words are packed (see gather_index).
+template <class T, int CAP, bool PACK>
 __global__ __launch_bounds__(kBlock) void k_step_dual(Args<T> a) {
   __shared__ T stage[CAP];
   __shared__ T red_t[kBlock / kWave];
   __shared__ double red[kBlock / kWave];
-  for_each_tile<WALK>(a.n_ta, [&](int tile) {
+  {
+    const int tile = xcd_tile(blockIdx.x, gridDim.x);
     const Tile t = a.ta[tile];
     const uint32_t f = a.fa[tile];
     if (!a.force && !(f & kFlagStepping)) return;
@@ -196,8 +208,10 @@ __global__ __launch_bounds__(kBlock) void k_step_dual(Args<T> a) {
     double d2 = 0.0;
     T yi = 0, lo = 0, hi = 0;
     double sum = 0.0;
+    const uint16_t* __restrict__ pos = a.a_pos;
     rowblock_product<T, true>(
-        t, a.a_ptr, a.a_pos, [&](int q) { return val[q] * xbar[idx[q]]; },
+        t, a.a_ptr, [&](int q) { return val[q] * xbar[gather_index<PACK>(t, idx, q)]; },
+        [&](int q, int) { return staging_slot<PACK>(idx, pos, q); },
         [&](int i) {
           yi = y[i];
           lo = a.rlhat[i];
@@ -215,7 +229,7 @@ __global__ __launch_bounds__(kBlock) void k_step_dual(Args<T> a) {
         stage, red_t);
     d2 = block_sum(d2, red);
     if (threadIdx.x == 0) a.ps_a[tile] = d2;
-  });
+  }
 }
 
 // Reduce `count` per-tile partials starting at tile `base` with the whole workgroup in a
@@ -281,12 +295,13 @@ __device__ void decide_step(InstState& s, double dx2, double dy2, double inter_s
   }
 }
 
-template <class T, int CAP, bool WALK>
+template <class T, int CAP, bool PACK>
 __global__ __launch_bounds__(kBlock) void k_step_aty(Args<T> a) {
   __shared__ T stage[CAP];
   __shared__ T red_t[kBlock / kWave];
   __shared__ double red[kBlock / kWave];
-  for_each_tile<WALK>(a.n_tt, [&](int tile) {
+  {
+    const int tile = xcd_tile(blockIdx.x, gridDim.x);
     const Tile t = a.tt[tile];
     const uint32_t f = a.ft[tile];
     if (!a.force && !(f & kFlagStepping)) return;
@@ -300,8 +315,10 @@ __global__ __launch_bounds__(kBlock) void k_step_aty(Args<T> a) {
     const int32_t* __restrict__ idx = a.t_sidx;
     double inter = 0.0;
     T xj = 0, xnj = 0, atyj = 0;
+    const uint16_t* __restrict__ pos = a.t_pos;
     rowblock_product<T, true>(
-        t, a.t_ptr, a.t_pos, [&](int q) { return val[q] * yn[idx[q]]; },
+        t, a.t_ptr, [&](int q) { return val[q] * yn[gather_index<PACK>(t, idx, q)]; },
+        [&](int q, int) { return staging_slot<PACK>(idx, pos, q); },
         [&](int j) {
           xj = x[j];
           xnj = xn[j];
@@ -314,7 +331,7 @@ __global__ __launch_bounds__(kBlock) void k_step_aty(Args<T> a) {
         stage, red_t);
     inter = block_sum(inter, red);
     if (threadIdx.x == 0) a.ps_t[tile] = inter;
-  });
+  }
 }
 
 // One workgroup per instance: reduce the step partials in tile order and decide the step.
@@ -398,12 +415,12 @@ __global__ __launch_bounds__(kBlock) void k_eval_rows(Args<T> a) {
   const int32_t* __restrict__ idx = a.a_idx;
   double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   rowblock_product<D2, false>(
-      t, a.a_ptr, nullptr,
+      t, a.a_ptr,
       [&](int q) {
         const int c = idx[q];
         return D2{val[q] * xa[c], val[q] * xb[c]};
       },
-      [](int) {},
+      SlotIdentity{}, [](int) {},
       [&](int i, D2 act) {
         const double lo = a.rl[i], hi = a.ru[i];
         const bool lo_f = dev_finite_bound(lo), hi_f = dev_finite_bound(hi);
@@ -539,12 +556,12 @@ __global__ __launch_bounds__(kBlock) void k_eval_cols(Args<T> a) {
   const int32_t* __restrict__ idx = a.t_idx;
   double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   rowblock_product<D2, false>(
-      t, a.t_ptr, nullptr,
+      t, a.t_ptr,
       [&](int q) {
         const int r = idx[q];
         return D2{val[q] * ya[r], val[q] * yb[r]};
       },
-      [](int) {},
+      SlotIdentity{}, [](int) {},
       [&](int j, D2 dot) {
         const double lo = a.cl[j], hi = a.cu[j];
         const bool lo_f = dev_finite_bound(lo), hi_f = dev_finite_bound(hi);
@@ -741,18 +758,26 @@ __global__ __launch_bounds__(kBlock) void k_convert(const double* __restrict__ s
 __global__ __launch_bounds__(kBlock) void k_sort_tiles(const Tile* __restrict__ tiles,
                                                         const int32_t* __restrict__ ptr,
                                                         const int32_t* __restrict__ idx,
-                                                        int32_t sorted, int32_t* __restrict__ sidx,
+                                                        int32_t sorted, int32_t pack,
+                                                        int32_t* __restrict__ sidx,
                                                         uint16_t* __restrict__ pos) {
   __shared__ uint64_t keys[kMaxTileNnz];
   const Tile t = tiles[blockIdx.x];
   const int tid = threadIdx.x;
   const int p0 = ptr[t.begin], nnz = ptr[t.end] - p0;
   const bool vector = t.end - t.begin == 1 && nnz > t.cap;
-  if (!sorted || vector) {
-    for (int k = tid; k < nnz; k += kBlock) {
-      sidx[p0 + k] = idx[p0 + k];
-      pos[p0 + k] = static_cast<uint16_t>(k);
+  // one staging entry: gather index g, CSR slot k (a vector tile never uses its slots)
+  auto put = [&](int q, int g, int k) {
+    if (pack) {
+      sidx[q] = static_cast<int32_t>((static_cast<uint32_t>(g - t.base) << kPackShift) |
+                                     static_cast<uint32_t>(vector ? 0 : k));
+    } else {
+      sidx[q] = g;
+      pos[q] = static_cast<uint16_t>(k);
     }
+  };
+  if (!sorted || vector) {
+    for (int k = tid; k < nnz; k += kBlock) put(p0 + k, idx[p0 + k], k);
     return;
   }
   // a stream tile holds at most cap <= kMaxTileNnz nonzeros, so slots fit 16 bits
@@ -777,10 +802,8 @@ __global__ __launch_bounds__(kBlock) void k_sort_tiles(const Tile* __restrict__ 
       __syncthreads();
     }
   }
-  for (int k = tid; k < nnz; k += kBlock) {
-    sidx[p0 + k] = static_cast<int32_t>(keys[k] >> 16);
-    pos[p0 + k] = static_cast<uint16_t>(keys[k] & 0xFFFFu);
-  }
+  for (int k = tid; k < nnz; k += kBlock)
+    put(p0 + k, static_cast<int32_t>(keys[k] >> 16), static_cast<int>(keys[k] & 0xFFFFu));
 }
 
 // The scaled values in each tile's staging order, in the iterate precision (a vector tile keeps
@@ -788,14 +811,19 @@ __global__ __launch_bounds__(kBlock) void k_sort_tiles(const Tile* __restrict__ 
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_stage_values(const Tile* __restrict__ tiles,
                                                           const int32_t* __restrict__ ptr,
+                                                          const int32_t* __restrict__ sidx,
                                                           const uint16_t* __restrict__ pos,
-                                                          const double* __restrict__ src,
+                                                          int32_t pack, const double* __restrict__ src,
                                                           T* __restrict__ dst) {
   const Tile t = tiles[blockIdx.x];
   const int p0 = ptr[t.begin], nnz = ptr[t.end] - p0;
   const bool vector = t.end - t.begin == 1 && nnz > t.cap;
-  for (int k = threadIdx.x; k < nnz; k += kBlock)
-    dst[p0 + k] = static_cast<T>(src[p0 + (vector ? k : static_cast<int>(pos[p0 + k]))]);
+  for (int k = threadIdx.x; k < nnz; k += kBlock) {
+    const int slot = vector ? k
+                     : pack ? static_cast<int>(static_cast<uint32_t>(sidx[p0 + k]) & 0xFFFu)
+                            : static_cast<int>(pos[p0 + k]);
+    dst[p0 + k] = static_cast<T>(src[p0 + slot]);
+  }
 }
 
 // ---- power iteration (scaling.cpp:161-194) ---------------------------------------------------
@@ -803,6 +831,7 @@ __global__ __launch_bounds__(kBlock) void k_stage_values(const Tile* __restrict_
 // y = A x over the row-block tiles of a CSR matrix in FP64 with the step kernels' staged
 // schedule (gathers in each tile's staging order, rows reduced in CSR order); `val` is in CSR
 // order and read through the slot map. The power iteration only sets the starting step size.
+template <bool PACK>
 __global__ __launch_bounds__(kBlock) void k_spmv_tiles(const Tile* __restrict__ tiles,
                                                         const int32_t* __restrict__ ptr,
                                                         const int32_t* __restrict__ sidx,
@@ -815,9 +844,14 @@ __global__ __launch_bounds__(kBlock) void k_spmv_tiles(const Tile* __restrict__ 
   const Tile t = tiles[xcd_tile(blockIdx.x, gridDim.x)];
   // a vector tile keeps the CSR order, so its value index is the position itself
   const bool vector = t.end - t.begin == 1 && t.p1 - t.p0 > t.cap;
+  auto gather = [&](int q) {
+    return PACK ? t.base + static_cast<int>(static_cast<uint32_t>(sidx[q]) >> kPackShift) : sidx[q];
+  };
+  auto slot = [&](int q, int) {
+    return PACK ? static_cast<int>(static_cast<uint32_t>(sidx[q]) & 0xFFFu) : static_cast<int>(pos[q]);
+  };
   rowblock_product<double, true>(
-      t, ptr, pos,
-      [&](int q) { return val[vector ? q : t.p0 + static_cast<int>(pos[q])] * x[sidx[q]]; },
+      t, ptr, [&](int q) { return val[vector ? q : t.p0 + slot(q, 0)] * x[gather(q)]; }, slot,
       [](int) {}, [&](int r, double acc) { y[r] = acc; }, stage, red);
 }
 
