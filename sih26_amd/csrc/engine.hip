@@ -169,20 +169,24 @@ class EngineImpl final : public Engine {
   int launch_spmv(int kernel, const Args<T>& a) {
     hipStream_t s = stream_;
     const unsigned grid = static_cast<unsigned>(kernel == 1 ? n_ta_ : n_tt_);
-    auto go = [&](auto cap) -> int {
+    return with_layout([&](auto cap, auto pack) -> int {
       constexpr int C = decltype(cap)::value;
-      if (pack_) {
-        if (kernel == 1) SK_LAUNCH(k_step_dual<T, C, true><<<grid, kBlock, 0, s>>>(a));
-        else SK_LAUNCH(k_step_aty<T, C, true><<<grid, kBlock, 0, s>>>(a));
-      } else {
-        if (kernel == 1) SK_LAUNCH(k_step_dual<T, C, false><<<grid, kBlock, 0, s>>>(a));
-        else SK_LAUNCH(k_step_aty<T, C, false><<<grid, kBlock, 0, s>>>(a));
-      }
+      constexpr bool P = decltype(pack)::value;
+      if (kernel == 1) SK_LAUNCH(k_step_dual<T, C, P><<<grid, kBlock, 0, s>>>(a));
+      else SK_LAUNCH(k_step_aty<T, C, P><<<grid, kBlock, 0, s>>>(a));
       return kStatusSuccess;
+    });
+  }
+  // Calls f(capacity, packing) with the staging capacity (sizing LDS) and the index packing
+  // of the tile tables as compile-time constants.
+  template <class F>
+  int with_layout(F f) {
+    auto packed = [&](auto cap) -> int {
+      return pack_ ? f(cap, std::true_type{}) : f(cap, std::false_type{});
     };
-    if (cap_ <= 1024) return go(std::integral_constant<int, 1024>{});
-    if (cap_ <= 2048) return go(std::integral_constant<int, 2048>{});
-    return go(std::integral_constant<int, kMaxTileNnz>{});
+    if (cap_ <= 1024) return packed(std::integral_constant<int, 1024>{});
+    if (cap_ <= 2048) return packed(std::integral_constant<int, 2048>{});
+    return packed(std::integral_constant<int, kMaxTileNnz>{});
   }
   int extract_finished(const InstState* snapshot, std::vector<int32_t>* newly_done);
   int stop_on_time_limit(std::vector<int32_t>* newly_done);
@@ -692,8 +696,14 @@ int EngineImpl<T>::enqueue_eval(int report_mode) {
   a.report = report_mode;
   hipStream_t s = stream_;
   if (!report_mode) SK_LAUNCH(k_eval_prepare<T><<<n_tc_ + n_tr_, kBlock, 0, s>>>(a));
-  SK_LAUNCH(k_eval_rows<T><<<n_ta_, kBlock, 0, s>>>(a));
-  SK_LAUNCH(k_eval_cols<T><<<n_tt_, kBlock, 0, s>>>(a));
+  const int st = with_layout([&](auto cap, auto pack) -> int {
+    constexpr int C = decltype(cap)::value;
+    constexpr bool P = decltype(pack)::value;
+    SK_LAUNCH(k_eval_rows<T, C, P><<<n_ta_, kBlock, 0, s>>>(a));
+    SK_LAUNCH(k_eval_cols<T, C, P><<<n_tt_, kBlock, 0, s>>>(a));
+    return kStatusSuccess;
+  });
+  if (st) return st;
   SK_LAUNCH(k_eval_finalize<T><<<K_, kBlock, 0, s>>>(a));
   if (!report_mode) SK_LAUNCH(k_eval_commit<T><<<n_tc_ + n_tr_, kBlock, 0, s>>>(a));
   return kStatusSuccess;

@@ -399,8 +399,9 @@ __global__ __launch_bounds__(kBlock) void k_eval_prepare(Args<T> a) {
 // k_eval_rows: activity = A x for both candidates on the original A, and every row term of
 // evaluate() (pdhg.cpp:154-170, 203-222, 227-238). Partials per tile, per candidate c:
 // [4c+0] sum violation^2, [4c+1] support, [4c+2] dual violation from y, [4c+3] complementarity.
-template <class T>
+template <class T, int CAP, bool PACK>
 __global__ __launch_bounds__(kBlock) void k_eval_rows(Args<T> a) {
+  __shared__ D2 stage[CAP];
   __shared__ D2 red2[kBlock / kWave];
   __shared__ double red[kBlock / kWave];
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
@@ -412,15 +413,20 @@ __global__ __launch_bounds__(kBlock) void k_eval_rows(Args<T> a) {
   const double* __restrict__ ya = a.report ? a.besty : a.yu0;
   const double* __restrict__ yb = a.report ? a.besty : a.yu1;
   const double* __restrict__ val = a.a_orig;
-  const int32_t* __restrict__ idx = a.a_idx;
+  const int32_t* __restrict__ sidx = a.a_sidx;
+  const uint16_t* __restrict__ pos = a.a_pos;
+  // the step kernels' staging order; the FP64 values stay in CSR order (a vector tile's
+  // staging order is the CSR order)
+  const bool vector = t.end - t.begin == 1 && t.p1 - t.p0 > t.cap;
   double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  rowblock_product<D2, false>(
+  rowblock_product<D2, true>(
       t, a.a_ptr,
       [&](int q) {
-        const int c = idx[q];
-        return D2{val[q] * xa[c], val[q] * xb[c]};
+        const int c = gather_index<PACK>(t, sidx, q);
+        const double v = val[vector ? q : t.p0 + staging_slot<PACK>(sidx, pos, q)];
+        return D2{v * xa[c], v * xb[c]};
       },
-      SlotIdentity{}, [](int) {},
+      [&](int q, int) { return staging_slot<PACK>(sidx, pos, q); }, [](int) {},
       [&](int i, D2 act) {
         const double lo = a.rl[i], hi = a.ru[i];
         const bool lo_f = dev_finite_bound(lo), hi_f = dev_finite_bound(hi);
@@ -446,7 +452,7 @@ __global__ __launch_bounds__(kBlock) void k_eval_rows(Args<T> a) {
           }
         }
       },
-      (D2*)nullptr, red2);
+      stage, red2);
   for (int k = 0; k < 8; ++k) {
     const double r = (k & 3) == 3 ? block_max(acc[k], red) : block_sum(acc[k], red);
     if (threadIdx.x == 0) a.pe_a[(size_t)tile * 8 + k] = r;
@@ -540,8 +546,9 @@ __device__ void decide_evaluation(InstState& s, const Residuals& cur, const Resi
 // term of evaluate() (pdhg.cpp:175-199, 239-257). Partials per tile, per candidate c:
 // [4c+0] dual violation from d, [4c+1] bound contribution, [4c+2] complementarity,
 // [4c+3] primal objective.
-template <class T>
+template <class T, int CAP, bool PACK>
 __global__ __launch_bounds__(kBlock) void k_eval_cols(Args<T> a) {
+  __shared__ D2 stage[CAP];
   __shared__ D2 red2[kBlock / kWave];
   __shared__ double red[kBlock / kWave];
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
@@ -553,15 +560,18 @@ __global__ __launch_bounds__(kBlock) void k_eval_cols(Args<T> a) {
   const double* __restrict__ ya = a.report ? a.besty : a.yu0;
   const double* __restrict__ yb = a.report ? a.besty : a.yu1;
   const double* __restrict__ val = a.t_orig;
-  const int32_t* __restrict__ idx = a.t_idx;
+  const int32_t* __restrict__ sidx = a.t_sidx;
+  const uint16_t* __restrict__ pos = a.t_pos;
+  const bool vector = t.end - t.begin == 1 && t.p1 - t.p0 > t.cap;
   double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  rowblock_product<D2, false>(
+  rowblock_product<D2, true>(
       t, a.t_ptr,
       [&](int q) {
-        const int r = idx[q];
-        return D2{val[q] * ya[r], val[q] * yb[r]};
+        const int r = gather_index<PACK>(t, sidx, q);
+        const double v = val[vector ? q : t.p0 + staging_slot<PACK>(sidx, pos, q)];
+        return D2{v * ya[r], v * yb[r]};
       },
-      SlotIdentity{}, [](int) {},
+      [&](int q, int) { return staging_slot<PACK>(sidx, pos, q); }, [](int) {},
       [&](int j, D2 dot) {
         const double lo = a.cl[j], hi = a.cu[j];
         const bool lo_f = dev_finite_bound(lo), hi_f = dev_finite_bound(hi);
@@ -585,7 +595,7 @@ __global__ __launch_bounds__(kBlock) void k_eval_cols(Args<T> a) {
           acc[4 * c + 3] += cj * xj;
         }
       },
-      (D2*)nullptr, red2);
+      stage, red2);
   for (int k = 0; k < 8; ++k) {
     const double r = (k & 3) == 2 ? block_max(acc[k], red) : block_sum(acc[k], red);
     if (threadIdx.x == 0) a.pe_t[(size_t)tile * 8 + k] = r;
