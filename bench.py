@@ -191,6 +191,8 @@ def main() -> int:
     ap.add_argument("--spmv-stage", type=int, default=-1,
                     help="staging order: -1 default, 0 CSR order, 1 sorted by gather index")
     ap.add_argument("--tile-nnz", type=int, default=0, help="row-block capacity, 0 auto")
+    ap.add_argument("--eval-stride", type=int, default=10,
+                    help="evaluation stage after every this many steps of a captured block")
     ap.add_argument("--requests", type=int, default=0,
                     help="serving stream length (0: 10000 for --workload c4, 1000 otherwise)")
     ap.add_argument("--rate", type=float, default=200.0, help="serving arrivals per second")
@@ -212,7 +214,8 @@ def main() -> int:
     if args.workload == "c4":
         return main_serving(sk, args)
     lps = instances(sk, args.workload, args.instances)
-    batch = sk.Batch(solve_options(sk, spmv_stage=args.spmv_stage, tile_nnz=args.tile_nnz))
+    batch = sk.Batch(solve_options(sk, spmv_stage=args.spmv_stage, tile_nnz=args.tile_nnz,
+                                   eval_stride=args.eval_stride))
     for lp in lps:
         batch.add(lp)
     batch.upload()

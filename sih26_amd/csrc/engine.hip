@@ -60,8 +60,17 @@ void build_row_tiles(const std::vector<int32_t>& ptr, int32_t row0, int32_t rows
   auto emit = [&](int32_t b, int32_t e) {
     out->push_back(Tile{b, e, inst, cap, ptr[static_cast<size_t>(b)], ptr[static_cast<size_t>(e)], base, 0});
   };
+  // A tile of n rows reduces each row with G = min(64, 256 / n rounded down to a power of two)
+  // lanes; a row may leave at most kMaxPerLane of its nonzeros to one lane, so a long row
+  // closes the tile early (power-law matrices mix rows of 5 and of 2,000 nonzeros).
+  constexpr int64_t kMaxPerLane = 64;
+  auto lanes_for = [](int64_t n) -> int64_t {
+    int64_t g = 1;
+    while (g * 2 * n <= kBlock && g < kWave) g *= 2;
+    return g;
+  };
   int32_t start = row0;
-  int64_t acc = 0;
+  int64_t acc = 0, longest = 0;
   const int32_t end = row0 + rows;
   for (int32_t r = row0; r < end; ++r) {
     const int64_t len = ptr[static_cast<size_t>(r) + 1] - ptr[static_cast<size_t>(r)];
@@ -69,15 +78,17 @@ void build_row_tiles(const std::vector<int32_t>& ptr, int32_t row0, int32_t rows
       if (start < r) emit(start, r);
       emit(r, r + 1);
       start = r + 1;
-      acc = 0;
+      acc = longest = 0;
       continue;
     }
-    if (acc + len > cap || r - start == kBlock) {
+    const int64_t n = r - start + 1, top = std::max(longest, len);
+    if (acc + len > cap || r - start == kBlock || (n > 1 && top > kMaxPerLane * lanes_for(n))) {
       emit(start, r);
       start = r;
-      acc = 0;
+      acc = longest = 0;
     }
     acc += len;
+    longest = std::max(longest, len);
   }
   if (start < end) emit(start, end);
   if (out->size() == before) emit(row0, row0);
