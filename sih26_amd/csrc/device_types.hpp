@@ -22,7 +22,7 @@ constexpr int kMaxTileNnz = 4096;
 // [p0, p1) is its nonzero range (row-block tables; zero for chunks), so a kernel can issue
 // its matrix loads as soon as the tile record arrives. `base` is the first gather index of
 // the tile's instance (its first column for CSR(A), first row for CSR(A^T)): packed index
-// words hold gather indices relative to it.
+// words hold gather indices relative to it. `kind` is the row-block schedule (kTile*).
 struct Tile {
   int32_t begin;
   int32_t end;
@@ -31,7 +31,18 @@ struct Tile {
   int32_t p0;
   int32_t p1;
   int32_t base;
-  int32_t pad;
+  int32_t kind;
+};
+
+// Row-block schedules. Each row's sum has one fixed order that depends only on its length, so
+// the result of a row is the same in every tiling (tile capacity and neighbours do not matter):
+// a row of at most kLongRow nonzeros is summed by one lane in CSR order; a longer row by the
+// 64 lanes of a wavefront, lane l taking nonzeros l, l+64, ... in order, then a butterfly.
+constexpr int kLongRow = 64;
+enum : int32_t {
+  kTileShort = 0,   // up to kBlock short rows, one lane each
+  kTileLong = 1,    // up to kBlock / kWave long rows, one wavefront each
+  kTileVector = 2   // one row with more than `cap` nonzeros, read straight from memory
 };
 
 // Per-tile step flags, rewritten for every tile of an instance whenever its state changes

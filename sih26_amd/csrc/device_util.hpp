@@ -73,18 +73,6 @@ __device__ __forceinline__ double block_max(double v, double* lds) {
   return r;
 }
 
-template <class A>
-__device__ __forceinline__ A block_sum_any(A v, A* lds) {
-  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
-  v = group_sum(v, kWave);
-  if (lane == 0) lds[wave] = v;
-  __syncthreads();
-  A r = lds[0];
-  for (int w = 1; w < kBlock / kWave; ++w) r = r + lds[w];
-  __syncthreads();
-  return r;
-}
-
 // Workgroups are dispatched round-robin over the 8 XCDs of the chip (workgroup b runs on
 // XCD b % 8), and each XCD has its own L2. Tile tables are ordered by instance, so giving
 // XCD x the x-th contiguous eighth of a table keeps each instance's tiles, and the slices of
@@ -105,104 +93,87 @@ struct SlotIdentity {
 constexpr int kPackShift = 12;
 constexpr int32_t kPackMaxLocal = 1 << 20;
 
-__device__ __forceinline__ int pow2_floor(int v) {
-  int p = 1;
-  while (p * 2 <= v) p *= 2;
-  return p;
-}
-
 // ---- the row-block product ---------------------------------------------------------------
 // Computes acc_r = sum_p prod(p) for every row r of tile t of a CSR matrix and calls
 // epi(r, acc_r) exactly once per row, on one lane; that lane calls pre(r) first, before any
 // product is loaded.
 //
-// Stream tile (rows > 1 or nnz <= cap): when STAGE, all lanes first write prod(p) for the
-// tile's nonzeros into LDS with coalesced loads (CSR-Stream; Greathouse & Daga, SC'14), then
-// G = 256 / rows lanes (power of two, <= 64) reduce each row from LDS. Without STAGE the
-// same G-lane groups read their row straight from memory.
-// Staging order: prod(q) is called for staging positions q in [p0, p1), and slot(q, k) (k =
+// Stream tile (kTileShort, kTileLong): all lanes first write prod(q) for the tile's nonzeros
+// into LDS with coalesced loads (CSR-Stream; Greathouse & Daga, SC'14), then each row is
+// reduced from LDS in its fixed order (see kLongRow): one lane per short row, one wavefront
+// per long row. prod(q) is called for staging positions q in [p0, p1), and slot(q, k) (k =
 // q - p0) names the CSR-order slot within the tile that product q is staged to. The step
-// kernels store a tile's nonzeros sorted by gather index, so the lanes of one load instruction
-// gather from a few cache lines; the rows are still reduced in CSR order, so the sums are the
-// same as with the identity slot map.
-// Vector tile (one row with nnz > cap): the workgroup strides over the row and reduces; its
-// staging order is always the CSR order.
-template <class A, bool STAGE, class Prod, class Slot, class Pre, class Epi>
+// kernels store a tile's nonzeros sorted by gather index, so the lanes of one load
+// instruction gather from a few cache lines; the reduction reads the slots in CSR order.
+// Vector tile (kTileVector, one row longer than the staging capacity): the row's staging
+// order is its CSR order; wavefront 0 reduces it straight from memory in the long-row order.
+template <class A, class Prod, class Slot, class Pre, class Epi>
 __device__ __forceinline__ void rowblock_product(const Tile t, const int32_t* __restrict__ ptr,
-                                                 Prod prod, Slot slot, Pre pre, Epi epi, A* stage,
-                                                 A* red_lds) {
+                                                 Prod prod, Slot slot, Pre pre, Epi epi,
+                                                 A* stage) {
   const int tid = threadIdx.x;
   const int r0 = t.begin, nrows = t.end - t.begin;
   const int p0 = t.p0;
   const int p1 = t.p1;
   const int nnz = p1 - p0;
-  if (nrows == 1 && nnz > t.cap) {
-    if (tid == 0) pre(r0);
-    A acc = zero_of<A>();
-    constexpr int U = 4;
-    for (int base = p0; base < p1; base += U * kBlock) {
-      A part[U];
-#pragma unroll
-      for (int u = 0; u < U; ++u) {
-        const int p = base + u * kBlock + tid;
-        part[u] = p < p1 ? prod(p) : zero_of<A>();
-      }
-#pragma unroll
-      for (int u = 0; u < U; ++u) acc = acc + part[u];
-    }
-    acc = block_sum_any(acc, red_lds);
-    if (tid == 0) epi(r0, acc);
-    return;
-  }
-  const int g = nrows > 0 ? min(kWave, pow2_floor(kBlock / nrows)) : kWave;
-  const int row = tid / g, lane = tid % g;
-  // the lane that finalises a row loads its epilogue operands first, so those loads are in
-  // flight together with the product loads below
-  if (row < nrows && lane == 0) pre(r0 + row);
-  const int b = row < nrows ? ptr[r0 + row] : 0;
-  const int e = row < nrows ? ptr[r0 + row + 1] : 0;
-  if (STAGE) {
-    // Unrolled batches: every lane issues all its loads of a batch before any is consumed,
-    // so each wavefront keeps U gathers in flight instead of one.
-    constexpr int U = 8;
-    for (int base = 0; base < nnz; base += U * kBlock) {
-      A part[U];
-      int to[U];
-#pragma unroll
-      for (int u = 0; u < U; ++u) {
-        const int k = base + u * kBlock + tid;
-        if (k < nnz) {
-          part[u] = prod(p0 + k);
-          to[u] = slot(p0 + k, k);
-        }
-      }
-#pragma unroll
-      for (int u = 0; u < U; ++u) {
-        const int k = base + u * kBlock + tid;
-        if (k < nnz) stage[to[u]] = part[u];
-      }
-    }
-    __syncthreads();
-  }
-  A acc = zero_of<A>();
-  if (row < nrows) {
-    if (STAGE) {
-      for (int k = b - p0 + lane; k < e - p0; k += g) acc = acc + stage[k];
-    } else {
+  if (t.kind == kTileVector) {
+    if (tid < kWave) {
+      if (tid == 0) pre(r0);
+      A acc = zero_of<A>();
       constexpr int U = 4;
-      for (int base = b + lane; base < e; base += U * g) {
+      for (int base = p0 + tid; base < p1; base += U * kWave) {
         A part[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-          const int p = base + u * g;
-          part[u] = p < e ? prod(p) : zero_of<A>();
+          const int p = base + u * kWave;
+          part[u] = p < p1 ? prod(p) : zero_of<A>();
         }
+        // in order: lane l adds nonzeros l, l + 64, l + 128, ... (adding a zero is exact)
 #pragma unroll
         for (int u = 0; u < U; ++u) acc = acc + part[u];
       }
+      acc = group_sum(acc, kWave);
+      if (tid == 0) epi(r0, acc);
+    }
+    return;
+  }
+  const bool long_rows = t.kind == kTileLong;
+  const int row = long_rows ? tid / kWave : tid;
+  const int lane = long_rows ? tid % kWave : 0;
+  // the lane that finalises a row loads its epilogue operands first, so those loads are in
+  // flight together with the product loads below
+  if (row < nrows && lane == 0) pre(r0 + row);
+  const int b = row < nrows ? ptr[r0 + row] - p0 : 0;
+  const int e = row < nrows ? ptr[r0 + row + 1] - p0 : 0;
+  // Unrolled batches: every lane issues all its loads of a batch before any is consumed, so
+  // each wavefront keeps U gathers in flight instead of one.
+  constexpr int U = 8;
+  for (int base = 0; base < nnz; base += U * kBlock) {
+    A part[U];
+    int to[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int k = base + u * kBlock + tid;
+      if (k < nnz) {
+        part[u] = prod(p0 + k);
+        to[u] = slot(p0 + k, k);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int k = base + u * kBlock + tid;
+      if (k < nnz) stage[to[u]] = part[u];
     }
   }
-  acc = group_sum(acc, g);
+  __syncthreads();
+  A acc = zero_of<A>();
+  if (long_rows) {
+    if (row < nrows)
+      for (int k = b + lane; k < e; k += kWave) acc = acc + stage[k];
+    acc = group_sum(acc, kWave);
+  } else if (row < nrows) {
+    for (int k = b; k < e; ++k) acc = acc + stage[k];
+  }
   if (row < nrows && lane == 0) epi(r0 + row, acc);
 }
 

@@ -51,47 +51,41 @@ struct DevBuf {
   }
 };
 
-// Row-block tiles over rows [row0, row0 + rows) of a CSR matrix: consecutive rows share a
-// tile while their nonzeros fit `cap` and there are at most kBlock of them; a row with more
-// than `cap` nonzeros gets a tile of its own. Every instance gets at least one tile.
+// Row-block tiles over rows [row0, row0 + rows) of a CSR matrix. Short rows (at most kLongRow
+// nonzeros) share kTileShort tiles of at most kBlock rows; long rows share kTileLong tiles of
+// at most kBlock / kWave rows; either kind holds at most `cap` nonzeros, and a row with more
+// than `cap` nonzeros is a kTileVector tile of its own. A row's sum does not depend on the
+// tiling (see kLongRow), so `cap` is free to tune. Every instance gets at least one tile.
 void build_row_tiles(const std::vector<int32_t>& ptr, int32_t row0, int32_t rows, int32_t inst,
                      int32_t cap, int32_t base, std::vector<Tile>* out) {
   const size_t before = out->size();
-  auto emit = [&](int32_t b, int32_t e) {
-    out->push_back(Tile{b, e, inst, cap, ptr[static_cast<size_t>(b)], ptr[static_cast<size_t>(e)], base, 0});
+  auto emit = [&](int32_t b, int32_t e, int32_t kind) {
+    out->push_back(Tile{b, e, inst, cap, ptr[static_cast<size_t>(b)], ptr[static_cast<size_t>(e)], base, kind});
   };
-  // A tile of n rows reduces each row with G = min(64, 256 / n rounded down to a power of two)
-  // lanes; a row may leave at most kMaxPerLane of its nonzeros to one lane, so a long row
-  // closes the tile early (power-law matrices mix rows of 5 and of 2,000 nonzeros).
-  constexpr int64_t kMaxPerLane = 64;
-  auto lanes_for = [](int64_t n) -> int64_t {
-    int64_t g = 1;
-    while (g * 2 * n <= kBlock && g < kWave) g *= 2;
-    return g;
-  };
-  int32_t start = row0;
-  int64_t acc = 0, longest = 0;
+  int32_t start = row0, kind = kTileShort;
+  int64_t acc = 0;
   const int32_t end = row0 + rows;
   for (int32_t r = row0; r < end; ++r) {
     const int64_t len = ptr[static_cast<size_t>(r) + 1] - ptr[static_cast<size_t>(r)];
     if (len > cap) {
-      if (start < r) emit(start, r);
-      emit(r, r + 1);
+      if (start < r) emit(start, r, kind);
+      emit(r, r + 1, kTileVector);
       start = r + 1;
-      acc = longest = 0;
+      acc = 0;
       continue;
     }
-    const int64_t n = r - start + 1, top = std::max(longest, len);
-    if (acc + len > cap || r - start == kBlock || (n > 1 && top > kMaxPerLane * lanes_for(n))) {
-      emit(start, r);
+    const int32_t k = len > kLongRow ? kTileLong : kTileShort;
+    const int32_t max_rows = k == kTileLong ? kBlock / kWave : kBlock;
+    if (start < r && (k != kind || acc + len > cap || r - start == max_rows)) {
+      emit(start, r, kind);
       start = r;
-      acc = longest = 0;
+      acc = 0;
     }
+    if (start == r) kind = k;
     acc += len;
-    longest = std::max(longest, len);
   }
-  if (start < end) emit(start, end);
-  if (out->size() == before) emit(row0, row0);
+  if (start < end) emit(start, end, kind);
+  if (out->size() == before) emit(row0, row0, kTileShort);
 }
 
 void build_chunks(int32_t begin, int32_t count, int32_t inst, std::vector<Tile>* out) {

@@ -189,7 +189,6 @@ __device__ __forceinline__ int staging_slot(const int32_t* __restrict__ sidx,
 template <class T, int CAP, bool PACK>
 __global__ __launch_bounds__(kBlock) void k_step_dual(Args<T> a) {
   __shared__ T stage[CAP];
-  __shared__ T red_t[kBlock / kWave];
   __shared__ double red[kBlock / kWave];
   {
     const int tile = xcd_tile(blockIdx.x, gridDim.x);
@@ -209,7 +208,7 @@ __global__ __launch_bounds__(kBlock) void k_step_dual(Args<T> a) {
     T yi = 0, lo = 0, hi = 0;
     double sum = 0.0;
     const uint16_t* __restrict__ pos = a.a_pos;
-    rowblock_product<T, true>(
+    rowblock_product<T>(
         t, a.a_ptr, [&](int q) { return val[q] * xbar[gather_index<PACK>(t, idx, q)]; },
         [&](int q, int) { return staging_slot<PACK>(idx, pos, q); },
         [&](int i) {
@@ -226,7 +225,7 @@ __global__ __launch_bounds__(kBlock) void k_step_dual(Args<T> a) {
           const double d = static_cast<double>(ynew - yi);
           d2 += d * d;
         },
-        stage, red_t);
+        stage);
     d2 = block_sum(d2, red);
     if (threadIdx.x == 0) a.ps_a[tile] = d2;
   }
@@ -298,7 +297,6 @@ __device__ void decide_step(InstState& s, double dx2, double dy2, double inter_s
 template <class T, int CAP, bool PACK>
 __global__ __launch_bounds__(kBlock) void k_step_aty(Args<T> a) {
   __shared__ T stage[CAP];
-  __shared__ T red_t[kBlock / kWave];
   __shared__ double red[kBlock / kWave];
   {
     const int tile = xcd_tile(blockIdx.x, gridDim.x);
@@ -316,7 +314,7 @@ __global__ __launch_bounds__(kBlock) void k_step_aty(Args<T> a) {
     double inter = 0.0;
     T xj = 0, xnj = 0, atyj = 0;
     const uint16_t* __restrict__ pos = a.t_pos;
-    rowblock_product<T, true>(
+    rowblock_product<T>(
         t, a.t_ptr, [&](int q) { return val[q] * yn[gather_index<PACK>(t, idx, q)]; },
         [&](int q, int) { return staging_slot<PACK>(idx, pos, q); },
         [&](int j) {
@@ -328,7 +326,7 @@ __global__ __launch_bounds__(kBlock) void k_step_aty(Args<T> a) {
           atyn[j] = r;
           inter += static_cast<double>(xnj - xj) * static_cast<double>(r - atyj);
         },
-        stage, red_t);
+        stage);
     inter = block_sum(inter, red);
     if (threadIdx.x == 0) a.ps_t[tile] = inter;
   }
@@ -402,7 +400,6 @@ __global__ __launch_bounds__(kBlock) void k_eval_prepare(Args<T> a) {
 template <class T, int CAP, bool PACK>
 __global__ __launch_bounds__(kBlock) void k_eval_rows(Args<T> a) {
   __shared__ D2 stage[CAP];
-  __shared__ D2 red2[kBlock / kWave];
   __shared__ double red[kBlock / kWave];
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
   const Tile t = a.ta[tile];
@@ -417,9 +414,9 @@ __global__ __launch_bounds__(kBlock) void k_eval_rows(Args<T> a) {
   const uint16_t* __restrict__ pos = a.a_pos;
   // the step kernels' staging order; the FP64 values stay in CSR order (a vector tile's
   // staging order is the CSR order)
-  const bool vector = t.end - t.begin == 1 && t.p1 - t.p0 > t.cap;
+  const bool vector = t.kind == kTileVector;
   double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  rowblock_product<D2, true>(
+  rowblock_product<D2>(
       t, a.a_ptr,
       [&](int q) {
         const int c = gather_index<PACK>(t, sidx, q);
@@ -452,7 +449,7 @@ __global__ __launch_bounds__(kBlock) void k_eval_rows(Args<T> a) {
           }
         }
       },
-      stage, red2);
+      stage);
   for (int k = 0; k < 8; ++k) {
     const double r = (k & 3) == 3 ? block_max(acc[k], red) : block_sum(acc[k], red);
     if (threadIdx.x == 0) a.pe_a[(size_t)tile * 8 + k] = r;
@@ -549,7 +546,6 @@ __device__ void decide_evaluation(InstState& s, const Residuals& cur, const Resi
 template <class T, int CAP, bool PACK>
 __global__ __launch_bounds__(kBlock) void k_eval_cols(Args<T> a) {
   __shared__ D2 stage[CAP];
-  __shared__ D2 red2[kBlock / kWave];
   __shared__ double red[kBlock / kWave];
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
   const Tile t = a.tt[tile];
@@ -562,9 +558,9 @@ __global__ __launch_bounds__(kBlock) void k_eval_cols(Args<T> a) {
   const double* __restrict__ val = a.t_orig;
   const int32_t* __restrict__ sidx = a.t_sidx;
   const uint16_t* __restrict__ pos = a.t_pos;
-  const bool vector = t.end - t.begin == 1 && t.p1 - t.p0 > t.cap;
+  const bool vector = t.kind == kTileVector;
   double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  rowblock_product<D2, true>(
+  rowblock_product<D2>(
       t, a.t_ptr,
       [&](int q) {
         const int r = gather_index<PACK>(t, sidx, q);
@@ -595,7 +591,7 @@ __global__ __launch_bounds__(kBlock) void k_eval_cols(Args<T> a) {
           acc[4 * c + 3] += cj * xj;
         }
       },
-      stage, red2);
+      stage);
   for (int k = 0; k < 8; ++k) {
     const double r = (k & 3) == 2 ? block_max(acc[k], red) : block_sum(acc[k], red);
     if (threadIdx.x == 0) a.pe_t[(size_t)tile * 8 + k] = r;
@@ -775,7 +771,7 @@ __global__ __launch_bounds__(kBlock) void k_sort_tiles(const Tile* __restrict__ 
   const Tile t = tiles[blockIdx.x];
   const int tid = threadIdx.x;
   const int p0 = ptr[t.begin], nnz = ptr[t.end] - p0;
-  const bool vector = t.end - t.begin == 1 && nnz > t.cap;
+  const bool vector = t.kind == kTileVector;
   // one staging entry: gather index g, CSR slot k (a vector tile never uses its slots)
   auto put = [&](int q, int g, int k) {
     if (pack) {
@@ -827,7 +823,7 @@ __global__ __launch_bounds__(kBlock) void k_stage_values(const Tile* __restrict_
                                                           T* __restrict__ dst) {
   const Tile t = tiles[blockIdx.x];
   const int p0 = ptr[t.begin], nnz = ptr[t.end] - p0;
-  const bool vector = t.end - t.begin == 1 && nnz > t.cap;
+  const bool vector = t.kind == kTileVector;
   for (int k = threadIdx.x; k < nnz; k += kBlock) {
     const int slot = vector ? k
                      : pack ? static_cast<int>(static_cast<uint32_t>(sidx[p0 + k]) & 0xFFFu)
@@ -853,16 +849,16 @@ __global__ __launch_bounds__(kBlock) void k_spmv_tiles(const Tile* __restrict__ 
   __shared__ double red[kBlock / kWave];
   const Tile t = tiles[xcd_tile(blockIdx.x, gridDim.x)];
   // a vector tile keeps the CSR order, so its value index is the position itself
-  const bool vector = t.end - t.begin == 1 && t.p1 - t.p0 > t.cap;
+  const bool vector = t.kind == kTileVector;
   auto gather = [&](int q) {
     return PACK ? t.base + static_cast<int>(static_cast<uint32_t>(sidx[q]) >> kPackShift) : sidx[q];
   };
   auto slot = [&](int q, int) {
     return PACK ? static_cast<int>(static_cast<uint32_t>(sidx[q]) & 0xFFFu) : static_cast<int>(pos[q]);
   };
-  rowblock_product<double, true>(
+  rowblock_product<double>(
       t, ptr, [&](int q) { return val[vector ? q : t.p0 + slot(q, 0)] * x[gather(q)]; }, slot,
-      [](int) {}, [&](int r, double acc) { y[r] = acc; }, stage, red);
+      [](int) {}, [&](int r, double acc) { y[r] = acc; }, stage);
 }
 
 // Per column chunk: sum of v^2 (slot 0 of pe_c).
