@@ -238,6 +238,18 @@ int32_t sk_pdhg_solve(const sk_lp* lp, const sk_pdhg_options* options, sk_pdhg_r
                       double* col_value, double* row_dual, double* col_dual,
                       double* row_activity);
 
+/* writer.cpp:119-227 write_solution: the .sol file the reference's independent checker
+ * (tools/verify_solution.py) reads. Header lines, then `columns` (name value reduced_cost
+ * basis_status) and `rows` (name activity dual basis_status) blocks, every number with 17
+ * significant digits; a status that claims no point (model.hpp:99-112) writes the header only.
+ * Names default to C<j> and R<i>; col_names / row_names may be NULL. The arrays are those
+ * sk_pdhg_solve fills (NULL writes zeros). */
+int32_t sk_write_solution(const char* path, const sk_lp* lp, const sk_pdhg_result* result,
+                          const double* col_value, const double* row_dual,
+                          const double* col_dual, const double* row_activity,
+                          const char* const* col_names, const char* const* row_names,
+                          const char* model_name);
+
 /* ---- batches: many LPs of uneven size in one block-diagonal solve -------------------------- */
 typedef struct sk_batch sk_batch;
 

@@ -114,6 +114,7 @@ SIGNATURES = {
     "sk_pdhg_default_options": (None, [P(PdhgOptions)]),
     "sk_pdhg_set_option": (i32, [P(PdhgOptions), cstr, cstr]),
     "sk_pdhg_solve": (i32, [vp, P(PdhgOptions), P(PdhgResult), vp, vp, vp, vp]),
+    "sk_write_solution": (i32, [cstr, vp, P(PdhgResult), vp, vp, vp, vp, vp, vp, cstr]),
     "sk_batch_create": (i32, [P(PdhgOptions), P(vp)]),
     "sk_batch_add": (i32, [vp, vp, P(i32)]),
     "sk_batch_upload": (i32, [vp]),
@@ -137,7 +138,7 @@ SIGNATURES = {
 def load(path: Path = LIB_PATH) -> C.CDLL:
     if not path.exists():
         raise ImportError(
-            f"{path} is not built; run `python -m sih26_amd.build` (or __graft_entry__.build())")
+            f"{path} is not built; run `python sih26_amd/build.py` (or __graft_entry__.build())")
     lib = C.CDLL(str(path))
     for name, (res, args) in SIGNATURES.items():
         fn = getattr(lib, name)

@@ -181,7 +181,24 @@ def _result(res: _ffi.PdhgResult, x, yd, d, act) -> SolveResult:
     info = {f: getattr(res, f) for f, _ in res._fields_}
     info["algorithm"] = info["algorithm"].decode()
     info["message"] = info["message"].decode()
-    return SolveResult(res.status, STATUS_NAMES.get(res.status, "?"), info, x, yd, d, act)
+    return SolveResult(res.status, STATUS_NAMES.get(res.status, "?"), info, x, yd, d, act,
+                       extra={"raw": res})
+
+
+def write_solution(path, lp: Lp, result: SolveResult, model_name: str | None = None,
+                   col_names=None, row_names=None) -> None:
+    """The reference's .sol file for one answer (sk_write_solution)."""
+    def names(seq):
+        if seq is None:
+            return None
+        arr = (C.c_char_p * len(seq))(*[s.encode() for s in seq])
+        return C.cast(arr, C.c_void_p), arr
+    cn, rn = names(col_names), names(row_names)
+    check(lib().sk_write_solution(str(path).encode(), lp._h, C.byref(result.extra["raw"]),
+                                  _ptr(result.col_value), _ptr(result.row_dual),
+                                  _ptr(result.col_dual), _ptr(result.row_activity),
+                                  cn[0] if cn else None, rn[0] if rn else None,
+                                  (model_name or "").encode()), "sk_write_solution")
 
 
 def solve(lp: Lp, options: _ffi.PdhgOptions | None = None, **kw) -> SolveResult:

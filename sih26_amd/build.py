@@ -1,6 +1,6 @@
 """Build recipe for libsk_gpu.so (HIP for gfx950) from sih26_amd/csrc.
 
-    python -m sih26_amd.build            # compile into sih26_amd/lib/libsk_gpu.so
+    python sih26_amd/build.py            # compile into sih26_amd/lib/libsk_gpu.so
 
 Device sources (*.hip) go through hipcc for gfx950 only; host sources (*.cpp) through g++.
 Everything links into one shared library with a plain C ABI (include/sk_gpu.h).

@@ -24,9 +24,7 @@
 #include "server.hpp"
 #include "status.hpp"
 
-struct sk_lp {
-  sk::HostLp lp;
-};
+#include "handles.hpp"
 
 struct sk_batch {
   sk_pdhg_options options;

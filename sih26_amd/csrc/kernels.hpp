@@ -846,7 +846,6 @@ __global__ __launch_bounds__(kBlock) void k_spmv_tiles(const Tile* __restrict__ 
                                                         const double* __restrict__ x,
                                                         double* __restrict__ y) {
   __shared__ double stage[kMaxTileNnz];
-  __shared__ double red[kBlock / kWave];
   const Tile t = tiles[xcd_tile(blockIdx.x, gridDim.x)];
   // a vector tile keeps the CSR order, so its value index is the position itself
   const bool vector = t.kind == kTileVector;
