@@ -172,7 +172,16 @@ __device__ __forceinline__ void rowblock_product(const Tile t, const int32_t* __
       for (int k = b + lane; k < e; k += kWave) acc = acc + stage[k];
     acc = group_sum(acc, kWave);
   } else if (row < nrows) {
-    for (int k = b; k < e; ++k) acc = acc + stage[k];
+    // in CSR order; the reads of a batch of 8 are all issued before the first add
+    int k = b;
+    for (; k + 8 <= e; k += 8) {
+      A v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) v[u] = stage[k + u];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) acc = acc + v[u];
+    }
+    for (; k < e; ++k) acc = acc + stage[k];
   }
   if (row < nrows && lane == 0) epi(r0 + row, acc);
 }

@@ -596,8 +596,8 @@ int EngineImpl<T>::enqueue_setup() {
   // 10 Ruiz rounds, then one Pock-Chambolle round (scaling.cpp:121-122)
   for (int pass = 0; pass < 11; ++pass) {
     const int use_sum = pass == 10;
-    SK_LAUNCH(k_row_pass<<<blocks_for(M), kBlock, 0, s>>>(a.a_ptr, d_sa_, M, use_sum, d_rpass_));
-    SK_LAUNCH(k_row_pass<<<blocks_for(N), kBlock, 0, s>>>(a.t_ptr, d_st_, N, use_sum, d_cpass_));
+    SK_LAUNCH(k_row_pass<<<n_ta_, kBlock, 0, s>>>(a.ta, a.a_ptr, d_sa_, use_sum, d_rpass_));
+    SK_LAUNCH(k_row_pass<<<n_tt_, kBlock, 0, s>>>(a.tt, a.t_ptr, d_st_, use_sum, d_cpass_));
     SK_LAUNCH(k_apply_pass<<<n_ta_, kBlock, 0, s>>>(a.ta, a.a_ptr, a.a_idx, d_sa_, 1, d_rpass_, d_cpass_));
     SK_LAUNCH(k_apply_pass<<<n_tt_, kBlock, 0, s>>>(a.tt, a.t_ptr, a.t_idx, d_st_, 0, d_rpass_, d_cpass_));
     SK_LAUNCH(k_scale_mul<<<blocks_for(M), kBlock, 0, s>>>(const_cast<double*>(a.dr), d_rpass_, M));
@@ -633,11 +633,11 @@ int EngineImpl<T>::enqueue_setup() {
     SK_LAUNCH(k_inst_norm<<<K_, kBlock, 0, s>>>(a.desc, a.ps_c, d_len_, d_zero_seen_));
     SK_LAUNCH(k_chunk_div<<<n_tc_, kBlock, 0, s>>>(a.tc, d_len_, d_v_));
     if (pack_) {
-      SK_LAUNCH(k_spmv_tiles<true><<<n_ta_, kBlock, 0, s>>>(a.ta, a.a_ptr, a.a_sidx, a.a_pos, d_sa_, d_v_, d_av_));
-      SK_LAUNCH(k_spmv_tiles<true><<<n_tt_, kBlock, 0, s>>>(a.tt, a.t_ptr, a.t_sidx, a.t_pos, d_st_, d_av_, d_v_));
+      SK_LAUNCH(k_spmv_tiles<T, true><<<n_ta_, kBlock, 0, s>>>(a.ta, a.a_ptr, a.a_sidx, a.a_pos, a.a_val, d_v_, d_av_));
+      SK_LAUNCH(k_spmv_tiles<T, true><<<n_tt_, kBlock, 0, s>>>(a.tt, a.t_ptr, a.t_sidx, a.t_pos, a.t_val, d_av_, d_v_));
     } else {
-      SK_LAUNCH(k_spmv_tiles<false><<<n_ta_, kBlock, 0, s>>>(a.ta, a.a_ptr, a.a_sidx, a.a_pos, d_sa_, d_v_, d_av_));
-      SK_LAUNCH(k_spmv_tiles<false><<<n_tt_, kBlock, 0, s>>>(a.tt, a.t_ptr, a.t_sidx, a.t_pos, d_st_, d_av_, d_v_));
+      SK_LAUNCH(k_spmv_tiles<T, false><<<n_ta_, kBlock, 0, s>>>(a.ta, a.a_ptr, a.a_sidx, a.a_pos, a.a_val, d_v_, d_av_));
+      SK_LAUNCH(k_spmv_tiles<T, false><<<n_tt_, kBlock, 0, s>>>(a.tt, a.t_ptr, a.t_sidx, a.t_pos, a.t_val, d_av_, d_v_));
     }
   }
   SK_LAUNCH(k_chunk_sumsq<<<n_tc_, kBlock, 0, s>>>(a.tc, d_v_, a.ps_c));

@@ -675,18 +675,31 @@ __global__ __launch_bounds__(kBlock) void k_eval_commit(Args<T> a) {
 
 // Ruiz (max) or Pock-Chambolle (sum) statistic of every row of a CSR matrix, as the pass
 // multiplier 1/sqrt(stat) (1 for an empty row). Sums run in stored order, which is the
-// reference's accumulation order (scaling.cpp:44-100). One lane per row.
-__global__ __launch_bounds__(kBlock) void k_row_pass(const int32_t* __restrict__ ptr,
-                                                      const double* __restrict__ val, int32_t rows,
+// reference's accumulation order (scaling.cpp:44-100): one lane per row, over the tile's
+// |values| staged in LDS with coalesced loads (a row longer than the staging capacity is read
+// from memory by its lane).
+__global__ __launch_bounds__(kBlock) void k_row_pass(const Tile* __restrict__ tiles,
+                                                      const int32_t* __restrict__ ptr,
+                                                      const double* __restrict__ val,
                                                       int32_t use_sum, double* __restrict__ pass) {
-  const int r = blockIdx.x * kBlock + threadIdx.x;
-  if (r >= rows) return;
-  double acc = 0.0;
-  for (int p = ptr[r]; p < ptr[r + 1]; ++p) {
-    const double m = fabs(val[p]);
-    acc = use_sum ? acc + m : dev_max(acc, m);
+  __shared__ double mag[kMaxTileNnz];
+  const Tile t = tiles[blockIdx.x];
+  const int nrows = t.end - t.begin, nnz = t.p1 - t.p0;
+  const bool staged = t.kind != kTileVector;
+  if (staged) {
+    for (int k = threadIdx.x; k < nnz; k += kBlock) mag[k] = fabs(val[t.p0 + k]);
+    __syncthreads();
   }
-  pass[r] = acc > 0.0 ? 1.0 / sqrt(acc) : 1.0;
+  for (int row = threadIdx.x; row < nrows; row += kBlock) {
+    const int r = t.begin + row;
+    const int b = ptr[r] - t.p0, e = ptr[r + 1] - t.p0;
+    double acc = 0.0;
+    for (int k = b; k < e; ++k) {
+      const double m = staged ? mag[k] : fabs(val[t.p0 + k]);
+      acc = use_sum ? acc + m : dev_max(acc, m);
+    }
+    pass[r] = acc > 0.0 ? 1.0 / sqrt(acc) : 1.0;
+  }
 }
 
 // value * row_pass[i] * column_pass[j] in both layouts (scaling.cpp:14-40), one workgroup per
@@ -834,30 +847,26 @@ __global__ __launch_bounds__(kBlock) void k_stage_values(const Tile* __restrict_
 
 // ---- power iteration (scaling.cpp:161-194) ---------------------------------------------------
 
-// y = A x over the row-block tiles of a CSR matrix in FP64 with the step kernels' staged
-// schedule (gathers in each tile's staging order, rows reduced in CSR order); `val` is in CSR
-// order and read through the slot map. The power iteration only sets the starting step size.
-template <bool PACK>
+// y = A x over the row-block tiles of a CSR matrix with the step kernels' staged schedule:
+// the scaled values in the staging order and the iterate precision V (gathers coalesced,
+// rows reduced in CSR order), products and sums in FP64. With V = double the products are
+// those of the FP64 scaled matrix; with V = float, of its FP32 copy that the steps use. The
+// power iteration only sets the starting step size and its ceiling.
+template <class V, bool PACK>
 __global__ __launch_bounds__(kBlock) void k_spmv_tiles(const Tile* __restrict__ tiles,
                                                         const int32_t* __restrict__ ptr,
                                                         const int32_t* __restrict__ sidx,
                                                         const uint16_t* __restrict__ pos,
-                                                        const double* __restrict__ val,
+                                                        const V* __restrict__ val,
                                                         const double* __restrict__ x,
                                                         double* __restrict__ y) {
   __shared__ double stage[kMaxTileNnz];
   const Tile t = tiles[xcd_tile(blockIdx.x, gridDim.x)];
-  // a vector tile keeps the CSR order, so its value index is the position itself
-  const bool vector = t.kind == kTileVector;
-  auto gather = [&](int q) {
-    return PACK ? t.base + static_cast<int>(static_cast<uint32_t>(sidx[q]) >> kPackShift) : sidx[q];
-  };
-  auto slot = [&](int q, int) {
-    return PACK ? static_cast<int>(static_cast<uint32_t>(sidx[q]) & 0xFFFu) : static_cast<int>(pos[q]);
-  };
   rowblock_product<double>(
-      t, ptr, [&](int q) { return val[vector ? q : t.p0 + slot(q, 0)] * x[gather(q)]; }, slot,
-      [](int) {}, [&](int r, double acc) { y[r] = acc; }, stage);
+      t, ptr,
+      [&](int q) { return static_cast<double>(val[q]) * x[gather_index<PACK>(t, sidx, q)]; },
+      [&](int q, int) { return staging_slot<PACK>(sidx, pos, q); }, [](int) {},
+      [&](int r, double acc) { y[r] = acc; }, stage);
 }
 
 // Per column chunk: sum of v^2 (slot 0 of pe_c).
