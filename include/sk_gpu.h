@@ -184,7 +184,9 @@ typedef struct {
   int32_t eval_stride;       /* an evaluation stage follows every this many steps inside a
                                 captured block, so an instance due for evaluation waits at most
                                 this many steps (default 10) */
-  int32_t reserved0;
+  int32_t streams;           /* a batch runs as this many sub-batches side by side, each on
+                                its own stream: 0 automatic (3 for batches of 2^20 nonzeros and
+                                more, else 1), 1 one block-diagonal launch per kernel */
   int32_t reserved1;
 } sk_pdhg_options;
 
@@ -263,6 +265,8 @@ typedef struct {
   int64_t total_nnz;
   int32_t instances;
   int32_t spmv_stage;
+  int32_t streams;  /* sub-batches the batch ran as (sk_pdhg_options.streams) */
+  int32_t reserved;
 } sk_batch_stats;
 
 int32_t sk_batch_create(const sk_pdhg_options* options, sk_batch** out);

@@ -43,7 +43,7 @@ class PdhgOptions(C.Structure):
                 ("restart", i32), ("stop_at_request", i32), ("absolute_primal", i32),
                 ("random_seed", i64), ("precision", i32), ("steps_per_block", i32),
                 ("use_graph", i32), ("spmv_stage", i32), ("tile_nnz", i32),
-                ("compute_quality", i32), ("eval_stride", i32), ("reserved0", i32),
+                ("compute_quality", i32), ("eval_stride", i32), ("streams", i32),
                 ("reserved1", i32)]
 
 
@@ -64,7 +64,8 @@ class PdhgResult(C.Structure):
 class BatchStats(C.Structure):
     _fields_ = [("setup_ms", f64), ("loop_ms", f64), ("report_ms", f64),
                 ("graph_launches", i64), ("total_rows", i64), ("total_cols", i64),
-                ("total_nnz", i64), ("instances", i32), ("spmv_stage", i32)]
+                ("total_nnz", i64), ("instances", i32), ("spmv_stage", i32),
+                ("streams", i32), ("reserved", i32)]
 
 
 class ServerOptions(C.Structure):

@@ -67,6 +67,7 @@ sk::EngineOptions to_engine(const sk_pdhg_options& o) {
   e.use_graph = o.use_graph;
   e.stage = o.spmv_stage;
   e.tile_nnz = o.tile_nnz;
+  e.streams = o.streams;
   return e;
 }
 
@@ -342,7 +343,7 @@ void sk_pdhg_default_options(sk_pdhg_options* o) {
   o->tile_nnz = 0;
   o->compute_quality = 1;
   o->eval_stride = 10;
-  o->reserved0 = 0;
+  o->streams = 0;
   o->reserved1 = 0;
 }
 
@@ -514,6 +515,8 @@ int32_t sk_batch_stats_get(const sk_batch* b, sk_batch_stats* out) {
   out->total_nnz = b->engine->total_nnz();
   out->instances = b->engine->size();
   out->spmv_stage = b->engine->stage_choice();
+  out->streams = b->engine->streams();
+  out->reserved = 0;
   return SK_GPU_SUCCESS;
 }
 

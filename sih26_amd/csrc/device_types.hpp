@@ -15,6 +15,9 @@ constexpr int kChunk = 1024;
 // Largest nonzero count a row-block tile may stage in LDS; a single row above it is
 // processed by the whole workgroup without staging ("vector" tile).
 constexpr int kMaxTileNnz = 4096;
+// Extra elements allocated after the staging-order matrix arrays: the step kernels load them
+// in aligned groups of four, and the last group may reach three elements past the end.
+constexpr int kStagePad = 4;
 
 // A tile is a half-open range [begin, end) of rows (CSR tables) or of elements (chunk
 // tables) that belongs to exactly one instance. `cap` is the staging capacity the tile was

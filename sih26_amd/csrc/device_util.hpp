@@ -84,41 +84,42 @@ __device__ __forceinline__ int xcd_tile(int b, int n) {
   return x * per + min(x, extra) + k;
 }
 
-// The identity slot map of rowblock_product (staging order = CSR order).
-struct SlotIdentity {
-  __device__ __forceinline__ int operator()(int, int k) const { return k; }
-};
 // Packed index words: (gather index - tile base) << 12 | CSR slot. Usable when every stream
 // tile holds at most 4096 nonzeros and every instance has fewer than 2^20 rows and columns.
 constexpr int kPackShift = 12;
 constexpr int32_t kPackMaxLocal = 1 << 20;
 
 // ---- the row-block product ---------------------------------------------------------------
-// Computes acc_r = sum_p prod(p) for every row r of tile t of a CSR matrix and calls
-// epi(r, acc_r) exactly once per row, on one lane; that lane calls pre(r) first, before any
-// product is loaded.
+// Computes acc_r = sum_p prod(p) for every row r of tile t of a CSR matrix with a workgroup of
+// NT lanes (NT = kWave or kBlock) and calls epi(r, k, acc_r) exactly once per row, on one
+// lane; that lane calls pre(r, k) first, before any product is loaded. k in [0, kBlock / NT)
+// numbers the rows a lane finalises, so the caller can keep per-row operands in registers.
 //
-// Stream tile (kTileShort, kTileLong): all lanes first write prod(q) for the tile's nonzeros
-// into LDS with coalesced loads (CSR-Stream; Greathouse & Daga, SC'14), then each row is
-// reduced from LDS in its fixed order (see kLongRow): one lane per short row, one wavefront
-// per long row. prod(q) is called for staging positions q in [p0, p1), and slot(q, k) (k =
-// q - p0) names the CSR-order slot within the tile that product q is staged to. The step
-// kernels store a tile's nonzeros sorted by gather index, so the lanes of one load
-// instruction gather from a few cache lines; the reduction reads the slots in CSR order.
+// Stream tile (kTileShort, kTileLong): all lanes first write the tile's products into LDS with
+// coalesced loads (CSR-Stream; Greathouse & Daga, SC'14) through stager(t, stage), then each
+// row is reduced from LDS in its fixed order (see kLongRow): one lane per short row (lane l
+// takes rows l, l + NT, ...), one wavefront per long row. The stager writes the product of
+// every staging position q in [t.p0, t.p1) to the CSR-order slot it belongs to within the
+// tile (stage[0 .. p1 - p0)). The step kernels store a tile's nonzeros sorted by gather index,
+// so the lanes of one load instruction gather from a few cache lines; the reduction reads the
+// slots in CSR order.
 // Vector tile (kTileVector, one row longer than the staging capacity): the row's staging
-// order is its CSR order; wavefront 0 reduces it straight from memory in the long-row order.
-template <class A, class Prod, class Slot, class Pre, class Epi>
+// order is its CSR order; wavefront 0 reduces prod(q), q in [p0, p1), straight from memory in
+// the long-row order.
+template <int NT, class A, class Prod, class Stager, class Pre, class Epi>
 __device__ __forceinline__ void rowblock_product(const Tile t, const int32_t* __restrict__ ptr,
-                                                 Prod prod, Slot slot, Pre pre, Epi epi,
+                                                 Prod prod, Stager stager, Pre pre, Epi epi,
                                                  A* stage) {
+  static_assert(NT % kWave == 0 && kBlock % NT == 0, "NT must be a multiple of the wavefront");
+  constexpr int R = kBlock / NT;  // short rows per lane; long rows per wavefront
+  constexpr int W = NT / kWave;   // wavefronts
   const int tid = threadIdx.x;
   const int r0 = t.begin, nrows = t.end - t.begin;
   const int p0 = t.p0;
   const int p1 = t.p1;
-  const int nnz = p1 - p0;
   if (t.kind == kTileVector) {
     if (tid < kWave) {
-      if (tid == 0) pre(r0);
+      if (tid == 0) pre(r0, 0);
       A acc = zero_of<A>();
       constexpr int U = 4;
       for (int base = p0 + tid; base < p1; base += U * kWave) {
@@ -133,57 +134,48 @@ __device__ __forceinline__ void rowblock_product(const Tile t, const int32_t* __
         for (int u = 0; u < U; ++u) acc = acc + part[u];
       }
       acc = group_sum(acc, kWave);
-      if (tid == 0) epi(r0, acc);
+      if (tid == 0) epi(r0, 0, acc);
     }
     return;
   }
   const bool long_rows = t.kind == kTileLong;
-  const int row = long_rows ? tid / kWave : tid;
-  const int lane = long_rows ? tid % kWave : 0;
+  const int wave = tid / kWave, lane = tid % kWave;
   // the lane that finalises a row loads its epilogue operands first, so those loads are in
   // flight together with the product loads below
-  if (row < nrows && lane == 0) pre(r0 + row);
-  const int b = row < nrows ? ptr[r0 + row] - p0 : 0;
-  const int e = row < nrows ? ptr[r0 + row + 1] - p0 : 0;
-  // Unrolled batches: every lane issues all its loads of a batch before any is consumed, so
-  // each wavefront keeps U gathers in flight instead of one.
-  constexpr int U = 8;
-  for (int base = 0; base < nnz; base += U * kBlock) {
-    A part[U];
-    int to[U];
+  int b[R], e[R];
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int k = base + u * kBlock + tid;
-      if (k < nnz) {
-        part[u] = prod(p0 + k);
-        to[u] = slot(p0 + k, k);
-      }
-    }
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int k = base + u * kBlock + tid;
-      if (k < nnz) stage[to[u]] = part[u];
-    }
+  for (int k = 0; k < R; ++k) {
+    const int row = long_rows ? wave + W * k : tid + NT * k;
+    if (row < nrows && (!long_rows || lane == 0)) pre(r0 + row, k);
+    b[k] = row < nrows ? ptr[r0 + row] - p0 : 0;
+    e[k] = row < nrows ? ptr[r0 + row + 1] - p0 : 0;
   }
+  stager(t, stage);
   __syncthreads();
-  A acc = zero_of<A>();
-  if (long_rows) {
-    if (row < nrows)
-      for (int k = b + lane; k < e; k += kWave) acc = acc + stage[k];
-    acc = group_sum(acc, kWave);
-  } else if (row < nrows) {
-    // in CSR order; the reads of a batch of 8 are all issued before the first add
-    int k = b;
-    for (; k + 8 <= e; k += 8) {
-      A v[8];
 #pragma unroll
-      for (int u = 0; u < 8; ++u) v[u] = stage[k + u];
+  for (int k = 0; k < R; ++k) {
+    const int row = long_rows ? wave + W * k : tid + NT * k;
+    A acc = zero_of<A>();
+    if (long_rows) {
+      // `row` is the same in every lane of the wavefront
+      if (row < nrows)
+        for (int q = b[k] + lane; q < e[k]; q += kWave) acc = acc + stage[q];
+      acc = group_sum(acc, kWave);
+      if (row < nrows && lane == 0) epi(r0 + row, k, acc);
+    } else if (row < nrows) {
+      // in CSR order; the reads of a batch of 8 are all issued before the first add
+      int q = b[k];
+      for (; q + 8 <= e[k]; q += 8) {
+        A v[8];
 #pragma unroll
-      for (int u = 0; u < 8; ++u) acc = acc + v[u];
+        for (int u = 0; u < 8; ++u) v[u] = stage[q + u];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) acc = acc + v[u];
+      }
+      for (; q < e[k]; ++q) acc = acc + stage[q];
+      epi(r0 + row, k, acc);
     }
-    for (; k < e; ++k) acc = acc + stage[k];
   }
-  if (row < nrows && lane == 0) epi(r0 + row, acc);
 }
 
 }  // namespace sk

@@ -121,6 +121,7 @@ class EngineImpl final : public Engine {
   int64_t total_rows() const override { return m_; }
   int64_t total_cols() const override { return n_; }
   int stage_choice() const override { return stage_; }
+  int32_t streams() const override { return 1; }
 
   int upload() override;
   int solve() override;
@@ -177,8 +178,8 @@ class EngineImpl final : public Engine {
     return with_layout([&](auto cap, auto pack) -> int {
       constexpr int C = decltype(cap)::value;
       constexpr bool P = decltype(pack)::value;
-      if (kernel == 1) SK_LAUNCH(k_step_dual<T, C, P><<<grid, kBlock, 0, s>>>(a));
-      else SK_LAUNCH(k_step_aty<T, C, P><<<grid, kBlock, 0, s>>>(a));
+      if (kernel == 1) SK_LAUNCH(k_step_dual<T, C, P><<<grid, kStepLanes, 0, s>>>(a));
+      else SK_LAUNCH(k_step_aty<T, C, P><<<grid, kStepLanes, 0, s>>>(a));
       return kStatusSuccess;
     });
   }
@@ -427,7 +428,6 @@ int EngineImpl<T>::upload() {
     SK_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev), kStatusInitFailed);
     cus_ = std::max(1, cus);
   }
-  if (!copy_stream_) SK_HIP(hipStreamCreateWithFlags(&copy_stream_, hipStreamNonBlocking), kStatusInitFailed);
   if (!host_done_) SK_HIP(hipHostMalloc(reinterpret_cast<void**>(&host_done_), 4 * sizeof(uint32_t)), kStatusOutOfMemory);
   if (host_state_cap_ < static_cast<size_t>(K_)) {
     if (host_state_) (void)hipHostFree(host_state_);
@@ -461,8 +461,9 @@ int EngineImpl<T>::upload() {
   SK_TRY(alloc(b_sa_, Z, &d_sa_));
   SK_TRY(alloc(b_st_, Z, &d_st_));
   T *av, *tv;
-  SK_TRY(alloc(b_a_val_, Z, &av));
-  SK_TRY(alloc(b_t_val_, Z, &tv));
+  // the staging-order arrays carry kStagePad elements of slack (read in aligned groups of 4)
+  SK_TRY(alloc(b_a_val_, Z + kStagePad, &av));
+  SK_TRY(alloc(b_t_val_, Z + kStagePad, &tv));
   a.a_val = av;
   a.t_val = tv;
   double *pcm, *pcl, *pcu, *prl, *pru, *pdc, *pdr;
@@ -531,10 +532,10 @@ int EngineImpl<T>::upload() {
     for (const InstDesc& d : desc_) pack_ = pack_ && d.rows < kPackMaxLocal && d.cols < kPackMaxLocal;
     int32_t *asx, *tsx;
     uint16_t *apos, *tpos;
-    SK_TRY(alloc(b_a_sidx_, Z, &asx));
-    SK_TRY(alloc(b_t_sidx_, Z, &tsx));
-    SK_TRY(alloc(b_a_pos_, pack_ ? 0 : Z, &apos));
-    SK_TRY(alloc(b_t_pos_, pack_ ? 0 : Z, &tpos));
+    SK_TRY(alloc(b_a_sidx_, Z + kStagePad, &asx));
+    SK_TRY(alloc(b_t_sidx_, Z + kStagePad, &tsx));
+    SK_TRY(alloc(b_a_pos_, pack_ ? 0 : Z + kStagePad, &apos));
+    SK_TRY(alloc(b_t_pos_, pack_ ? 0 : Z + kStagePad, &tpos));
     const int32_t sorted = stage_ != 0, pack = pack_ ? 1 : 0;
     SK_LAUNCH(k_sort_tiles<<<n_ta_, kBlock, 0, stream_>>>(pta, ap, ai, sorted, pack, asx, apos));
     SK_LAUNCH(k_sort_tiles<<<n_tt_, kBlock, 0, stream_>>>(ptt, tp, ti, sorted, pack, tsx, tpos));
@@ -844,6 +845,9 @@ int EngineImpl<T>::start() {
     set_last_error("start() while the previous solve is still running");
     return kStatusInvalidState;
   }
+  // the read-back stream of the serving path (created on first use: solve() does not need it,
+  // and every stream of a process takes one of its few hardware queues)
+  if (!copy_stream_) SK_HIP(hipStreamCreateWithFlags(&copy_stream_, hipStreamNonBlocking), kStatusInitFailed);
   started_ = std::chrono::steady_clock::now();
   int st = setup_device();
   if (st) return st;
@@ -1151,9 +1155,19 @@ int64_t estimate_device_bytes(int64_t rows, int64_t cols, int64_t nnz, int64_t i
 void set_last_error(const std::string& message) { g_last_error = message; }
 const char* last_error() { return g_last_error.c_str(); }
 
-std::unique_ptr<Engine> make_engine(const EngineOptions& options) {
+std::unique_ptr<Engine> make_single_engine(const EngineOptions& options) {
   if (options.precision == 1) return std::make_unique<EngineImpl<double>>(options);
   return std::make_unique<EngineImpl<float>>(options);
+}
+
+int current_device(int* device) {
+  SK_HIP(hipGetDevice(device), kStatusNoDevice);
+  return kStatusSuccess;
+}
+
+int use_device(int device) {
+  SK_HIP(hipSetDevice(device), kStatusNoDevice);
+  return kStatusSuccess;
 }
 
 }  // namespace sk

@@ -31,6 +31,7 @@ struct EngineOptions {
   int32_t use_graph = 1;
   int32_t stage = -1;             // staging order of SpMV tiles: -1 default (1), 0 CSR, 1 sorted
   int32_t tile_nnz = 0;           // row-block capacity; 0 picks per instance
+  int32_t streams = 0;            // concurrent sub-batches: 0 automatic, 1 one stream, n >= 2
 };
 
 struct InstanceResult {
@@ -89,9 +90,19 @@ class Engine {
   virtual int64_t total_rows() const = 0;
   virtual int64_t total_cols() const = 0;
   virtual int stage_choice() const = 0;
+  // Sub-batches the batch runs as (each on its own stream).
+  virtual int32_t streams() const = 0;
 };
 
+// An engine for a batch: with options.streams other than 1 the batch is split into
+// sub-batches that run concurrently, each on its own stream (split.cpp).
 std::unique_ptr<Engine> make_engine(const EngineOptions& options);
+// One block-diagonal batch on one stream.
+std::unique_ptr<Engine> make_single_engine(const EngineOptions& options);
+
+// The calling thread's HIP device, and selecting one (status codes of status.hpp).
+int current_device(int* device);
+int use_device(int device);
 
 // Device bytes an engine allocates for a batch with these totals (rows, columns, nonzeros
 // and instances summed over the batch), rounded up; used for admission by memory footprint.

@@ -172,63 +172,151 @@ __global__ __launch_bounds__(kBlock) void k_step_primal(Args<T> a) {
   if (threadIdx.x == 0) a.ps_c[tile] = d2;
 }
 
-// Gather index and CSR slot of staging position q: a packed word holds (local index << 12 |
-// slot), relative to the tile's base; otherwise separate index and 16-bit slot arrays.
+// Gather index of staging position q: a packed word holds (local index << 12 | CSR slot),
+// relative to the tile's base; otherwise separate index and 16-bit slot arrays.
 template <bool PACK>
 __device__ __forceinline__ int gather_index(const Tile& t, const int32_t* __restrict__ sidx, int q) {
   return PACK ? t.base + static_cast<int>(static_cast<uint32_t>(sidx[q]) >> kPackShift) : sidx[q];
 }
-template <bool PACK>
-__device__ __forceinline__ int staging_slot(const int32_t* __restrict__ sidx,
-                                            const uint16_t* __restrict__ pos, int q) {
-  return PACK ? static_cast<int>(static_cast<uint32_t>(sidx[q]) & 0xFFFu) : static_cast<int>(pos[q]);
+
+// Four consecutive elements from a 16-byte aligned address (two 16-byte loads for FP64).
+__device__ __forceinline__ void load4(const float* __restrict__ p, float v[4]) {
+  const float4 w = *reinterpret_cast<const float4*>(p);
+  v[0] = w.x, v[1] = w.y, v[2] = w.z, v[3] = w.w;
+}
+__device__ __forceinline__ void load4(const double* __restrict__ p, double v[4]) {
+  const double2 w0 = *reinterpret_cast<const double2*>(p);
+  const double2 w1 = *reinterpret_cast<const double2*>(p + 2);
+  v[0] = w0.x, v[1] = w0.y, v[2] = w1.x, v[3] = w1.y;
+}
+__device__ __forceinline__ void load4(const int32_t* __restrict__ p, int v[4]) {
+  const int4 w = *reinterpret_cast<const int4*>(p);
+  v[0] = w.x, v[1] = w.y, v[2] = w.z, v[3] = w.w;
+}
+__device__ __forceinline__ void load4(const uint16_t* __restrict__ p, int v[4]) {
+  const ushort4 w = *reinterpret_cast<const ushort4*>(p);
+  v[0] = w.x, v[1] = w.y, v[2] = w.z, v[3] = w.w;
+}
+
+// Stager of rowblock_product over the staging arrays, in groups of four stored positions: each
+// lane loads four index words (and, with HAS_VAL, four staged values) with one 16-byte load per
+// array, so a wavefront reads 1 KB of each contiguously, then stages f(c, slot, v) for each of
+// the four entries (c its gather index, slot its CSR-order slot, v its staged value). The
+// stored order (group_rank) makes the e-th gathers of a wavefront's lanes consecutive sorted
+// indices. Groups start at multiples of four, so a tile's first and last groups also cover
+// positions of the neighbouring tiles, which are loaded and ignored; the staging arrays hold
+// 16-byte aligned storage for at least kStagePad more elements than nonzeros, so the last group
+// of the last tile stays inside them.
+template <int NT, class A, bool PACK, bool HAS_VAL, class V, class F>
+struct GroupStager {
+  const int32_t* __restrict__ sidx;
+  const uint16_t* __restrict__ pos;
+  const V* __restrict__ val;
+  F f;
+  __device__ __forceinline__ void operator()(const Tile& t, A* stage) const {
+    const int p0 = t.p0, p1 = t.p1, qa = p0 & ~3;
+    const int groups = (p1 - qa + 3) >> 2;
+    constexpr int U = NT >= kBlock ? 2 : 4;  // groups per lane in flight
+    for (int g0 = 0; g0 < groups; g0 += U * NT) {
+      int w[U][4], sl[U][4];
+      V v[U][4];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int g = g0 + u * NT + static_cast<int>(threadIdx.x);
+        if (g < groups) {
+          load4(sidx + qa + 4 * g, w[u]);
+          if (HAS_VAL) load4(val + qa + 4 * g, v[u]);
+          if (!PACK) load4(pos + qa + 4 * g, sl[u]);
+        }
+      }
+      A prod[U][4];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int g = g0 + u * NT + static_cast<int>(threadIdx.x);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const int q = qa + 4 * g + e;
+          if (g < groups && q >= p0 && q < p1) {
+            const uint32_t word = static_cast<uint32_t>(w[u][e]);
+            const int c = PACK ? t.base + static_cast<int>(word >> kPackShift) : w[u][e];
+            if (PACK) sl[u][e] = static_cast<int>(word & 0xFFFu);
+            prod[u][e] = f(c, sl[u][e], v[u][e]);
+          }
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int g = g0 + u * NT + static_cast<int>(threadIdx.x);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const int q = qa + 4 * g + e;
+          if (g < groups && q >= p0 && q < p1) stage[sl[u][e]] = prod[u][e];
+        }
+      }
+    }
+  }
+};
+// stages val[q] * x[c] (products in A) with NT lanes
+template <int NT, class A, bool PACK, class V, class X>
+__device__ __forceinline__ auto gather_stager(const int32_t* sidx, const uint16_t* pos,
+                                              const V* val, const X* x) {
+  auto f = [x](int c, int, V v) { return static_cast<A>(v) * static_cast<A>(x[c]); };
+  return GroupStager<NT, A, PACK, true, V, decltype(f)>{sidx, pos, val, f};
+}
+// stages f(c, slot) (no staged values) with NT lanes
+template <int NT, class A, bool PACK, class F>
+__device__ __forceinline__ auto slot_stager(const int32_t* sidx, const uint16_t* pos, F g) {
+  auto f = [g](int c, int slot, int) { return g(c, slot); };
+  return GroupStager<NT, A, PACK, false, int, decltype(f)>{sidx, pos, nullptr, f};
 }
 
 // CAP: the tile capacity the tables were built for, which sizes the LDS staging buffer (and
 // with it how many workgroups a CU holds). PACK: index words are packed (see gather_index).
+// kStepLanes: lanes per workgroup of the two sparse step kernels (one tile per workgroup);
+// kStepRows: the rows each lane finalises.
+constexpr int kStepLanes = kBlock;
+constexpr int kStepRows = kBlock / kStepLanes;
+
 template <class T, int CAP, bool PACK>
-__global__ __launch_bounds__(kBlock) void k_step_dual(Args<T> a) {
+__global__ __launch_bounds__(kStepLanes) void k_step_dual(Args<T> a) {
   __shared__ T stage[CAP];
-  __shared__ double red[kBlock / kWave];
-  {
-    const int tile = xcd_tile(blockIdx.x, gridDim.x);
-    const Tile t = a.ta[tile];
-    const uint32_t f = a.fa[tile];
-    if (!a.force && !(f & kFlagStepping)) return;
-    const int p = (f & kFlagParity) ? 1 : 0;
-    const bool acc = a.force || (f & kFlagAccept);
-    const InstState& s = a.state[t.inst];
-    const T sigma = static_cast<T>(s.eta * s.omega);
-    const T* __restrict__ y = p ? a.y1 : a.y0;
-    T* __restrict__ yn = p ? a.y0 : a.y1;
-    const T* __restrict__ xbar = a.xbar;
-    const T* __restrict__ val = a.a_val;
-    const int32_t* __restrict__ idx = a.a_sidx;
-    double d2 = 0.0;
-    T yi = 0, lo = 0, hi = 0;
-    double sum = 0.0;
-    const uint16_t* __restrict__ pos = a.a_pos;
-    rowblock_product<T>(
-        t, a.a_ptr, [&](int q) { return val[q] * xbar[gather_index<PACK>(t, idx, q)]; },
-        [&](int q, int) { return staging_slot<PACK>(idx, pos, q); },
-        [&](int i) {
-          yi = y[i];
-          lo = a.rlhat[i];
-          hi = a.ruhat[i];
-          if (acc) sum = a.ysum[i];
-        },
-        [&](int i, T r) {
-          if (acc) a.ysum[i] = sum + static_cast<double>(yi);
-          const T v = yi + sigma * r;
-          const T ynew = v - sigma * dev_project<T>(v / sigma, lo, hi);
-          yn[i] = ynew;
-          const double d = static_cast<double>(ynew - yi);
-          d2 += d * d;
-        },
-        stage);
-    d2 = block_sum(d2, red);
-    if (threadIdx.x == 0) a.ps_a[tile] = d2;
-  }
+  __shared__ double red[kStepLanes / kWave];
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const Tile t = a.ta[tile];
+  const uint32_t f = a.fa[tile];
+  if (!a.force && !(f & kFlagStepping)) return;
+  const int p = (f & kFlagParity) ? 1 : 0;
+  const bool acc = a.force || (f & kFlagAccept);
+  const InstState& s = a.state[t.inst];
+  const T sigma = static_cast<T>(s.eta * s.omega);
+  const T* __restrict__ y = p ? a.y1 : a.y0;
+  T* __restrict__ yn = p ? a.y0 : a.y1;
+  const T* __restrict__ xbar = a.xbar;
+  const T* __restrict__ val = a.a_val;
+  const int32_t* __restrict__ idx = a.a_sidx;
+  double d2 = 0.0;
+  T yi[kStepRows], lo[kStepRows], hi[kStepRows];
+  double sum[kStepRows];
+  rowblock_product<kStepLanes, T>(
+      t, a.a_ptr, [&](int q) { return val[q] * xbar[gather_index<PACK>(t, idx, q)]; },
+      gather_stager<kStepLanes, T, PACK>(idx, a.a_pos, val, xbar),
+      [&](int i, int k) {
+        yi[k] = y[i];
+        lo[k] = a.rlhat[i];
+        hi[k] = a.ruhat[i];
+        if (acc) sum[k] = a.ysum[i];
+      },
+      [&](int i, int k, T r) {
+        if (acc) a.ysum[i] = sum[k] + static_cast<double>(yi[k]);
+        const T v = yi[k] + sigma * r;
+        const T ynew = v - sigma * dev_project<T>(v / sigma, lo[k], hi[k]);
+        yn[i] = ynew;
+        const double d = static_cast<double>(ynew - yi[k]);
+        d2 += d * d;
+      },
+      stage);
+  d2 = block_sum(d2, red);
+  if (threadIdx.x == 0) a.ps_a[tile] = d2;
 }
 
 // Reduce `count` per-tile partials starting at tile `base` with the whole workgroup in a
@@ -295,41 +383,38 @@ __device__ void decide_step(InstState& s, double dx2, double dy2, double inter_s
 }
 
 template <class T, int CAP, bool PACK>
-__global__ __launch_bounds__(kBlock) void k_step_aty(Args<T> a) {
+__global__ __launch_bounds__(kStepLanes) void k_step_aty(Args<T> a) {
   __shared__ T stage[CAP];
-  __shared__ double red[kBlock / kWave];
-  {
-    const int tile = xcd_tile(blockIdx.x, gridDim.x);
-    const Tile t = a.tt[tile];
-    const uint32_t f = a.ft[tile];
-    if (!a.force && !(f & kFlagStepping)) return;
-    const int p = (f & kFlagParity) ? 1 : 0;
-    const T* __restrict__ x = p ? a.x1 : a.x0;
-    const T* __restrict__ xn = p ? a.x0 : a.x1;
-    const T* __restrict__ yn = p ? a.y0 : a.y1;
-    const T* __restrict__ aty = p ? a.aty1 : a.aty0;
-    T* __restrict__ atyn = p ? a.aty0 : a.aty1;
-    const T* __restrict__ val = a.t_val;
-    const int32_t* __restrict__ idx = a.t_sidx;
-    double inter = 0.0;
-    T xj = 0, xnj = 0, atyj = 0;
-    const uint16_t* __restrict__ pos = a.t_pos;
-    rowblock_product<T>(
-        t, a.t_ptr, [&](int q) { return val[q] * yn[gather_index<PACK>(t, idx, q)]; },
-        [&](int q, int) { return staging_slot<PACK>(idx, pos, q); },
-        [&](int j) {
-          xj = x[j];
-          xnj = xn[j];
-          atyj = aty[j];
-        },
-        [&](int j, T r) {
-          atyn[j] = r;
-          inter += static_cast<double>(xnj - xj) * static_cast<double>(r - atyj);
-        },
-        stage);
-    inter = block_sum(inter, red);
-    if (threadIdx.x == 0) a.ps_t[tile] = inter;
-  }
+  __shared__ double red[kStepLanes / kWave];
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const Tile t = a.tt[tile];
+  const uint32_t f = a.ft[tile];
+  if (!a.force && !(f & kFlagStepping)) return;
+  const int p = (f & kFlagParity) ? 1 : 0;
+  const T* __restrict__ x = p ? a.x1 : a.x0;
+  const T* __restrict__ xn = p ? a.x0 : a.x1;
+  const T* __restrict__ yn = p ? a.y0 : a.y1;
+  const T* __restrict__ aty = p ? a.aty1 : a.aty0;
+  T* __restrict__ atyn = p ? a.aty0 : a.aty1;
+  const T* __restrict__ val = a.t_val;
+  const int32_t* __restrict__ idx = a.t_sidx;
+  double inter = 0.0;
+  T xj[kStepRows], xnj[kStepRows], atyj[kStepRows];
+  rowblock_product<kStepLanes, T>(
+      t, a.t_ptr, [&](int q) { return val[q] * yn[gather_index<PACK>(t, idx, q)]; },
+      gather_stager<kStepLanes, T, PACK>(idx, a.t_pos, val, yn),
+      [&](int j, int k) {
+        xj[k] = x[j];
+        xnj[k] = xn[j];
+        atyj[k] = aty[j];
+      },
+      [&](int j, int k, T r) {
+        atyn[j] = r;
+        inter += static_cast<double>(xnj[k] - xj[k]) * static_cast<double>(r - atyj[k]);
+      },
+      stage);
+  inter = block_sum(inter, red);
+  if (threadIdx.x == 0) a.ps_t[tile] = inter;
 }
 
 // One workgroup per instance: reduce the step partials in tile order and decide the step.
@@ -412,19 +497,23 @@ __global__ __launch_bounds__(kBlock) void k_eval_rows(Args<T> a) {
   const double* __restrict__ val = a.a_orig;
   const int32_t* __restrict__ sidx = a.a_sidx;
   const uint16_t* __restrict__ pos = a.a_pos;
-  // the step kernels' staging order; the FP64 values stay in CSR order (a vector tile's
-  // staging order is the CSR order)
-  const bool vector = t.kind == kTileVector;
+  // the step kernels' staging order; the FP64 values stay in CSR order, read through the slot
+  // map (a vector tile's staging order is its CSR order)
   double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  rowblock_product<D2>(
-      t, a.a_ptr,
-      [&](int q) {
-        const int c = gather_index<PACK>(t, sidx, q);
-        const double v = val[vector ? q : t.p0 + staging_slot<PACK>(sidx, pos, q)];
+  const auto prod = [&](int q) {
+    const int c = gather_index<PACK>(t, sidx, q);
+    const double v = val[q];
+    return D2{v * xa[c], v * xb[c]};
+  };
+  const double* __restrict__ v0 = val + t.p0;
+  rowblock_product<kBlock, D2>(
+      t, a.a_ptr, prod,
+      slot_stager<kBlock, D2, PACK>(sidx, pos, [=](int c, int slot) {
+        const double v = v0[slot];
         return D2{v * xa[c], v * xb[c]};
-      },
-      [&](int q, int) { return staging_slot<PACK>(sidx, pos, q); }, [](int) {},
-      [&](int i, D2 act) {
+      }),
+      [](int, int) {},
+      [&](int i, int, D2 act) {
         const double lo = a.rl[i], hi = a.ru[i];
         const bool lo_f = dev_finite_bound(lo), hi_f = dev_finite_bound(hi);
         if (a.report) a.activity[i] = act.a;
@@ -558,17 +647,23 @@ __global__ __launch_bounds__(kBlock) void k_eval_cols(Args<T> a) {
   const double* __restrict__ val = a.t_orig;
   const int32_t* __restrict__ sidx = a.t_sidx;
   const uint16_t* __restrict__ pos = a.t_pos;
-  const bool vector = t.kind == kTileVector;
+  // the FP64 values stay in CSR order, read through the slot map (a vector tile's staging
+  // order is its CSR order)
   double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  rowblock_product<D2>(
-      t, a.t_ptr,
-      [&](int q) {
-        const int r = gather_index<PACK>(t, sidx, q);
-        const double v = val[vector ? q : t.p0 + staging_slot<PACK>(sidx, pos, q)];
+  const auto prod = [&](int q) {
+    const int r = gather_index<PACK>(t, sidx, q);
+    const double v = val[q];
+    return D2{v * ya[r], v * yb[r]};
+  };
+  const double* __restrict__ v0 = val + t.p0;
+  rowblock_product<kBlock, D2>(
+      t, a.t_ptr, prod,
+      slot_stager<kBlock, D2, PACK>(sidx, pos, [=](int r, int slot) {
+        const double v = v0[slot];
         return D2{v * ya[r], v * yb[r]};
-      },
-      [&](int q, int) { return staging_slot<PACK>(sidx, pos, q); }, [](int) {},
-      [&](int j, D2 dot) {
+      }),
+      [](int, int) {},
+      [&](int j, int, D2 dot) {
         const double lo = a.cl[j], hi = a.cu[j];
         const bool lo_f = dev_finite_bound(lo), hi_f = dev_finite_bound(hi);
         const double cj = a.cmin[j];
@@ -769,11 +864,30 @@ __global__ __launch_bounds__(kBlock) void k_convert(const double* __restrict__ s
   if (k < n) dst[k] = static_cast<T>(src[k]);
 }
 
+// Rank, in gather-index order, of the entry stored at position j (from the tile's first
+// nonzero) of a stream tile of nnz entries that starts o = p0 % 4 positions past a multiple of
+// four. GroupStager has lane l of a wavefront load the four stored entries 4l .. 4l+3 of a
+// 256-entry block (blocks aligned to p0 - o) and gather entry 4l+e with its e-th gather
+// instruction; storing the rank order down the lanes (entry 4l+e of a block holds the block's
+// (64e + l)-th entry, counting only the tile's own positions) makes each gather instruction's
+// 64 lanes read 64 consecutive sorted indices. A bijection of [0, nnz) for every o and nnz.
+__device__ __forceinline__ int group_rank(int j, int o, int nnz) {
+  const int jj = j + o, blk = jj >> 8, l = (jj & 255) >> 2, e = jj & 3;
+  const int lo = blk == 0 ? o : 0;                 // the block's first own position
+  const int hi = min(256, o + nnz - 256 * blk);    // one past its last
+  // lanes l' >= 0 with 4 l' + e' < x
+  auto below = [](int x, int ep) { return max(0, (x - ep + 3) >> 2); };
+  int r = blk == 0 ? 0 : 256 * blk - o;
+  for (int ep = 0; ep < e; ++ep) r += below(hi, ep) - below(lo, ep);
+  return r + l - below(lo, e);
+}
+
 // Staging order of every row-block tile (structure only; run once per upload). A stream tile's
 // nonzeros are sorted by gather index (column of A for CSR(A), row of A for CSR(A^T)), ties
-// broken by CSR position, with a bitonic sort of (index << 16 | slot) keys in LDS; sidx gets
-// the sorted indices and pos the CSR-order slot of each. The keys are distinct, so the order
-// is unique. With `sorted` = 0, or for a vector tile, the staging order is the CSR order.
+// broken by CSR position, with a bitonic sort of (index << 16 | slot) keys in LDS, and stored
+// in GroupStager's lane order (group_rank); sidx gets the indices and pos the CSR-order slot of
+// each. The keys are distinct, so the order is unique. With `sorted` = 0, or for a vector tile,
+// the staging order is the CSR order.
 __global__ __launch_bounds__(kBlock) void k_sort_tiles(const Tile* __restrict__ tiles,
                                                         const int32_t* __restrict__ ptr,
                                                         const int32_t* __restrict__ idx,
@@ -821,8 +935,11 @@ __global__ __launch_bounds__(kBlock) void k_sort_tiles(const Tile* __restrict__ 
       __syncthreads();
     }
   }
-  for (int k = tid; k < nnz; k += kBlock)
-    put(p0 + k, static_cast<int32_t>(keys[k] >> 16), static_cast<int>(keys[k] & 0xFFFFu));
+  const int o = p0 & 3;
+  for (int j = tid; j < nnz; j += kBlock) {
+    const uint64_t key = keys[group_rank(j, o, nnz)];
+    put(p0 + j, static_cast<int32_t>(key >> 16), static_cast<int>(key & 0xFFFFu));
+  }
 }
 
 // The scaled values in each tile's staging order, in the iterate precision (a vector tile keeps
@@ -862,11 +979,11 @@ __global__ __launch_bounds__(kBlock) void k_spmv_tiles(const Tile* __restrict__ 
                                                         double* __restrict__ y) {
   __shared__ double stage[kMaxTileNnz];
   const Tile t = tiles[xcd_tile(blockIdx.x, gridDim.x)];
-  rowblock_product<double>(
+  rowblock_product<kBlock, double>(
       t, ptr,
       [&](int q) { return static_cast<double>(val[q]) * x[gather_index<PACK>(t, sidx, q)]; },
-      [&](int q, int) { return staging_slot<PACK>(sidx, pos, q); }, [](int) {},
-      [&](int r, double acc) { y[r] = acc; }, stage);
+      gather_stager<kBlock, double, PACK>(sidx, pos, val, x), [](int, int) {},
+      [&](int r, int, double acc) { y[r] = acc; }, stage);
 }
 
 // Per column chunk: sum of v^2 (slot 0 of pe_c).

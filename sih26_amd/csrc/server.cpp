@@ -58,7 +58,7 @@ int Server::open() {
   if (opt_.memory_budget <= 0) opt_.memory_budget = static_cast<int64_t>(0.9 * static_cast<double>(free_bytes));
   stats_.memory_budget = opt_.memory_budget;
   slots_.resize(static_cast<size_t>(opt_.slots));
-  for (auto& s : slots_) s.engine = make_engine(opt_.engine);
+  for (auto& s : slots_) s.engine = make_single_engine(opt_.engine);
   for (int i = 0; i < opt_.slots; ++i) threads_.emplace_back(&Server::run_slot, this, i);
   return kStatusSuccess;
 }

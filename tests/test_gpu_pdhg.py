@@ -152,3 +152,28 @@ def test_batch_resolve_is_reproducible(gpu):
     b.solve()
     for k in range(3):
         np.testing.assert_array_equal(first[k], b.result(k).col_value)
+
+
+def test_split_batch_matches_one_stream(gpu):
+    """A batch run as three sub-batches on their own streams gives every instance bitwise the
+    answer of the one-stream batch, and reports results in batch order."""
+    lps = [gpu.Lp.generate(3, k) for k in range(30, 40)]
+    kw = dict(pdhg_tolerance=1e-4, pdhg_stop_at_request=True, pdhg_absolute_primal=False)
+    one = gpu.Batch(streams=1, **kw)
+    split = gpu.Batch(streams=3, **kw)
+    for b in (one, split):
+        for lp in lps:
+            b.add(lp)
+        b.upload()
+        b.solve()
+    assert one.stats()["streams"] == 1 and split.stats()["streams"] == 3
+    assert split.stats()["total_nnz"] == one.stats()["total_nnz"]
+    for k in range(len(lps)):
+        a, c = one.result(k), split.result(k)
+        assert a.iterations == c.iterations and a.status == c.status
+        np.testing.assert_array_equal(a.col_value, c.col_value)
+        np.testing.assert_array_equal(a.row_dual, c.row_dual)
+    # kernel timing is defined on one block-diagonal launch only
+    with pytest.raises(gpu.SkError):
+        split.time_kernel(1, 2)
+    assert one.time_kernel(1, 2)[0] > 0
