@@ -62,15 +62,44 @@ __device__ __forceinline__ double block_sum(double v, double* lds) {
   return r;
 }
 
-__device__ __forceinline__ double block_max(double v, double* lds) {
+// N workgroup reductions at once: sums (slots whose bit is clear in `max_mask`), each bit for
+// bit a block_sum, and maxima (bit set: the same butterfly and wavefront order with max).
+// Every wavefront reduces all N values, then one LDS exchange and one barrier instead of two
+// barriers per value. The results are valid in every thread. `lds` holds N * (kBlock / kWave)
+// doubles.
+template <int N, uint32_t MAX_MASK>
+__device__ __forceinline__ void block_reduce_n(double (&v)[N], double* lds) {
+  constexpr uint32_t max_mask = MAX_MASK;
+  constexpr int W = kBlock / kWave;
   const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
-  for (int o = kWave >> 1; o >= 1; o >>= 1) v = dev_max(v, shfl_xor(v, o));
-  if (lane == 0) lds[wave] = v;
+#pragma unroll
+  for (int o = kWave >> 1; o >= 1; o >>= 1) {
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+      const double u = shfl_xor(v[i], o);
+      v[i] = ((max_mask >> i) & 1u) ? dev_max(v[i], u) : v[i] + u;
+    }
+  }
+  if (lane == 0) {
+#pragma unroll
+    for (int i = 0; i < N; ++i) lds[i * W + wave] = v[i];
+  }
   __syncthreads();
-  double r = lds[0];
-  for (int w = 1; w < kBlock / kWave; ++w) r = dev_max(r, lds[w]);
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    if ((max_mask >> i) & 1u) {
+      double r = lds[i * W];
+#pragma unroll
+      for (int w = 1; w < W; ++w) r = dev_max(r, lds[i * W + w]);
+      v[i] = r;
+    } else {
+      double r = 0.0;
+#pragma unroll
+      for (int w = 0; w < W; ++w) r += lds[i * W + w];
+      v[i] = r;
+    }
+  }
   __syncthreads();
-  return r;
 }
 
 // Workgroups are dispatched round-robin over the 8 XCDs of the chip (workgroup b runs on

@@ -216,7 +216,7 @@ struct GroupStager {
   __device__ __forceinline__ void operator()(const Tile& t, A* stage) const {
     const int p0 = t.p0, p1 = t.p1, qa = p0 & ~3;
     const int groups = (p1 - qa + 3) >> 2;
-    constexpr int U = NT >= kBlock ? 2 : 4;  // groups per lane in flight
+    constexpr int U = NT >= kBlock ? 1 : 4;  // groups per lane in flight (measured best)
     for (int g0 = 0; g0 < groups; g0 += U * NT) {
       int w[U][4], sl[U][4];
       V v[U][4];
@@ -327,13 +327,6 @@ __device__ __forceinline__ double tile_sum(const double* part, int base, int cou
   for (int k = threadIdx.x; k < count; k += kBlock) v += part[(size_t)(base + k) * stride + slot];
   return block_sum(v, red);
 }
-__device__ __forceinline__ double tile_max(const double* part, int base, int count, int stride,
-                                           int slot, double* red) {
-  double v = 0.0;
-  for (int k = threadIdx.x; k < count; k += kBlock)
-    v = dev_max(v, part[(size_t)(base + k) * stride + slot]);
-  return block_max(v, red);
-}
 
 __device__ __forceinline__ void finish(InstState& s, int status, uint32_t* done_count) {
   s.done = 1;
@@ -422,16 +415,19 @@ __global__ __launch_bounds__(kStepLanes) void k_step_aty(Args<T> a) {
 // XCDs through the kernel boundary, with no per-tile device-scope fence.
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_step_finalize(Args<T> a) {
-  __shared__ double red[kBlock / kWave];
+  __shared__ double red[3 * (kBlock / kWave)];
   InstState& s = a.state[blockIdx.x];
   if (!stepping(s)) return;
   const InstDesc d = a.desc[blockIdx.x];
-  const double dx2 = tile_sum(a.ps_c, d.tc0, d.tc1 - d.tc0, 1, 0, red);
-  const double dy2 = tile_sum(a.ps_a, d.ta0, d.ta1 - d.ta0, 1, 0, red);
-  const double isum = tile_sum(a.ps_t, d.tt0, d.tt1 - d.tt0, 1, 0, red);
+  // ||x'-x||^2, ||y'-y||^2 and the interaction: lane sums over the tiles, then one reduction
+  double v[3] = {0.0, 0.0, 0.0};
+  for (int k = threadIdx.x; k < d.tc1 - d.tc0; k += kBlock) v[0] += a.ps_c[d.tc0 + k];
+  for (int k = threadIdx.x; k < d.ta1 - d.ta0; k += kBlock) v[1] += a.ps_a[d.ta0 + k];
+  for (int k = threadIdx.x; k < d.tt1 - d.tt0; k += kBlock) v[2] += a.ps_t[d.tt0 + k];
+  block_reduce_n<3, 0u>(v, red);
   __shared__ uint32_t flags;
   if (threadIdx.x == 0) {
-    decide_step(s, dx2, dy2, isum, d.rows > 0, a.done_count);
+    decide_step(s, v[0], v[1], v[2], d.rows > 0, a.done_count);
     flags = step_flags(s);
   }
   __syncthreads();
@@ -485,7 +481,7 @@ __global__ __launch_bounds__(kBlock) void k_eval_prepare(Args<T> a) {
 template <class T, int CAP, bool PACK>
 __global__ __launch_bounds__(kBlock) void k_eval_rows(Args<T> a) {
   __shared__ D2 stage[CAP];
-  __shared__ double red[kBlock / kWave];
+  __shared__ double red[8 * (kBlock / kWave)];
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
   const Tile t = a.ta[tile];
   const InstState& s = a.state[t.inst];
@@ -539,9 +535,10 @@ __global__ __launch_bounds__(kBlock) void k_eval_rows(Args<T> a) {
         }
       },
       stage);
-  for (int k = 0; k < 8; ++k) {
-    const double r = (k & 3) == 3 ? block_max(acc[k], red) : block_sum(acc[k], red);
-    if (threadIdx.x == 0) a.pe_a[(size_t)tile * 8 + k] = r;
+  block_reduce_n<8, (1u << 3) | (1u << 7)>(acc, red);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < 8; ++k) a.pe_a[(size_t)tile * 8 + k] = acc[k];
   }
 }
 
@@ -635,7 +632,7 @@ __device__ void decide_evaluation(InstState& s, const Residuals& cur, const Resi
 template <class T, int CAP, bool PACK>
 __global__ __launch_bounds__(kBlock) void k_eval_cols(Args<T> a) {
   __shared__ D2 stage[CAP];
-  __shared__ double red[kBlock / kWave];
+  __shared__ double red[8 * (kBlock / kWave)];
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
   const Tile t = a.tt[tile];
   const InstState& s = a.state[t.inst];
@@ -687,34 +684,52 @@ __global__ __launch_bounds__(kBlock) void k_eval_cols(Args<T> a) {
         }
       },
       stage);
-  for (int k = 0; k < 8; ++k) {
-    const double r = (k & 3) == 2 ? block_max(acc[k], red) : block_sum(acc[k], red);
-    if (threadIdx.x == 0) a.pe_t[(size_t)tile * 8 + k] = r;
+  block_reduce_n<8, (1u << 2) | (1u << 6)>(acc, red);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < 8; ++k) a.pe_t[(size_t)tile * 8 + k] = acc[k];
   }
-
 }
 
 // One workgroup per evaluated instance: reduce every evaluation partial in tile order, form
 // both Residuals and decide (or store the report).
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_eval_finalize(Args<T> a) {
-  __shared__ double red[kBlock / kWave];
+  __shared__ double red[18 * (kBlock / kWave)];
   InstState& s = a.state[blockIdx.x];
   if (!(a.report ? s.report_pending : s.eval_pending)) return;
   const InstDesc d = a.desc[blockIdx.x];
+  // per candidate c, v[8c + ...]: 0 viol^2, 1 support, 2 dual violation (rows), 3 compl (rows),
+  // 4 dual violation (cols), 5 bound, 6 compl (cols), 7 primal objective; 16, 17 the squared
+  // distances to the restart point. Lane sums (maxima) over the tiles, then one reduction.
   double v[18];
-  for (int c = 0; c < 2; ++c) {
-    v[8 * c + 0] = tile_sum(a.pe_a, d.ta0, d.ta1 - d.ta0, 8, 4 * c + 0, red);  // viol^2
-    v[8 * c + 1] = tile_sum(a.pe_a, d.ta0, d.ta1 - d.ta0, 8, 4 * c + 1, red);  // support
-    v[8 * c + 2] = tile_sum(a.pe_a, d.ta0, d.ta1 - d.ta0, 8, 4 * c + 2, red);  // dviol (rows)
-    v[8 * c + 3] = tile_max(a.pe_a, d.ta0, d.ta1 - d.ta0, 8, 4 * c + 3, red);  // compl (rows)
-    v[8 * c + 4] = tile_sum(a.pe_t, d.tt0, d.tt1 - d.tt0, 8, 4 * c + 0, red);  // dviol (cols)
-    v[8 * c + 5] = tile_sum(a.pe_t, d.tt0, d.tt1 - d.tt0, 8, 4 * c + 1, red);  // bound
-    v[8 * c + 6] = tile_max(a.pe_t, d.tt0, d.tt1 - d.tt0, 8, 4 * c + 2, red);  // compl (cols)
-    v[8 * c + 7] = tile_sum(a.pe_t, d.tt0, d.tt1 - d.tt0, 8, 4 * c + 3, red);  // pobj
+#pragma unroll
+  for (int i = 0; i < 18; ++i) v[i] = 0.0;
+  for (int k = threadIdx.x; k < d.ta1 - d.ta0; k += kBlock) {
+    const double* q = a.pe_a + (size_t)(d.ta0 + k) * 8;
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      v[8 * c + 0] += q[4 * c + 0];
+      v[8 * c + 1] += q[4 * c + 1];
+      v[8 * c + 2] += q[4 * c + 2];
+      v[8 * c + 3] = dev_max(v[8 * c + 3], q[4 * c + 3]);
+    }
   }
-  v[16] = a.report ? 0.0 : tile_sum(a.pe_c, d.tc0, d.tc1 - d.tc0, 2, 0, red);
-  v[17] = a.report ? 0.0 : tile_sum(a.pe_r, d.tr0, d.tr1 - d.tr0, 2, 0, red);
+  for (int k = threadIdx.x; k < d.tt1 - d.tt0; k += kBlock) {
+    const double* q = a.pe_t + (size_t)(d.tt0 + k) * 8;
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      v[8 * c + 4] += q[4 * c + 0];
+      v[8 * c + 5] += q[4 * c + 1];
+      v[8 * c + 6] = dev_max(v[8 * c + 6], q[4 * c + 2]);
+      v[8 * c + 7] += q[4 * c + 3];
+    }
+  }
+  if (!a.report) {
+    for (int k = threadIdx.x; k < d.tc1 - d.tc0; k += kBlock) v[16] += a.pe_c[(size_t)(d.tc0 + k) * 2];
+    for (int k = threadIdx.x; k < d.tr1 - d.tr0; k += kBlock) v[17] += a.pe_r[(size_t)(d.tr0 + k) * 2];
+  }
+  block_reduce_n<18, (1u << 3) | (1u << 6) | (1u << 11) | (1u << 14)>(v, red);
   __shared__ uint32_t flags;
   if (threadIdx.x == 0) {
     Residuals r[2];
