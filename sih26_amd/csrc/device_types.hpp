@@ -15,6 +15,9 @@ constexpr int kChunk = 1024;
 // Largest nonzero count a row-block tile may stage in LDS; a single row above it is
 // processed by the whole workgroup without staging ("vector" tile).
 constexpr int kMaxTileNnz = 4096;
+// Most rows a row-block tile holds: a short-row tile gives each lane of a 256-lane workgroup
+// up to two rows, a long-row tile each wavefront up to two.
+constexpr int kMaxTileRows = 2 * kBlock;
 // Extra elements allocated after the staging-order matrix arrays: the step kernels load them
 // in aligned groups of four, and the last group may reach three elements past the end.
 constexpr int kStagePad = 4;
@@ -43,8 +46,8 @@ struct Tile {
 // 64 lanes of a wavefront, lane l taking nonzeros l, l+64, ... in order, then a butterfly.
 constexpr int kLongRow = 64;
 enum : int32_t {
-  kTileShort = 0,   // up to kBlock short rows, one lane each
-  kTileLong = 1,    // up to kBlock / kWave long rows, one wavefront each
+  kTileShort = 0,   // up to kMaxTileRows short rows, one lane each
+  kTileLong = 1,    // up to kMaxTileRows / kWave long rows, one wavefront each
   kTileVector = 2   // one row with more than `cap` nonzeros, read straight from memory
 };
 

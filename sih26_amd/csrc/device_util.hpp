@@ -121,8 +121,9 @@ constexpr int32_t kPackMaxLocal = 1 << 20;
 // ---- the row-block product ---------------------------------------------------------------
 // Computes acc_r = sum_p prod(p) for every row r of tile t of a CSR matrix with a workgroup of
 // NT lanes (NT = kWave or kBlock) and calls epi(r, k, acc_r) exactly once per row, on one
-// lane; that lane calls pre(r, k) first, before any product is loaded. k in [0, kBlock / NT)
-// numbers the rows a lane finalises, so the caller can keep per-row operands in registers.
+// lane; that lane calls pre(r, k) first, before any product is loaded. k in
+// [0, kMaxTileRows / NT) numbers the rows a lane finalises, so the caller can keep per-row
+// operands in registers.
 //
 // Stream tile (kTileShort, kTileLong): all lanes first write the tile's products into LDS with
 // coalesced loads (CSR-Stream; Greathouse & Daga, SC'14) through stager(t, stage), then each
@@ -139,8 +140,8 @@ template <int NT, class A, class Prod, class Stager, class Pre, class Epi>
 __device__ __forceinline__ void rowblock_product(const Tile t, const int32_t* __restrict__ ptr,
                                                  Prod prod, Stager stager, Pre pre, Epi epi,
                                                  A* stage) {
-  static_assert(NT % kWave == 0 && kBlock % NT == 0, "NT must be a multiple of the wavefront");
-  constexpr int R = kBlock / NT;  // short rows per lane; long rows per wavefront
+  static_assert(NT % kWave == 0 && kMaxTileRows % NT == 0, "NT must be a multiple of the wavefront");
+  constexpr int R = kMaxTileRows / NT;  // short rows per lane; long rows per wavefront
   constexpr int W = NT / kWave;   // wavefronts
   const int tid = threadIdx.x;
   const int r0 = t.begin, nrows = t.end - t.begin;

@@ -52,8 +52,8 @@ struct DevBuf {
 };
 
 // Row-block tiles over rows [row0, row0 + rows) of a CSR matrix. Short rows (at most kLongRow
-// nonzeros) share kTileShort tiles of at most kBlock rows; long rows share kTileLong tiles of
-// at most kBlock / kWave rows; either kind holds at most `cap` nonzeros, and a row with more
+// nonzeros) share kTileShort tiles of at most kMaxTileRows rows; long rows share kTileLong
+// tiles of at most kMaxTileRows / kWave rows; either kind holds at most `cap` nonzeros, and a row with more
 // than `cap` nonzeros is a kTileVector tile of its own. A row's sum does not depend on the
 // tiling (see kLongRow), so `cap` is free to tune. Every instance gets at least one tile.
 void build_row_tiles(const std::vector<int32_t>& ptr, int32_t row0, int32_t rows, int32_t inst,
@@ -75,7 +75,7 @@ void build_row_tiles(const std::vector<int32_t>& ptr, int32_t row0, int32_t rows
       continue;
     }
     const int32_t k = len > kLongRow ? kTileLong : kTileShort;
-    const int32_t max_rows = k == kTileLong ? kBlock / kWave : kBlock;
+    const int32_t max_rows = k == kTileLong ? kMaxTileRows / kWave : kMaxTileRows;
     if (start < r && (k != kind || acc + len > cap || r - start == max_rows)) {
       emit(start, r, kind);
       start = r;

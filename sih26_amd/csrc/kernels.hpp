@@ -275,7 +275,7 @@ __device__ __forceinline__ auto slot_stager(const int32_t* sidx, const uint16_t*
 // kStepLanes: lanes per workgroup of the two sparse step kernels (one tile per workgroup);
 // kStepRows: the rows each lane finalises.
 constexpr int kStepLanes = kBlock;
-constexpr int kStepRows = kBlock / kStepLanes;
+constexpr int kStepRows = kMaxTileRows / kStepLanes;
 
 template <class T, int CAP, bool PACK>
 __global__ __launch_bounds__(kStepLanes) void k_step_dual(Args<T> a) {
@@ -823,16 +823,16 @@ __global__ __launch_bounds__(kBlock) void k_apply_pass(const Tile* __restrict__ 
                                                         double* __restrict__ val, int32_t row_major,
                                                         const double* __restrict__ row_pass,
                                                         const double* __restrict__ col_pass) {
-  __shared__ uint8_t row_of[kMaxTileNnz];
+  __shared__ uint16_t row_of[kMaxTileNnz];
   const Tile t = tiles[blockIdx.x];
   const int nrows = t.end - t.begin, nnz = t.p1 - t.p0;
   if (nrows <= 0 || nnz <= 0) return;
   const bool one_row = nrows == 1;  // also every vector tile (more than kMaxTileNnz nonzeros)
   if (!one_row) {
-    // at most kBlock rows, so one lane per row and row numbers fit 8 bits
-    if (threadIdx.x < nrows) {
-      const int r = t.begin + threadIdx.x;
-      for (int p = ptr[r]; p < ptr[r + 1]; ++p) row_of[p - t.p0] = static_cast<uint8_t>(threadIdx.x);
+    // at most kMaxTileRows rows: lane l expands rows l, l + kBlock
+    for (int row = threadIdx.x; row < nrows; row += kBlock) {
+      const int r = t.begin + row;
+      for (int p = ptr[r]; p < ptr[r + 1]; ++p) row_of[p - t.p0] = static_cast<uint16_t>(row);
     }
     __syncthreads();
   }
