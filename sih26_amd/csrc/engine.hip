@@ -1077,9 +1077,9 @@ int EngineImpl<T>::time_kernel(int kernel, int reps, double* mean_ms) {
 
 // Algorithmic bytes per launch (each array element counted once):
 //   0 primal: per column read x, c, l, u, A^T y (5 words), write x', xbar (2 words),
-//             read-modify-write the FP64 running sum (16 B)
+//             read-modify-write the running sum (2 words)
 //   1 dual:   CSR(A): (w + 4) B per nonzero + 4 (m + 1) row pointers + w n for xbar;
-//             per row read y, rl, ru (3 words), write y' (1 word), sum RMW (16 B)
+//             per row read y, rl, ru (3 words), write y' (1 word), sum RMW (2 words)
 //   2 A^T:    CSR(A^T): (w + 4) B per nonzero + 4 (n + 1) + w m for y';
 //             per column read x, x', A^T y (3 words), write A^T y' (1 word)
 template <class T>
@@ -1087,8 +1087,8 @@ double EngineImpl<T>::kernel_bytes(int kernel) const {
   kernel %= 3;
   const double w = sizeof(T), z = static_cast<double>(nnz_);
   const double m = static_cast<double>(m_), n = static_cast<double>(n_);
-  if (kernel == 0) return n * (7.0 * w + 16.0);
-  if (kernel == 1) return (w + 4.0) * z + 4.0 * (m + 1.0) + w * n + m * (4.0 * w + 16.0);
+  if (kernel == 0) return n * 9.0 * w;
+  if (kernel == 1) return (w + 4.0) * z + 4.0 * (m + 1.0) + w * n + m * 6.0 * w;
   return (w + 4.0) * z + 4.0 * (n + 1.0) + w * m + n * 4.0 * w;
 }
 

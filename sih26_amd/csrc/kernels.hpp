@@ -78,8 +78,10 @@ struct Args {
   T* xbar;
   T* xrst;  // restart point
   T* yrst;
-  double* xsum;  // running sums since the last restart
-  double* ysum;
+  // running sums since the last restart, in the iterate precision (the average is an iterate
+  // quantity; FP64 is kept for the KKT, gap and objective reductions and the reported point)
+  T* xsum;
+  T* ysum;
   // unscaled candidates: *u0 current, *u1 running average; best; report outputs
   double* xu0;
   double* xu1;
@@ -160,7 +162,7 @@ __global__ __launch_bounds__(kBlock) void k_step_primal(Args<T> a) {
     const int j = t.begin + u * kBlock + threadIdx.x;
     if (j >= t.end) continue;
     const T xj = x[j];
-    if (acc) a.xsum[j] += static_cast<double>(xj);
+    if (acc) a.xsum[j] += xj;
     const T g = a.chat[j] + aty[j];
     const T v = dev_project<T>(xj - tau * g, a.lhat[j], a.uhat[j]);
     xn[j] = v;
@@ -295,8 +297,7 @@ __global__ __launch_bounds__(kStepLanes) void k_step_dual(Args<T> a) {
   const T* __restrict__ val = a.a_val;
   const int32_t* __restrict__ idx = a.a_sidx;
   double d2 = 0.0;
-  T yi[kStepRows], lo[kStepRows], hi[kStepRows];
-  double sum[kStepRows];
+  T yi[kStepRows], lo[kStepRows], hi[kStepRows], sum[kStepRows];
   rowblock_product<kStepLanes, T>(
       t, a.a_ptr, [&](int q) { return val[q] * xbar[gather_index<PACK>(t, idx, q)]; },
       gather_stager<kStepLanes, T, PACK>(idx, a.a_pos, val, xbar),
@@ -307,7 +308,7 @@ __global__ __launch_bounds__(kStepLanes) void k_step_dual(Args<T> a) {
         if (acc) sum[k] = a.ysum[i];
       },
       [&](int i, int k, T r) {
-        if (acc) a.ysum[i] = sum[k] + static_cast<double>(yi[k]);
+        if (acc) a.ysum[i] = sum[k] + yi[k];
         const T v = yi[k] + sigma * r;
         const T ynew = v - sigma * dev_project<T>(v / sigma, lo[k], hi[k]);
         yn[i] = ynew;
@@ -454,20 +455,20 @@ __global__ __launch_bounds__(kBlock) void k_eval_prepare(Args<T> a) {
   const bool have_avg = s.averaged > 0;
   const T* __restrict__ v = is_col ? (p ? a.x1 : a.x0) : (p ? a.y1 : a.y0);
   const T* __restrict__ rst = is_col ? a.xrst : a.yrst;
-  double* __restrict__ sum = is_col ? a.xsum : a.ysum;
+  T* __restrict__ sum = is_col ? a.xsum : a.ysum;
   const double* __restrict__ scale = is_col ? a.dc : a.dr;
   double* __restrict__ u0 = is_col ? a.xu0 : a.yu0;
   double* __restrict__ u1 = is_col ? a.xu1 : a.yu1;
   double d2 = 0.0;
   for (int k = t.begin + threadIdx.x; k < t.end; k += kBlock) {
     const T vk = v[k];
-    double total = sum[k];
+    T total = sum[k];
     if (acc) {
-      total += static_cast<double>(vk);
+      total += vk;
       sum[k] = total;
     }
     u0[k] = static_cast<double>(vk) * scale[k];
-    u1[k] = have_avg ? (total / count) * scale[k] : 0.0;
+    u1[k] = have_avg ? (static_cast<double>(total) / count) * scale[k] : 0.0;
     const double d = static_cast<double>(vk) - static_cast<double>(rst[k]);
     d2 += d * d;
   }
@@ -767,13 +768,13 @@ __global__ __launch_bounds__(kBlock) void k_eval_commit(Args<T> a) {
   const bool copy = s.copy_best, restart = s.do_restart, from_avg = s.chosen_avg;
   const T* __restrict__ v = is_col ? (p ? a.x1 : a.x0) : (p ? a.y1 : a.y0);
   T* __restrict__ rst = is_col ? a.xrst : a.yrst;
-  double* __restrict__ sum = is_col ? a.xsum : a.ysum;
+  T* __restrict__ sum = is_col ? a.xsum : a.ysum;
   const double* __restrict__ src = is_col ? (from_avg ? a.xu1 : a.xu0) : (from_avg ? a.yu1 : a.yu0);
   double* __restrict__ best = is_col ? a.bestx : a.besty;
   for (int k = t.begin + threadIdx.x; k < t.end; k += kBlock) {
     if (copy) best[k] = src[k];
     if (restart) {
-      sum[k] = 0.0;
+      sum[k] = static_cast<T>(0);
       rst[k] = v[k];
     }
   }
@@ -1090,7 +1091,7 @@ __global__ __launch_bounds__(kBlock) void k_init_iterates(Args<T> a, const doubl
     a.xbar[k] = x;
     a.aty0[k] = static_cast<T>(0);
     a.aty1[k] = static_cast<T>(0);
-    a.xsum[k] = 0.0;
+    a.xsum[k] = static_cast<T>(0);
     // pdhg.cpp:377: best starts as the scaled start point, in FP64
     double b = 0.0;
     if (dev_finite_bound(lhat64[k]) && b < lhat64[k]) b = lhat64[k];
@@ -1101,7 +1102,7 @@ __global__ __launch_bounds__(kBlock) void k_init_iterates(Args<T> a, const doubl
     a.y0[k] = static_cast<T>(0);
     a.y1[k] = static_cast<T>(0);
     a.yrst[k] = static_cast<T>(0);
-    a.ysum[k] = 0.0;
+    a.ysum[k] = static_cast<T>(0);
     a.besty[k] = 0.0;
   }
 }
