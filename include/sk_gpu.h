@@ -230,6 +230,7 @@ typedef struct {
   double relative_gap;
   char algorithm[16]; /* "pdhg-gpu" (verify.cpp:86-89 audits on "gpu") */
   char message[256];
+  int64_t polish_iterations; /* interior-point iterations of a polish (solve.cpp:225-242) */
 } sk_pdhg_result;
 
 /* The drop-in for pdhg::solve_pdhg (include/sankhya/pdhg.hpp:15; src/pdhg/pdhg.cpp:267-726)
@@ -251,6 +252,42 @@ int32_t sk_write_solution(const char* path, const sk_lp* lp, const sk_pdhg_resul
                           const double* col_dual, const double* row_activity,
                           const char* const* col_names, const char* const* row_names,
                           const char* model_name);
+
+/* ---- interior-point finish (src/ipm/ipm.cpp; src/core/solve.cpp:184-243) ------------------ */
+/* The options the polish reads (src/util/options.cpp:342-393 and 183). */
+typedef struct {
+  int64_t iteration_limit;     /* polish_iteration_limit, default 50 */
+  double start_margin;         /* polish_start_margin, default 0.1 */
+  double max_seconds;          /* polish_max_seconds, default 30 */
+  int64_t max_factor_nonzeros; /* polish_max_factor_nonzeros, default 5e7, -1 = no cap; the
+                                  factor here is dense: m (m + 1) / 2 nonzeros */
+  int32_t scaling;             /* scaling, default 1 */
+  int32_t reserved;
+} sk_ipm_options;
+
+void sk_ipm_default_options(sk_ipm_options* options);
+/* ipm::solve_ipm (include/sankhya/ipm.hpp:17, 33-34): Mehrotra's predictor-corrector on the
+ * normal equations, assembled and factorised (dense Cholesky, pivots floored at 1e-10) on the
+ * current HIP device. Started from the warm point when all three warm arrays are given (the
+ * WarmStart of ipm.hpp:21-31, in the original units and the Solution's signs), else cold.
+ * `result` gets the status (optimal / feasible / iteration_limit / time_limit / not_solved
+ * when declined / numerical_error), iterations, objective, dual bound and quality measures;
+ * the arrays are filled when the status claims a point. algorithm = "ipm". */
+int32_t sk_ipm_solve(const sk_lp* lp, const sk_ipm_options* options,
+                     const double* warm_col_value, const double* warm_row_dual,
+                     const double* warm_col_dual, sk_pdhg_result* result, double* col_value,
+                     double* row_dual, double* col_dual, double* row_activity);
+/* polish_with_the_interior_point (src/core/solve.cpp:184-243): `first` (a PDHG answer and its
+ * arrays) is handed to the interior point as its starting point unless it is already optimal
+ * or claims no point. The polished answer replaces it only when it is optimal, or feasible
+ * with smaller scaled violations; then iterations are the sum of both phases,
+ * polish_iterations the second, and algorithm reads "<first>+ipm". Otherwise `first` is
+ * returned with the polish's iteration count and a note in the message. */
+int32_t sk_pdhg_polish(const sk_lp* lp, const sk_ipm_options* options,
+                       const sk_pdhg_result* first, const double* first_col_value,
+                       const double* first_row_dual, const double* first_col_dual,
+                       sk_pdhg_result* result, double* col_value, double* row_dual,
+                       double* col_dual, double* row_activity);
 
 /* ---- batches: many LPs of uneven size in one block-diagonal solve -------------------------- */
 typedef struct sk_batch sk_batch;

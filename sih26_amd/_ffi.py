@@ -58,7 +58,12 @@ class PdhgResult(C.Structure):
                 ("dual_infeasibility", f64), ("dual_infeasibility_scaled", f64),
                 ("complementarity_violation", f64), ("absolute_gap", f64),
                 ("relative_gap", f64), ("algorithm", C.c_char * 16),
-                ("message", C.c_char * 256)]
+                ("message", C.c_char * 256), ("polish_iterations", i64)]
+
+
+class IpmOptions(C.Structure):
+    _fields_ = [("iteration_limit", i64), ("start_margin", f64), ("max_seconds", f64),
+                ("max_factor_nonzeros", i64), ("scaling", i32), ("reserved", i32)]
 
 
 class BatchStats(C.Structure):
@@ -116,6 +121,10 @@ SIGNATURES = {
     "sk_pdhg_set_option": (i32, [P(PdhgOptions), cstr, cstr]),
     "sk_pdhg_solve": (i32, [vp, P(PdhgOptions), P(PdhgResult), vp, vp, vp, vp]),
     "sk_write_solution": (i32, [cstr, vp, P(PdhgResult), vp, vp, vp, vp, vp, vp, cstr]),
+    "sk_ipm_default_options": (None, [P(IpmOptions)]),
+    "sk_ipm_solve": (i32, [vp, P(IpmOptions), vp, vp, vp, P(PdhgResult), vp, vp, vp, vp]),
+    "sk_pdhg_polish": (i32, [vp, P(IpmOptions), P(PdhgResult), vp, vp, vp, P(PdhgResult), vp, vp,
+                             vp, vp]),
     "sk_batch_create": (i32, [P(PdhgOptions), P(vp)]),
     "sk_batch_add": (i32, [vp, vp, P(i32)]),
     "sk_batch_upload": (i32, [vp]),

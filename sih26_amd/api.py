@@ -212,6 +212,43 @@ def solve(lp: Lp, options: _ffi.PdhgOptions | None = None, **kw) -> SolveResult:
     return _result(res, x, yd, d, act)
 
 
+def ipm_options(**kw) -> _ffi.IpmOptions:
+    """Interior-point options with the reference defaults; keywords are the struct fields
+    (iteration_limit, start_margin, max_seconds, max_factor_nonzeros, scaling)."""
+    o = _ffi.IpmOptions()
+    lib().sk_ipm_default_options(C.byref(o))
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+def solve_ipm(lp: Lp, warm: SolveResult | None = None, **kw) -> SolveResult:
+    """ipm::solve_ipm on one LP, warm-started from `warm`'s point when given."""
+    o = ipm_options(**kw)
+    m, n, _ = lp.dims
+    x, d, yd, act = np.zeros(n), np.zeros(n), np.zeros(m), np.zeros(m)
+    res = _ffi.PdhgResult()
+    w = (_ptr(warm.col_value), _ptr(warm.row_dual), _ptr(warm.col_dual)) if warm else (None,) * 3
+    check(lib().sk_ipm_solve(lp._h, C.byref(o), *w, C.byref(res), _ptr(x), _ptr(yd), _ptr(d),
+                             _ptr(act)), "sk_ipm_solve")
+    return _result(res, x, yd, d, act)
+
+
+def polish(lp: Lp, first: SolveResult, **kw) -> SolveResult:
+    """The reference's PDHG-then-interior-point finish of one answer (sk_pdhg_polish)."""
+    o = ipm_options(**kw)
+    m, n, _ = lp.dims
+    x, d, yd, act = np.zeros(n), np.zeros(n), np.zeros(m), np.zeros(m)
+    res = _ffi.PdhgResult()
+    fx = np.ascontiguousarray(first.col_value, np.float64)
+    fy = np.ascontiguousarray(first.row_dual, np.float64)
+    fd = np.ascontiguousarray(first.col_dual, np.float64)
+    check(lib().sk_pdhg_polish(lp._h, C.byref(o), C.byref(first.extra["raw"]), _ptr(fx), _ptr(fy),
+                               _ptr(fd), C.byref(res), _ptr(x), _ptr(yd), _ptr(d), _ptr(act)),
+          "sk_pdhg_polish")
+    return _result(res, x, yd, d, act)
+
+
 class Batch:
     """Many LPs solved together in one block-diagonal device solve."""
 

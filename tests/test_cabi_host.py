@@ -167,3 +167,23 @@ def test_server_options_and_missing_device():
     assert sk.lib().sk_server_create(None, C.byref(bad), C.byref(h)) == 10  # invalid argument
     # null handles are argument errors, not crashes
     assert sk.lib().sk_server_submit(None, None, -1.0, None) == 10
+
+
+def test_ipm_options_and_argument_checks():
+    o = _ffi.IpmOptions()
+    sk.lib().sk_ipm_default_options(C.byref(o))
+    # the reference's polish defaults (src/util/options.cpp:355-393, 183)
+    assert (o.iteration_limit, o.start_margin, o.max_seconds, o.max_factor_nonzeros, o.scaling) == \
+        (50, 0.1, 30.0, 50_000_000, 1)
+    nulls = [None] * 11
+    assert sk.lib().sk_pdhg_polish(*nulls) == 10
+    assert sk.lib().sk_ipm_solve(None, None, None, None, None, None, None, None, None, None) == 10
+    lp = sk.Lp.generate(1, 1)
+    res = _ffi.PdhgResult()
+    for field, value in (("start_margin", 0.0), ("max_seconds", -1.0)):
+        bad = _ffi.IpmOptions()
+        sk.lib().sk_ipm_default_options(C.byref(bad))
+        setattr(bad, field, value)
+        st = sk.lib().sk_ipm_solve(lp._h, C.byref(bad), None, None, None, C.byref(res),
+                                   None, None, None, None)
+        assert st == 10, field  # rejected before any device work

@@ -1,0 +1,102 @@
+"""The interior-point finish on the GPU (sk_ipm_solve, sk_pdhg_polish): cold solves against
+closed-form optima and the published Netlib optima, and PDHG answers polished to optimality
+and judged by the independent answer checker (oracle/checker.py)."""
+import json
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from oracle import checker
+from sih26_amd.mps import read_mps
+from tests.lp_cases import CLOSED_FORM, closed_form
+
+pytestmark = pytest.mark.gpu
+NETLIB = Path(__file__).resolve().parent / "golden" / "netlib"
+OPTIMA = json.loads((NETLIB / "optima.json").read_text())["instances"]
+FIRST = dict(pdhg_tolerance=1e-4, pdhg_stop_at_request=True, pdhg_absolute_primal=False)
+
+
+def to_lp(sk, d):
+    return sk.Lp.from_arrays(d["num_rows"], d["num_cols"], d["col_starts"], d["row_indices"],
+                             d["values"], d["cost"], d["col_lower"], d["col_upper"],
+                             d["row_lower"], d["row_upper"], d.get("sense", 1), d.get("offset", 0.0))
+
+
+def rel(a, b):
+    return abs(a - b) / max(1.0, abs(b))
+
+
+@pytest.mark.parametrize("name", [c[0] for c in CLOSED_FORM])
+def test_cold_interior_point_reaches_closed_form_optima(gpu, name):
+    d, opt, xs = closed_form(name)
+    r = gpu.solve_ipm(to_lp(gpu, d), iteration_limit=300)
+    assert r.status_name == "optimal", r.info["message"]
+    assert r.algorithm == "ipm"
+    assert rel(r.objective, opt) <= 1e-7
+    if xs is not None:
+        np.testing.assert_allclose(r.col_value, xs, atol=1e-6)
+
+
+@pytest.mark.parametrize("name", sorted(OPTIMA))
+def test_cold_interior_point_reaches_published_netlib_optima(gpu, name):
+    d = read_mps(NETLIB / f"{name}.mps")
+    r = gpu.solve_ipm(to_lp(gpu, d), iteration_limit=300)
+    assert r.status_name == "optimal", r.info["message"]
+    # the published optima carry 11 significant digits
+    assert rel(r.objective, OPTIMA[name]["published_optimal"]) <= 1e-8, r.objective
+
+
+def polish_and_check(gpu, tmp_path, lp, d, name, col_names=None, row_names=None):
+    first = gpu.solve(lp, **FIRST)
+    assert first.status_name in ("feasible", "optimal"), first.info["message"]
+    r = gpu.polish(lp, first)
+    assert r.status_name == "optimal", r.info["message"]
+    if first.status_name != "optimal":
+        assert r.algorithm == "pdhg-gpu+ipm"
+        assert r.polish_iterations > 0
+        assert r.iterations == first.iterations + r.polish_iterations
+        assert "polished by the interior point" in r.info["message"]
+    sol = tmp_path / f"{name}.sol"
+    gpu.write_solution(sol, lp, r, model_name=name, col_names=col_names, row_names=row_names)
+    report = checker.verify(d, checker.read_sol(sol))
+    assert report.verified, [line for line in report.lines if not line[0]]
+    return first, r
+
+
+@pytest.mark.parametrize("name", ["afiro", "sc50a", "sc50b", "blend", "sc105", "adlittle"])
+def test_polished_netlib_answers_pass_the_checker(gpu, tmp_path, name):
+    d = read_mps(NETLIB / f"{name}.mps")
+    _, r = polish_and_check(gpu, tmp_path, to_lp(gpu, d), d, name, d["col_names"], d["row_names"])
+    # 1e-6 relative objective against the published optimum, as the north-star finish demands
+    assert rel(r.objective, OPTIMA[name]["published_optimal"]) <= 1e-6
+
+
+@pytest.mark.parametrize("config, seed", [(1, 1), (3, 0), (3, 7)])
+def test_polished_generated_answers_pass_the_checker(gpu, tmp_path, config, seed):
+    lp = gpu.Lp.generate(config, seed)
+    d = lp.export()
+    first, r = polish_and_check(gpu, tmp_path, lp, d, f"c{config}_{seed}")
+    # the FP32 first-order objective was within 1e-4 relative KKT; the polish closes it
+    assert rel(first.objective, r.objective) <= 1e-2
+
+
+def test_polish_keeps_an_optimal_answer(gpu):
+    d = read_mps(NETLIB / "afiro.mps")
+    lp = to_lp(gpu, d)
+    first = gpu.solve(lp, pdhg_tolerance=1e-9, pdhg_precision="fp64", iteration_limit=400000)
+    assert first.status_name == "optimal"
+    r = gpu.polish(lp, first)
+    assert r.status_name == "optimal" and r.algorithm == first.algorithm
+    assert r.iterations == first.iterations and r.polish_iterations == 0
+    np.testing.assert_array_equal(r.col_value, first.col_value)
+
+
+def test_polish_declines_a_factor_above_its_cap(gpu):
+    lp = gpu.Lp.generate(1, 1)
+    first = gpu.solve(lp, **FIRST)
+    assert first.status_name == "feasible"
+    r = gpu.polish(lp, first, max_factor_nonzeros=1000)
+    assert r.status_name == "feasible" and r.algorithm == first.algorithm
+    assert "declined" in r.info["message"]
+    np.testing.assert_array_equal(r.col_value, first.col_value)
