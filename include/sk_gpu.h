@@ -193,8 +193,9 @@ typedef struct {
 void sk_pdhg_default_options(sk_pdhg_options* options);
 /* String-keyed setter over the reference's option names (src/util/options.cpp:78-455):
  * algorithm (must be "pdhg"), backend ("gpu"|"auto"), gpu, pdhg_tolerance, iteration_limit,
- * time_limit, pdhg_restart, pdhg_stop_at_request, random_seed, pdhg_polish (accepted, the
- * finish is not part of this library: must be false), plus pdhg_precision ("fp32"|"fp64"),
+ * time_limit, pdhg_restart, pdhg_stop_at_request, random_seed, pdhg_polish (validated; as in
+ * the reference, the polish is a separate step after the first-order solve: sk_pdhg_polish),
+ * plus pdhg_precision ("fp32"|"fp64"),
  * pdhg_absolute_primal and pdhg_steps_per_block. Unknown keys and malformed values are
  * rejected with SK_GPU_INVALID_ARGUMENT, as options.cpp:481-565 rejects them. */
 int32_t sk_pdhg_set_option(sk_pdhg_options* options, const char* key, const char* value);

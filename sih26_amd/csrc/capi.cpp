@@ -371,8 +371,9 @@ int32_t sk_pdhg_set_option(sk_pdhg_options* o, const char* key_in, const char* v
     return SK_GPU_SUCCESS;
   }
   if (key == "pdhg_polish") {
+    // read by the caller, which then hands the answer to sk_pdhg_polish (solve.cpp:184-243);
+    // the first-order solve itself is the same either way
     if (!parse_bool(v, &b)) return bad("expected a boolean, got");
-    if (b) return bad("the interior-point finish is not part of this library; cannot take");
     return SK_GPU_SUCCESS;
   }
   if (key == "pdhg_tolerance") {

@@ -51,11 +51,14 @@ def test_option_registry_semantics():
     assert o.tolerance == 1e-6 and o.iteration_limit == 50 and o.precision == 1 and o.restart == 0
     for key, value in [("no_such_option", "1"), ("pdhg_tolerance", "abc"),
                        ("pdhg_tolerance", "0.5"), ("algorithm", "simplex"),
-                       ("backend", "cpu"), ("pdhg_polish", "true"), ("iteration_limit", "-2"),
+                       ("backend", "cpu"), ("pdhg_polish", "maybe"), ("iteration_limit", "-2"),
                        ("pdhg_precision", "fp16")]:
         with pytest.raises(sk.SkError) as e:
             sk.make_options(**{key: value})
         assert e.value.status == 10
+    # pdhg_polish is validated and accepted; the polish itself is sk_pdhg_polish
+    sk.make_options(pdhg_polish="true")
+    sk.make_options(pdhg_polish="false")
 
 
 def test_model_validation_rejects_malformed_models():
