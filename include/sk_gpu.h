@@ -162,6 +162,18 @@ int32_t sk_lp_export(const sk_lp* lp, int32_t* sense, double* objective_offset,
                      double* col_cost, double* col_lower, double* col_upper, double* row_lower,
                      double* row_upper);
 void sk_lp_free(sk_lp* lp);
+/* read_mps (include/sankhya/io.hpp:45-46; src/io/mps_reader.cpp, mps_bounds.cpp): an MPS file
+ * (free format; fixed columns when the free reading fails) as an LP. The first N row is the
+ * objective (its RHS is the negated constant), later N rows are dropped; RANGES and BOUNDS
+ * follow the reference (first vector only; UP < 0 with no explicit lower bound sets the lower
+ * bound to -inf); duplicate entries, non-finite coefficients and quadratic sections are
+ * refused. On failure returns SK_GPU_INVALID_ARGUMENT and sk_gpu_last_error() gives
+ * "path:line: reason". integer_columns (may be NULL) receives the number of columns the file
+ * marks integer; an LP solve relaxes them. */
+int32_t sk_lp_read_mps(const char* path, sk_lp** out, int32_t* integer_columns);
+/* The name of row (kind 0) or column (kind 1) `index` of an LP read from a file, or of the
+ * model (kind 2, index ignored); NULL otherwise. Valid until sk_lp_free. */
+const char* sk_lp_name(const sk_lp* lp, int32_t kind, int32_t index);
 
 /* ---- PDHG options (the options solve_pdhg reads: pdhg.cpp:304-315) ------------------------ */
 typedef struct {

@@ -122,6 +122,8 @@ SIGNATURES = {
     "sk_pdhg_solve": (i32, [vp, P(PdhgOptions), P(PdhgResult), vp, vp, vp, vp]),
     "sk_write_solution": (i32, [cstr, vp, P(PdhgResult), vp, vp, vp, vp, vp, vp, cstr]),
     "sk_ipm_default_options": (None, [P(IpmOptions)]),
+    "sk_lp_read_mps": (i32, [cstr, P(vp), P(i32)]),
+    "sk_lp_name": (cstr, [vp, i32, i32]),
     "sk_ipm_solve": (i32, [vp, P(IpmOptions), vp, vp, vp, P(PdhgResult), vp, vp, vp, vp]),
     "sk_pdhg_polish": (i32, [vp, P(IpmOptions), P(PdhgResult), vp, vp, vp, P(PdhgResult), vp, vp,
                              vp, vp]),

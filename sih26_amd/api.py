@@ -91,6 +91,23 @@ class Lp:
         return cls.from_arrays(m, n, starts, idx, val, cost, lo, hi, row_lower, row_upper, sense, offset)
 
     @classmethod
+    def read_mps(cls, path) -> "Lp":
+        """An MPS file read by the library (sk_lp_read_mps); `integer_columns` and `names()`
+        describe what the file said beyond the LP."""
+        h = C.c_void_p()
+        k = C.c_int32()
+        check(lib().sk_lp_read_mps(str(path).encode(), C.byref(h), C.byref(k)), "sk_lp_read_mps")
+        lp = cls(h)
+        lp.integer_columns = k.value
+        return lp
+
+    def names(self) -> tuple[str, list[str], list[str]]:
+        """(model name, row names, column names) of an LP read from a file."""
+        m, n, _ = self.dims
+        get = lambda kind, i: (lib().sk_lp_name(self._h, kind, i) or b"").decode()  # noqa: E731
+        return get(2, 0), [get(0, i) for i in range(m)], [get(1, j) for j in range(n)]
+
+    @classmethod
     def generate(cls, config: int, instance: int) -> "Lp":
         h = C.c_void_p()
         check(lib().sk_lp_generate(config, instance, C.byref(h)), "sk_lp_generate")

@@ -326,6 +326,30 @@ int32_t sk_lp_export(const sk_lp* h, int32_t* sense, double* offset, int32_t* co
 
 void sk_lp_free(sk_lp* lp) { delete lp; }
 
+int32_t sk_lp_read_mps(const char* path, sk_lp** out, int32_t* integer_columns) {
+  if (!path || !out) return SK_GPU_INVALID_ARGUMENT;
+  *out = nullptr;
+  return guarded([&]() -> int32_t {
+    auto h = std::make_unique<sk_lp>();
+    std::string error;
+    if (!sk::read_mps(path, &h->lp, &h->file, &error)) {
+      sk::set_last_error(error);
+      return SK_GPU_INVALID_ARGUMENT;
+    }
+    if (integer_columns) *integer_columns = h->file.integer_columns;
+    *out = h.release();
+    return SK_GPU_SUCCESS;
+  });
+}
+
+const char* sk_lp_name(const sk_lp* lp, int32_t kind, int32_t index) {
+  if (!lp) return nullptr;
+  if (kind == 2) return lp->file.name.c_str();
+  const auto& names = kind == 0 ? lp->file.row_names : lp->file.col_names;
+  if ((kind != 0 && kind != 1) || index < 0 || static_cast<size_t>(index) >= names.size()) return nullptr;
+  return names[static_cast<size_t>(index)].c_str();
+}
+
 void sk_pdhg_default_options(sk_pdhg_options* o) {
   if (!o) return;
   std::memset(o, 0, sizeof(*o));

@@ -2,7 +2,9 @@
 #pragma once
 
 #include "lp.hpp"
+#include "mps.hpp"
 
 struct sk_lp {
   sk::HostLp lp;
+  sk::MpsInfo file;  // names and markers when the LP was read from a file
 };

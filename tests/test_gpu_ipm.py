@@ -100,3 +100,14 @@ def test_polish_declines_a_factor_above_its_cap(gpu):
     assert r.status_name == "feasible" and r.algorithm == first.algorithm
     assert "declined" in r.info["message"]
     np.testing.assert_array_equal(r.col_value, first.col_value)
+
+
+@pytest.mark.parametrize("name", ["israel", "share2b", "stocfor1"])
+def test_library_mps_reader_feeds_the_published_optimum(gpu, name):
+    # the C-ABI reader (sk_lp_read_mps) straight into the solvers: PDHG at 1e-4, then the
+    # polish, lands on the published optimum
+    lp = gpu.Lp.read_mps(NETLIB / f"{name}.mps")
+    first = gpu.solve(lp, **FIRST)
+    r = gpu.polish(lp, first)
+    assert r.status_name == "optimal", r.info["message"]
+    assert rel(r.objective, OPTIMA[name]["published_optimal"]) <= 1e-6
