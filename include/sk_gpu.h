@@ -335,7 +335,9 @@ int32_t sk_batch_stats_get(const sk_batch* batch, sk_batch_stats* out);
  * events on the batch's stream after a solve: kernel 0 primal step, 1 A xbar + dual step,
  * 2 A^T y + interaction, with every instance stepping; 3, 4, 5 the same kernels in normal
  * mode, where the tiles of instances that stopped exit at once. `bytes` receives the
- * algorithmic bytes of one launch with every instance stepping. */
+ * algorithmic bytes of one launch with every instance stepping. A batch run as several
+ * sub-batches (streams != 1) launches the kernel on every sub-batch's stream at once and
+ * reports the wall time per repetition until the last is done. */
 int32_t sk_batch_time_kernel(sk_batch* batch, int32_t kernel, int32_t reps, double* mean_ms,
                              double* bytes);
 void sk_batch_destroy(sk_batch* batch);

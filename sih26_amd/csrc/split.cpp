@@ -5,6 +5,8 @@
 // another's work. Instances do not interact, so each instance's answer is bitwise the same in
 // any split (and alone).
 #include <algorithm>
+#include <atomic>
+#include <chrono>
 #include <exception>
 #include <memory>
 #include <numeric>
@@ -168,14 +170,45 @@ class SplitEngine final : public Engine {
   const EngineTimings& timings() const override { return timings_; }
   int32_t size() const override { return static_cast<int32_t>(lps_.size()); }
 
-  // Kernel timing is a property of one launch over one block-diagonal batch: only an unsplit
-  // batch (streams = 1, or a batch too small to split) answers it.
+  // One launch of a step kernel over the batch as the engine runs it: every sub-batch launches
+  // it on its own stream, all at once, `reps` times in a row; the result is the wall time per
+  // repetition from the common start until the last sub-batch is done. With one sub-batch this
+  // is the plain kernel timing (HIP events on its stream).
   int time_kernel(int kernel, int reps, double* mean_ms) override {
-    if (parts_.size() != 1) {
-      set_last_error("time_kernel(): the batch runs as several sub-batches; create it with streams = 1");
+    if (parts_.empty()) {
+      set_last_error("time_kernel(): engine not uploaded");
       return kStatusInvalidState;
     }
-    return parts_[0]->time_kernel(kernel, reps, mean_ms);
+    if (parts_.size() == 1) return parts_[0]->time_kernel(kernel, reps, mean_ms);
+    const size_t n = parts_.size();
+    std::vector<int> status(n, kStatusSuccess);
+    std::vector<std::string> errors(n);
+    std::vector<double> part_ms(n, 0.0);
+    std::atomic<size_t> ready{0};
+    std::atomic<bool> go{false};
+    auto run = [&](size_t p) {
+      int st = use_device(device_);
+      ready.fetch_add(1);
+      while (!go.load()) std::this_thread::yield();
+      if (!st) st = parts_[p]->time_kernel(kernel, reps, &part_ms[p]);
+      status[p] = st;
+      if (st) errors[p] = last_error();
+    };
+    std::vector<std::thread> workers;
+    for (size_t p = 0; p < n; ++p) workers.emplace_back(run, p);
+    while (ready.load() < n) std::this_thread::yield();
+    const auto t0 = std::chrono::steady_clock::now();
+    go.store(true);
+    for (auto& w : workers) w.join();
+    const double total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    for (size_t p = 0; p < n; ++p) {
+      if (status[p]) {
+        set_last_error(errors[p]);
+        return status[p];
+      }
+    }
+    *mean_ms = total / reps;
+    return kStatusSuccess;
   }
   double kernel_bytes(int kernel) const override {
     double b = 0.0;
