@@ -226,7 +226,7 @@ class EngineImpl final : public Engine {
   DevBuf b_cmin_, b_cl_, b_cu_, b_rl_, b_ru_, b_dc_, b_dr_, b_rpass_, b_cpass_;
   DevBuf b_chat_, b_lhat_, b_uhat_, b_rlhat_, b_ruhat_, b_lhat64_, b_uhat64_;
   DevBuf b_x0_, b_x1_, b_y0_, b_y1_, b_aty0_, b_aty1_, b_xbar_, b_xrst_, b_yrst_;
-  DevBuf b_xsum_, b_ysum_, b_xu0_, b_xu1_, b_yu0_, b_yu1_, b_bestx_, b_besty_, b_act_, b_red_;
+  DevBuf b_xsum_, b_ysum_, b_xtot_, b_ytot_, b_xu0_, b_xu1_, b_yu0_, b_yu1_, b_bestx_, b_besty_, b_act_, b_red_;
   DevBuf b_tc_, b_tr_, b_ta_, b_tt_, b_desc_, b_state_;
   DevBuf b_ps_c_, b_ps_a_, b_ps_t_, b_pe_c_, b_pe_r_, b_pe_a_, b_pe_t_;
   DevBuf b_done_, b_v_, b_v0_, b_av_, b_len_, b_zero_, b_fa_, b_ft_, b_fc_;
@@ -235,7 +235,7 @@ class EngineImpl final : public Engine {
             &b_a_val_, &b_t_val_, &b_a_sidx_, &b_t_sidx_, &b_a_pos_, &b_t_pos_, &b_cmin_, &b_cl_, &b_cu_, &b_rl_, &b_ru_, &b_dc_, &b_dr_,
             &b_rpass_, &b_cpass_, &b_chat_, &b_lhat_, &b_uhat_, &b_rlhat_, &b_ruhat_,
             &b_lhat64_, &b_uhat64_, &b_x0_, &b_x1_, &b_y0_, &b_y1_, &b_aty0_, &b_aty1_,
-            &b_xbar_, &b_xrst_, &b_yrst_, &b_xsum_, &b_ysum_, &b_xu0_, &b_xu1_, &b_yu0_,
+            &b_xbar_, &b_xrst_, &b_yrst_, &b_xsum_, &b_ysum_, &b_xtot_, &b_ytot_, &b_xu0_, &b_xu1_, &b_yu0_,
             &b_yu1_, &b_bestx_, &b_besty_, &b_act_, &b_red_, &b_tc_, &b_tr_, &b_ta_, &b_tt_,
             &b_desc_, &b_state_, &b_ps_c_, &b_ps_a_, &b_ps_t_, &b_pe_c_, &b_pe_r_, &b_pe_a_,
             &b_pe_t_, &b_done_, &b_v_, &b_v0_, &b_av_, &b_len_, &b_zero_, &b_fa_, &b_ft_,
@@ -507,6 +507,12 @@ int EngineImpl<T>::upload() {
   SK_TRY(alloc(b_yrst_, M, &a.yrst));
   SK_TRY(alloc(b_xsum_, N, &a.xsum));
   SK_TRY(alloc(b_ysum_, M, &a.ysum));
+  {
+    // FP64 totals of the FP32 running sums (unused with FP64 iterates)
+    const bool split = !std::is_same<T, double>::value;
+    SK_TRY(alloc(b_xtot_, split ? N : 0, &a.xtot));
+    SK_TRY(alloc(b_ytot_, split ? M : 0, &a.ytot));
+  }
   SK_TRY(alloc(b_xu0_, N, &a.xu0));
   SK_TRY(alloc(b_xu1_, N, &a.xu1));
   SK_TRY(alloc(b_yu0_, M, &a.yu0));
@@ -1144,8 +1150,9 @@ int64_t estimate_device_bytes(int64_t rows, int64_t cols, int64_t nnz, int64_t i
                               int32_t precision) {
   const double w = precision == 1 ? 8.0 : 4.0;
   const double per_nnz = 40.0 + 2.0 * w + 12.0 + 1.1;
-  const double per_row = 84.0 + 5.0 * w + 0.5;
-  const double per_col = 116.0 + 9.0 * w + 0.5;
+  // each includes the iterate-precision running sum and its FP64 total
+  const double per_row = 84.0 + 6.0 * w + 0.5;
+  const double per_col = 116.0 + 10.0 * w + 0.5;
   const double bytes = per_nnz * static_cast<double>(nnz) + per_row * static_cast<double>(rows) +
                        per_col * static_cast<double>(cols) + 1024.0 * static_cast<double>(instances) +
                        262144.0;

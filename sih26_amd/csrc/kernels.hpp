@@ -25,6 +25,7 @@
 
 #include <cmath>
 #include <cstdint>
+#include <type_traits>
 
 #include "device_types.hpp"
 #include "device_util.hpp"
@@ -78,10 +79,15 @@ struct Args {
   T* xbar;
   T* xrst;  // restart point
   T* yrst;
-  // running sums since the last restart, in the iterate precision (the average is an iterate
-  // quantity; FP64 is kept for the KKT, gap and objective reductions and the reported point)
+  // running sums of the accepted iterates since the last restart. The step kernels add in the
+  // iterate precision; with FP32 iterates every evaluation folds that sum into the FP64 total
+  // (xtot, ytot) and clears it, so FP32 only ever sums the iterates between two evaluations
+  // and the average of a long restart window keeps FP64 accuracy. With FP64 iterates the sum
+  // is the FP64 running sum itself and the totals are unused.
   T* xsum;
   T* ysum;
+  double* xtot;
+  double* ytot;
   // unscaled candidates: *u0 current, *u1 running average; best; report outputs
   double* xu0;
   double* xu1;
@@ -456,19 +462,29 @@ __global__ __launch_bounds__(kBlock) void k_eval_prepare(Args<T> a) {
   const T* __restrict__ v = is_col ? (p ? a.x1 : a.x0) : (p ? a.y1 : a.y0);
   const T* __restrict__ rst = is_col ? a.xrst : a.yrst;
   T* __restrict__ sum = is_col ? a.xsum : a.ysum;
+  double* __restrict__ tot = is_col ? a.xtot : a.ytot;
   const double* __restrict__ scale = is_col ? a.dc : a.dr;
   double* __restrict__ u0 = is_col ? a.xu0 : a.yu0;
   double* __restrict__ u1 = is_col ? a.xu1 : a.yu1;
+  constexpr bool kSplitSum = !std::is_same<T, double>::value;
   double d2 = 0.0;
   for (int k = t.begin + threadIdx.x; k < t.end; k += kBlock) {
     const T vk = v[k];
-    T total = sum[k];
-    if (acc) {
-      total += vk;
-      sum[k] = total;
+    double total;
+    if constexpr (kSplitSum) {
+      total = tot[k] + static_cast<double>(sum[k]);
+      if (acc) total += static_cast<double>(vk);
+      tot[k] = total;
+      sum[k] = static_cast<T>(0);
+    } else {
+      total = sum[k];
+      if (acc) {
+        total += vk;
+        sum[k] = total;
+      }
     }
     u0[k] = static_cast<double>(vk) * scale[k];
-    u1[k] = have_avg ? (static_cast<double>(total) / count) * scale[k] : 0.0;
+    u1[k] = have_avg ? (total / count) * scale[k] : 0.0;
     const double d = static_cast<double>(vk) - static_cast<double>(rst[k]);
     d2 += d * d;
   }
@@ -769,12 +785,15 @@ __global__ __launch_bounds__(kBlock) void k_eval_commit(Args<T> a) {
   const T* __restrict__ v = is_col ? (p ? a.x1 : a.x0) : (p ? a.y1 : a.y0);
   T* __restrict__ rst = is_col ? a.xrst : a.yrst;
   T* __restrict__ sum = is_col ? a.xsum : a.ysum;
+  double* __restrict__ tot = is_col ? a.xtot : a.ytot;
   const double* __restrict__ src = is_col ? (from_avg ? a.xu1 : a.xu0) : (from_avg ? a.yu1 : a.yu0);
   double* __restrict__ best = is_col ? a.bestx : a.besty;
+  constexpr bool kSplitSum = !std::is_same<T, double>::value;
   for (int k = t.begin + threadIdx.x; k < t.end; k += kBlock) {
     if (copy) best[k] = src[k];
     if (restart) {
       sum[k] = static_cast<T>(0);
+      if constexpr (kSplitSum) tot[k] = 0.0;
       rst[k] = v[k];
     }
   }
@@ -1092,6 +1111,7 @@ __global__ __launch_bounds__(kBlock) void k_init_iterates(Args<T> a, const doubl
     a.aty0[k] = static_cast<T>(0);
     a.aty1[k] = static_cast<T>(0);
     a.xsum[k] = static_cast<T>(0);
+    if constexpr (!std::is_same<T, double>::value) a.xtot[k] = 0.0;
     // pdhg.cpp:377: best starts as the scaled start point, in FP64
     double b = 0.0;
     if (dev_finite_bound(lhat64[k]) && b < lhat64[k]) b = lhat64[k];
@@ -1103,6 +1123,7 @@ __global__ __launch_bounds__(kBlock) void k_init_iterates(Args<T> a, const doubl
     a.y1[k] = static_cast<T>(0);
     a.yrst[k] = static_cast<T>(0);
     a.ysum[k] = static_cast<T>(0);
+    if constexpr (!std::is_same<T, double>::value) a.ytot[k] = 0.0;
     a.besty[k] = 0.0;
   }
 }
