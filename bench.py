@@ -212,9 +212,11 @@ def main() -> int:
     ap.add_argument("--spmv-stage", type=int, default=-1,
                     help="staging order: -1 default, 0 CSR order, 1 sorted by gather index")
     ap.add_argument("--tile-nnz", type=int, default=0, help="row-block capacity, 0 auto")
+    ap.add_argument("--steps-per-block", type=int, default=40,
+                    help="PDHG steps per captured graph")
     ap.add_argument("--streams", type=int, default=0,
                     help="sub-batches run side by side: 0 automatic, 1 one block-diagonal batch")
-    ap.add_argument("--eval-stride", type=int, default=10,
+    ap.add_argument("--eval-stride", type=int, default=20,
                     help="evaluation stage after every this many steps of a captured block")
     ap.add_argument("--requests", type=int, default=0,
                     help="serving stream length (0: 10000 for --workload c4, 1000 otherwise)")
@@ -239,7 +241,8 @@ def main() -> int:
     if args.workload == "c4":
         return main_serving(sk, args)
     lps = instances(sk, args.workload, args.instances)
-    layout = dict(spmv_stage=args.spmv_stage, tile_nnz=args.tile_nnz, eval_stride=args.eval_stride)
+    layout = dict(spmv_stage=args.spmv_stage, tile_nnz=args.tile_nnz, eval_stride=args.eval_stride,
+                  steps_per_block=args.steps_per_block)
     batch = sk.Batch(solve_options(sk, streams=args.streams, **layout))
     for lp in lps:
         batch.add(lp)

@@ -195,7 +195,7 @@ typedef struct {
   int32_t compute_quality;   /* 1: fill the recompute_quality measures (default) */
   int32_t eval_stride;       /* an evaluation stage follows every this many steps inside a
                                 captured block, so an instance due for evaluation waits at most
-                                this many steps (default 10) */
+                                this many steps (default 20) */
   int32_t streams;           /* a batch runs as this many sub-batches side by side, each on
                                 its own stream: 0 automatic (3 for batches of 2^20 nonzeros and
                                 more, else 1), 1 one block-diagonal launch per kernel */

@@ -63,7 +63,7 @@ sk::EngineOptions to_engine(const sk_pdhg_options& o) {
   e.random_seed = o.random_seed;
   e.precision = o.precision;
   e.steps_per_block = std::max(1, o.steps_per_block);
-  e.eval_stride = std::max(1, std::min(o.eval_stride > 0 ? o.eval_stride : 10, e.steps_per_block));
+  e.eval_stride = std::max(1, std::min(o.eval_stride > 0 ? o.eval_stride : 20, e.steps_per_block));
   e.use_graph = o.use_graph;
   e.stage = o.spmv_stage;
   e.tile_nnz = o.tile_nnz;
@@ -366,7 +366,7 @@ void sk_pdhg_default_options(sk_pdhg_options* o) {
   o->spmv_stage = -1;
   o->tile_nnz = 0;
   o->compute_quality = 1;
-  o->eval_stride = 10;
+  o->eval_stride = 20;
   o->streams = 0;
   o->reserved1 = 0;
 }

@@ -27,7 +27,7 @@ struct EngineOptions {
   int64_t random_seed = 0;        // power iteration uses std::mt19937(seed + 1)
   int32_t precision = 0;          // 0: FP32 iterates + FP64 evaluation; 1: FP64 throughout
   int32_t steps_per_block = 40;   // PDHG steps per captured graph
-  int32_t eval_stride = 10;       // evaluation stage after every this many steps
+  int32_t eval_stride = 20;       // evaluation stage after every this many steps
   int32_t use_graph = 1;
   int32_t stage = -1;             // staging order of SpMV tiles: -1 default (1), 0 CSR, 1 sorted
   int32_t tile_nnz = 0;           // row-block capacity; 0 picks per instance
