@@ -51,13 +51,17 @@ enum : int32_t {
   kTileVector = 2   // one row with more than `cap` nonzeros, read straight from memory
 };
 
-// Per-tile step flags, rewritten for every tile of an instance whenever its state changes
-// (set-up and the finalize kernels), so a step kernel learns whether its tile steps, and
-// which iterate buffers it uses, from a load that does not wait for the instance record.
+// Per-tile flags, rewritten for every tile of an instance whenever its state changes (set-up,
+// the finalize kernels, and the host after it edits a record), so a step or evaluation kernel
+// learns whether its tile has work, and which iterate buffers it uses, from a load that does
+// not wait for the instance record.
 enum : uint32_t {
   kFlagStepping = 1u,  // not stopped and no evaluation pending
   kFlagParity = 2u,    // the current iterate is buffer 1
-  kFlagAccept = 4u     // the last step was accepted and is not yet in the running sums
+  kFlagAccept = 4u,    // the last step was accepted and is not yet in the running sums
+  kFlagEval = 8u,      // an evaluation is pending
+  kFlagReport = 16u,   // the final (report) evaluation is pending
+  kFlagCommit = 32u    // the evaluation's commit stage has work (best copy or restart)
 };
 
 // pdhg.cpp:81-132 (Residuals)

@@ -229,7 +229,8 @@ class EngineImpl final : public Engine {
   DevBuf b_xsum_, b_ysum_, b_xtot_, b_ytot_, b_xu0_, b_xu1_, b_yu0_, b_yu1_, b_bestx_, b_besty_, b_act_, b_red_;
   DevBuf b_tc_, b_tr_, b_ta_, b_tt_, b_desc_, b_state_;
   DevBuf b_ps_c_, b_ps_a_, b_ps_t_, b_pe_c_, b_pe_r_, b_pe_a_, b_pe_t_;
-  DevBuf b_done_, b_v_, b_v0_, b_av_, b_len_, b_zero_, b_fa_, b_ft_, b_fc_;
+  DevBuf b_done_, b_v_, b_v0_, b_av_, b_len_, b_zero_, b_fa_, b_ft_, b_fc_, b_fr_;
+  DevBuf b_a_orig_st_, b_t_orig_st_;
   std::vector<DevBuf*> buffers() {
     return {&b_a_ptr_, &b_a_idx_, &b_a_orig_, &b_t_ptr_, &b_t_idx_, &b_t_orig_, &b_sa_, &b_st_,
             &b_a_val_, &b_t_val_, &b_a_sidx_, &b_t_sidx_, &b_a_pos_, &b_t_pos_, &b_cmin_, &b_cl_, &b_cu_, &b_rl_, &b_ru_, &b_dc_, &b_dr_,
@@ -239,7 +240,7 @@ class EngineImpl final : public Engine {
             &b_yu1_, &b_bestx_, &b_besty_, &b_act_, &b_red_, &b_tc_, &b_tr_, &b_ta_, &b_tt_,
             &b_desc_, &b_state_, &b_ps_c_, &b_ps_a_, &b_ps_t_, &b_pe_c_, &b_pe_r_, &b_pe_a_,
             &b_pe_t_, &b_done_, &b_v_, &b_v0_, &b_av_, &b_len_, &b_zero_, &b_fa_, &b_ft_,
-            &b_fc_};
+            &b_fc_, &b_fr_, &b_a_orig_st_, &b_t_orig_st_};
   }
   std::vector<const DevBuf*> buffers() const {
     auto all = const_cast<EngineImpl<T>*>(this)->buffers();
@@ -549,6 +550,14 @@ int EngineImpl<T>::upload() {
     a.t_sidx = tsx;
     a.a_pos = apos;
     a.t_pos = tpos;
+    // the original FP64 values in the same order, for the evaluation kernels
+    double *aos, *tos;
+    SK_TRY(alloc(b_a_orig_st_, Z + kStagePad, &aos));
+    SK_TRY(alloc(b_t_orig_st_, Z + kStagePad, &tos));
+    SK_LAUNCH(k_stage_values<double><<<n_ta_, kBlock, 0, stream_>>>(pta, ap, asx, apos, pack, a.a_orig, aos));
+    SK_LAUNCH(k_stage_values<double><<<n_tt_, kBlock, 0, stream_>>>(ptt, tp, tsx, tpos, pack, a.t_orig, tos));
+    a.a_orig_st = aos;
+    a.t_orig_st = tos;
   }
   InstDesc* pdesc;
   SK_TRY(put(b_desc_, desc_, &pdesc));
@@ -570,6 +579,7 @@ int EngineImpl<T>::upload() {
   SK_TRY(alloc(b_fa_, static_cast<size_t>(n_ta_), &a.fa));
   SK_TRY(alloc(b_ft_, static_cast<size_t>(n_tt_), &a.ft));
   SK_TRY(alloc(b_fc_, static_cast<size_t>(n_tc_), &a.fc));
+  SK_TRY(alloc(b_fr_, static_cast<size_t>(n_tr_), &a.fr));
   a.n_tc = n_tc_;
   a.n_ta = n_ta_;
   a.n_tt = n_tt_;
@@ -813,6 +823,7 @@ int EngineImpl<T>::run_loop() {
       }
     }
     SK_HIP(hipMemcpy(args_.state, cur.data(), cur.size() * sizeof(InstState), hipMemcpyHostToDevice), kStatusCopyFailed);
+    SK_LAUNCH(k_publish_flags<T><<<K_, kBlock, 0, s>>>(args_));
   }
   return kStatusSuccess;
 }
@@ -925,6 +936,7 @@ int EngineImpl<T>::stop_on_time_limit(std::vector<int32_t>* newly_done) {
     }
   }
   SK_HIP(hipMemcpy(args_.state, cur.data(), cur.size() * sizeof(InstState), hipMemcpyHostToDevice), kStatusCopyFailed);
+  SK_LAUNCH(k_publish_flags<T><<<K_, kBlock, 0, s>>>(args_));
   int st = enqueue_eval(1);
   if (st) return st;
   SK_HIP(hipMemcpyAsync(host_state_, args_.state, cur.size() * sizeof(InstState), hipMemcpyDeviceToHost, s),
@@ -1149,7 +1161,7 @@ int query_device_memory(int64_t* free_bytes, int64_t* total_bytes) {
 int64_t estimate_device_bytes(int64_t rows, int64_t cols, int64_t nnz, int64_t instances,
                               int32_t precision) {
   const double w = precision == 1 ? 8.0 : 4.0;
-  const double per_nnz = 40.0 + 2.0 * w + 12.0 + 1.1;
+  const double per_nnz = 56.0 + 2.0 * w + 12.0 + 1.1;
   // each includes the iterate-precision running sum and its FP64 total
   const double per_row = 84.0 + 6.0 * w + 0.5;
   const double per_col = 116.0 + 10.0 * w + 0.5;

@@ -52,9 +52,12 @@ struct Args {
   const int32_t* __restrict__ t_sidx;
   const T* __restrict__ t_val;
   const uint16_t* __restrict__ t_pos;
-  // the original A in the same two layouts, FP64 (evaluation)
+  // the original A in the same two layouts, FP64 (evaluation): CSR order, and the staging
+  // order of the step kernels' tiles
   const double* __restrict__ a_orig;
   const double* __restrict__ t_orig;
+  const double* __restrict__ a_orig_st;
+  const double* __restrict__ t_orig_st;
   // scaled problem data in the iterate precision; absent bounds are +-inf
   const T* __restrict__ chat;
   const T* __restrict__ lhat;
@@ -104,10 +107,11 @@ struct Args {
   const Tile* __restrict__ tt;  // row blocks of CSR(A^T)
   const InstDesc* __restrict__ desc;
   InstState* state;
-  // per-tile step flags (kFlag*) of ta, tt and tc
+  // per-tile flags (kFlag*) of ta, tt, tc and tr
   uint32_t* fa;
   uint32_t* ft;
   uint32_t* fc;
+  uint32_t* fr;
   // per-tile partials
   double* ps_c;  // step: ||x'-x||^2 per column chunk
   double* ps_a;  // step: ||y'-y||^2 per CSR(A) block
@@ -128,7 +132,9 @@ __device__ __forceinline__ bool stepping(const InstState& s) { return !s.done &&
 
 __device__ __forceinline__ uint32_t step_flags(const InstState& s) {
   return (stepping(s) ? kFlagStepping : 0u) | (s.parity ? kFlagParity : 0u) |
-         (s.pending_accept ? kFlagAccept : 0u);
+         (s.pending_accept ? kFlagAccept : 0u) | (s.eval_pending ? kFlagEval : 0u) |
+         (s.report_pending ? kFlagReport : 0u) |
+         ((s.copy_best || s.do_restart) ? kFlagCommit : 0u);
 }
 
 // Writes `f` into the step flags of every tile of the instance (whole workgroup).
@@ -137,6 +143,7 @@ __device__ __forceinline__ void publish_flags(const Args<T>& a, const InstDesc& 
   for (int k = d.ta0 + threadIdx.x; k < d.ta1; k += kBlock) a.fa[k] = f;
   for (int k = d.tt0 + threadIdx.x; k < d.tt1; k += kBlock) a.ft[k] = f;
   for (int k = d.tc0 + threadIdx.x; k < d.tc1; k += kBlock) a.fc[k] = f;
+  for (int k = d.tr0 + threadIdx.x; k < d.tr1; k += kBlock) a.fr[k] = f;
 }
 
 // Set-up: the first flags of every instance, one workgroup per instance.
@@ -452,9 +459,9 @@ __global__ __launch_bounds__(kBlock) void k_eval_prepare(Args<T> a) {
   const int b = xcd_tile(blockIdx.x, gridDim.x);
   const bool is_col = b < a.n_tc;
   const int tile = is_col ? b : b - a.n_tc;
+  if (!((is_col ? a.fc[tile] : a.fr[tile]) & kFlagEval)) return;
   const Tile t = is_col ? a.tc[tile] : a.tr[tile];
   const InstState& s = a.state[t.inst];
-  if (!s.eval_pending) return;
   const int p = s.parity;
   const bool acc = s.pending_accept;
   const double count = static_cast<double>(s.averaged);
@@ -500,39 +507,40 @@ __global__ __launch_bounds__(kBlock) void k_eval_rows(Args<T> a) {
   __shared__ D2 stage[CAP];
   __shared__ double red[8 * (kBlock / kWave)];
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  if (!(a.fa[tile] & (a.report ? kFlagReport : kFlagEval))) return;
   const Tile t = a.ta[tile];
-  const InstState& s = a.state[t.inst];
-  if (!(a.report ? s.report_pending : s.eval_pending)) return;
   const double* __restrict__ xa = a.report ? a.bestx : a.xu0;
   const double* __restrict__ xb = a.report ? a.bestx : a.xu1;
   const double* __restrict__ ya = a.report ? a.besty : a.yu0;
   const double* __restrict__ yb = a.report ? a.besty : a.yu1;
-  const double* __restrict__ val = a.a_orig;
+  const double* __restrict__ val = a.a_orig_st;
   const int32_t* __restrict__ sidx = a.a_sidx;
-  const uint16_t* __restrict__ pos = a.a_pos;
-  // the step kernels' staging order; the FP64 values stay in CSR order, read through the slot
-  // map (a vector tile's staging order is its CSR order)
+  // the step kernels' staging order, with the FP64 values stored in it too
   double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   const auto prod = [&](int q) {
     const int c = gather_index<PACK>(t, sidx, q);
     const double v = val[q];
     return D2{v * xa[c], v * xb[c]};
   };
-  const double* __restrict__ v0 = val + t.p0;
+  auto f = [xa, xb](int c, int, double v) { return D2{v * xa[c], v * xb[c]}; };
+  constexpr int R = kMaxTileRows / kBlock;
+  double lo_[R], hi_[R], ya_[R], yb_[R];
   rowblock_product<kBlock, D2>(
       t, a.a_ptr, prod,
-      slot_stager<kBlock, D2, PACK>(sidx, pos, [=](int c, int slot) {
-        const double v = v0[slot];
-        return D2{v * xa[c], v * xb[c]};
-      }),
-      [](int, int) {},
-      [&](int i, int, D2 act) {
-        const double lo = a.rl[i], hi = a.ru[i];
+      GroupStager<kBlock, D2, PACK, true, double, decltype(f)>{sidx, a.a_pos, val, f},
+      [&](int i, int k) {
+        lo_[k] = a.rl[i];
+        hi_[k] = a.ru[i];
+        ya_[k] = ya[i];
+        yb_[k] = yb[i];
+      },
+      [&](int i, int k, D2 act) {
+        const double lo = lo_[k], hi = hi_[k];
         const bool lo_f = dev_finite_bound(lo), hi_f = dev_finite_bound(hi);
         if (a.report) a.activity[i] = act.a;
         for (int c = 0; c < 2; ++c) {
           const double ai = c == 0 ? act.a : act.b;
-          const double yi = c == 0 ? ya[i] : yb[i];
+          const double yi = c == 0 ? ya_[k] : yb_[k];
           double viol = 0.0;
           if (lo_f) viol = dev_max(viol, lo - ai);
           if (hi_f) viol = dev_max(viol, ai - hi);
@@ -651,39 +659,41 @@ __global__ __launch_bounds__(kBlock) void k_eval_cols(Args<T> a) {
   __shared__ D2 stage[CAP];
   __shared__ double red[8 * (kBlock / kWave)];
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  if (!(a.ft[tile] & (a.report ? kFlagReport : kFlagEval))) return;
   const Tile t = a.tt[tile];
-  const InstState& s = a.state[t.inst];
-  if (!(a.report ? s.report_pending : s.eval_pending)) return;
   const double* __restrict__ xa = a.report ? a.bestx : a.xu0;
   const double* __restrict__ xb = a.report ? a.bestx : a.xu1;
   const double* __restrict__ ya = a.report ? a.besty : a.yu0;
   const double* __restrict__ yb = a.report ? a.besty : a.yu1;
-  const double* __restrict__ val = a.t_orig;
+  const double* __restrict__ val = a.t_orig_st;
   const int32_t* __restrict__ sidx = a.t_sidx;
-  const uint16_t* __restrict__ pos = a.t_pos;
-  // the FP64 values stay in CSR order, read through the slot map (a vector tile's staging
-  // order is its CSR order)
+  // the step kernels' staging order, with the FP64 values stored in it too
   double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   const auto prod = [&](int q) {
     const int r = gather_index<PACK>(t, sidx, q);
     const double v = val[q];
     return D2{v * ya[r], v * yb[r]};
   };
-  const double* __restrict__ v0 = val + t.p0;
+  auto f = [ya, yb](int r, int, double v) { return D2{v * ya[r], v * yb[r]}; };
+  constexpr int R = kMaxTileRows / kBlock;
+  double lo_[R], hi_[R], c_[R], xa_[R], xb_[R];
   rowblock_product<kBlock, D2>(
       t, a.t_ptr, prod,
-      slot_stager<kBlock, D2, PACK>(sidx, pos, [=](int r, int slot) {
-        const double v = v0[slot];
-        return D2{v * ya[r], v * yb[r]};
-      }),
-      [](int, int) {},
-      [&](int j, int, D2 dot) {
-        const double lo = a.cl[j], hi = a.cu[j];
+      GroupStager<kBlock, D2, PACK, true, double, decltype(f)>{sidx, a.t_pos, val, f},
+      [&](int j, int k) {
+        lo_[k] = a.cl[j];
+        hi_[k] = a.cu[j];
+        c_[k] = a.cmin[j];
+        xa_[k] = xa[j];
+        xb_[k] = xb[j];
+      },
+      [&](int j, int k, D2 dot) {
+        const double lo = lo_[k], hi = hi_[k];
         const bool lo_f = dev_finite_bound(lo), hi_f = dev_finite_bound(hi);
-        const double cj = a.cmin[j];
+        const double cj = c_[k];
         for (int c = 0; c < 2; ++c) {
           const double d = cj + (c == 0 ? dot.a : dot.b);
-          const double xj = c == 0 ? xa[j] : xb[j];
+          const double xj = c == 0 ? xa_[k] : xb_[k];
           if (c == 0 && a.report) a.reduced[j] = d;
           if (d > 0.0) {
             if (lo_f) acc[4 * c + 1] += d * lo;
@@ -767,7 +777,7 @@ __global__ __launch_bounds__(kBlock) void k_eval_finalize(Args<T> a) {
     flags = step_flags(s);
   }
   __syncthreads();
-  if (!a.report) publish_flags(a, d, flags);
+  publish_flags(a, d, flags);
 }
 
 // k_eval_commit: apply the decisions of k_eval_cols (pdhg.cpp:541-545, 592-594, 632-638).
@@ -777,9 +787,9 @@ __global__ __launch_bounds__(kBlock) void k_eval_commit(Args<T> a) {
   const int b = xcd_tile(blockIdx.x, gridDim.x);
   const bool is_col = b < a.n_tc;
   const int tile = is_col ? b : b - a.n_tc;
+  if (!((is_col ? a.fc[tile] : a.fr[tile]) & kFlagCommit)) return;
   const Tile t = is_col ? a.tc[tile] : a.tr[tile];
   const InstState& s = a.state[t.inst];
-  if (!s.copy_best && !s.do_restart) return;
   const int p = s.parity;
   const bool copy = s.copy_best, restart = s.do_restart, from_avg = s.chosen_avg;
   const T* __restrict__ v = is_col ? (p ? a.x1 : a.x0) : (p ? a.y1 : a.y0);
