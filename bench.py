@@ -55,25 +55,105 @@ def instances(sk, workload: str, count: int):
     return [sk.Lp.generate(cfg, seed)]
 
 
+def host_cpu() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def host_threads() -> int:
+    """The host threads this process may use, at most 16 (one GPU's share of the host)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
 def cpu_baseline(lps, budget_s: float) -> dict:
-    """The FP64 CPU oracle (restatement of src/pdhg/pdhg.cpp), one thread, same stop rule,
-    on instances 0, 1, 2, ... until `budget_s` of CPU time is used."""
+    """The FP64 CPU oracle (restatement of src/pdhg/pdhg.cpp), same stop rule, on instances
+    0, 1, 2, ... until `budget_s` of CPU time is used: one thread, then the same instances
+    with the A^T y gather on every host thread (the reference's OpenMP loop,
+    sparse.cpp:216-220). Rates with and without the set-up (scaling + power iteration).
+    A single-LP workload is run under a time limit of `budget_s` and reported in PDHG
+    iterations/s of the loop (set-up excluded): the oracle does not finish it in minutes."""
     from oracle import oracle
     oracle.build(ref=False, arch_flags="-march=native")
     oracle.lib()
-    done, iters, t0 = 0, 0, time.perf_counter()
-    for lp in lps:
-        r = oracle.solve_pdhg(lp.export(), tolerance=1e-4, stop_at_request=True,
-                              absolute_primal=False)
-        done += 1
-        iters += r["iterations"]
-        if time.perf_counter() - t0 > budget_s:
-            break
-    dt = time.perf_counter() - t0
+    exported = []
+    if len(lps) == 1:
+        return cpu_baseline_single(oracle, lps[0].export(), budget_s)
+
+    def run(sample):
+        done, iters, setup, t0 = 0, 0, 0.0, time.perf_counter()
+        for lp in sample:
+            r = oracle.solve_pdhg(lp, tolerance=1e-4, stop_at_request=True,
+                                  absolute_primal=False)
+            done += 1
+            iters += r["iterations"]
+            setup += r["setup_seconds"]
+            if sample is not exported:
+                exported.append(lp)
+            if time.perf_counter() - t0 > budget_s:
+                break
+        return done, iters, setup, time.perf_counter() - t0
+
+    oracle.set_threads(1)
+    done, iters, setup, dt = run(lp.export() for lp in lps)
+    threads = host_threads()
+    oracle.set_threads(threads)
+    done_t, iters_t, setup_t, dt_t = run(exported)
+    oracle.set_threads(1)
     return {"value": done / dt, "unit": "LPs/s", "cores": 1, "kind": "port",
             "sample": f"first {done} of the {len(lps)} instances, FP64 oracle, 1 thread, "
                       f"{dt:.1f} s, {iters / dt:.0f} PDHG iterations/s",
-            "iters_per_s": iters / dt}
+            "iters_per_s": iters / dt,
+            "value_without_setup": done / (dt - setup), "setup_s": setup,
+            "host_cpu": host_cpu(),
+            "all_threads": {"value": done_t / dt_t, "cores": threads,
+                            "iters_per_s": iters_t / dt_t,
+                            "value_without_setup": done_t / (dt_t - setup_t),
+                            "sample": f"the same {done_t} instances, A^T y gather on {threads} "
+                                      f"threads, {dt_t:.1f} s"}}
+
+
+def cpu_baseline_single(oracle, lp: dict, budget_s: float) -> dict:
+    legs = []
+    for threads in (1, host_threads()):
+        oracle.set_threads(threads)
+        solves, iters, setup, t0 = 0, 0, 0.0, time.perf_counter()
+        while True:
+            r = oracle.solve_pdhg(lp, tolerance=1e-4, stop_at_request=True,
+                                  absolute_primal=False, time_limit=budget_s)
+            solves += 1
+            iters += r["iterations"]
+            setup += r["setup_seconds"]
+            if r["status"] == 7 or time.perf_counter() - t0 > budget_s:
+                break
+        legs.append((threads, solves, iters, setup, time.perf_counter() - t0, r["status"]))
+    oracle.set_threads(1)
+    (_, n1, it1, su1, dt1, st1), (threads, nt, itt, sut, dtt, stt) = legs
+    if st1 != 7 and stt != 7:  # the LP finishes within the budget: LPs/s as for a batch
+        return {"value": n1 / dt1, "unit": "LPs/s", "cores": 1, "kind": "port",
+                "sample": f"the LP solved {n1} times ({it1 // n1} iterations each), FP64 "
+                          f"oracle, 1 thread, {dt1:.1f} s",
+                "iters_per_s": it1 / dt1, "value_without_setup": n1 / (dt1 - su1),
+                "setup_s": su1 / n1, "host_cpu": host_cpu(),
+                "all_threads": {"value": nt / dtt, "cores": threads, "iters_per_s": itt / dtt,
+                                "value_without_setup": nt / (dtt - sut),
+                                "sample": f"solved {nt} times, A^T y gather on {threads} "
+                                          "threads"}}
+    return {"value": it1 / (dt1 - su1), "unit": "PDHG iterations/s", "cores": 1, "kind": "port",
+            "sample": f"the LP's first {it1} iterations (time limit {budget_s:.0f} s, status "
+                      f"{st1}), FP64 oracle, 1 thread; set-up {su1:.1f} s excluded",
+            "setup_s": su1, "host_cpu": host_cpu(),
+            "all_threads": {"value": itt / (dtt - sut), "cores": threads, "setup_s": sut,
+                            "sample": f"first {itt} iterations, A^T y gather on {threads} "
+                                      "threads"}}
 
 
 def fp64_single(sk, lps, budget_s: float) -> dict:
@@ -372,8 +452,43 @@ def main_serving(sk, args) -> int:
         "latency_ms": srv["latency_ms"],
         "serving": srv,
     }
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_serving(sk, args.rate, args.cpu_budget)
     print(json.dumps(line))
     return 0
+
+
+def cpu_serving(sk, rate: float, budget_s: float) -> dict:
+    """The FP64 CPU oracle as a sequential server on the same request stream (same arrivals,
+    same LPs): request i starts when it has arrived and request i-1 is done; its service time
+    is the oracle's measured solve time. Run on the first requests until `budget_s` of solving
+    is spent, so the latencies are those of that prefix of the stream."""
+    import numpy as np
+    from oracle import oracle
+    oracle.build(ref=False, arch_flags="-march=native")
+    oracle.set_threads(1)
+    gaps = np.random.default_rng(42).exponential(1.0 / rate, 100000)
+    arrivals = np.cumsum(gaps) - gaps[0]
+    service, free_at, lat, spent = [], 0.0, [], 0.0
+    for i in range(len(arrivals)):
+        lp = sk.Lp.generate(4, i).export()
+        t0 = time.perf_counter()
+        oracle.solve_pdhg(lp, tolerance=1e-4, stop_at_request=True, absolute_primal=False)
+        s = time.perf_counter() - t0
+        spent += s
+        start = max(free_at, float(arrivals[i]))
+        free_at = start + s
+        service.append(s)
+        lat.append(free_at - float(arrivals[i]))
+        if spent > budget_s:
+            break
+    q = lambda v, p: 1e3 * float(np.percentile(v, p))  # noqa: E731
+    return {"value": len(service) / spent, "unit": "LPs/s", "cores": 1, "kind": "port",
+            "sample": f"the stream's first {len(service)} requests through the FP64 oracle as a "
+                      f"sequential server, 1 thread, {spent:.1f} s of solving",
+            "service_ms": {"p50": q(service, 50), "p99": q(service, 99)},
+            "latency_ms": {"p50": q(lat, 50), "p99": q(lat, 99)},
+            "host_cpu": host_cpu()}
 
 
 if __name__ == "__main__":

@@ -92,6 +92,7 @@ def lib() -> C.CDLL:
                                             C.POINTER(i32)]
         _lib.ok_multiply.argtypes = [i32, i32, vp, vp, vp, vp, vp]
         _lib.ok_transpose_multiply.argtypes = [i32, i32, vp, vp, vp, vp, vp]
+        _lib.ok_set_threads.argtypes = [i32]
         _lib.ok_build_scaling.argtypes = [C.POINTER(Model), vp, i32] + [vp] * 10
     return _lib
 
@@ -209,6 +210,11 @@ def spectral_norm(lp: dict[str, Any], values: np.ndarray | None = None, iteratio
     fn = ref().ref_spectral_norm if use_ref else lib().ok_spectral_norm
     return fn(a["num_rows"], a["num_cols"], _p(a["col_starts"]), _p(a["row_indices"]), _p(v),
               iterations, seed)
+
+
+def set_threads(threads: int) -> None:
+    """Threads of the A^T x gather (sparse.cpp:216-220); results do not depend on it."""
+    lib().ok_set_threads(int(threads))
 
 
 def multiply(lp: dict[str, Any], x: np.ndarray, transpose: bool = False, use_ref: bool = False) -> np.ndarray:

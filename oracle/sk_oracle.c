@@ -64,8 +64,16 @@ void ok_multiply(int32_t rows, int32_t cols, const int32_t* cs, const int32_t* r
   }
 }
 
+/* Threads of the A^T x gather, the one loop the reference parallelises (sparse.cpp:216-220:
+ * OpenMP, static schedule; each y[j] is written by one thread, so the result does not depend
+ * on the thread count). 1 = no parallel region at all. */
+static int g_threads = 1;
+
+void ok_set_threads(int32_t threads) { g_threads = threads > 1 ? threads : 1; }
+
 static void transpose_multiply_add(int32_t cols, const int32_t* cs, const int32_t* ri,
                                    const double* v, const double* x, double* y) {
+#pragma omp parallel for schedule(static) num_threads(g_threads) if (g_threads > 1)
   for (int32_t j = 0; j < cols; ++j) {
     double dot = 0.0;
     for (int32_t p = cs[j]; p < cs[j + 1]; ++p) dot += v[p] * x[ri[p]];

@@ -127,6 +127,9 @@ double ok_project(double value, double lower, double upper);
 /* sparse.cpp:202-212, 231-234: y = A x by column scatter, skipping x_j == 0 */
 void ok_multiply(int32_t rows, int32_t cols, const int32_t* col_starts,
                  const int32_t* row_indices, const double* values, const double* x, double* y);
+/* Threads of the A^T x gather (sparse.cpp:216-220); default 1. Results do not depend on it. */
+void ok_set_threads(int32_t threads);
+
 /* sparse.cpp:214-229, 236-239: y = A^T x by column gather */
 void ok_transpose_multiply(int32_t rows, int32_t cols, const int32_t* col_starts,
                            const int32_t* row_indices, const double* values, const double* x,

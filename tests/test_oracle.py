@@ -229,6 +229,22 @@ def test_oracle_iteration_limit_is_honest():  # test_pdhg.cpp:148-157
     assert r["status"] == 6 and np.isinf(r["dual_bound"]) and r["iterations"] == 20
 
 
+def test_gather_threads_do_not_change_the_answer():
+    # sparse.cpp:216-218: each A^T y entry has one writer, so any thread count gives the same
+    # bits (the CPU baseline's all-threads leg relies on it)
+    lp = kkt_lp(np.random.default_rng(7), 60, 150, 0.08)[0]
+    try:
+        oracle.set_threads(1)
+        a = oracle.solve_pdhg(lp, tolerance=1e-6)
+        oracle.set_threads(4)
+        b = oracle.solve_pdhg(lp, tolerance=1e-6)
+    finally:
+        oracle.set_threads(1)
+    assert a["iterations"] == b["iterations"] > 0
+    for k in ("col_value", "row_dual", "col_dual"):
+        np.testing.assert_array_equal(bits(a[k]), bits(b[k]))
+
+
 def test_oracle_step_size_survives_first_iteration():  # test_pdhg.cpp:248-263
     d, _, _ = closed_form("sum")
     r = oracle.solve_pdhg(d, tolerance=1e-8)
