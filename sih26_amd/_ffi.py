@@ -44,7 +44,7 @@ class PdhgOptions(C.Structure):
                 ("random_seed", i64), ("precision", i32), ("steps_per_block", i32),
                 ("use_graph", i32), ("spmv_stage", i32), ("tile_nnz", i32),
                 ("compute_quality", i32), ("eval_stride", i32), ("streams", i32),
-                ("reserved1", i32)]
+                ("column_order", i32)]
 
 
 class PdhgResult(C.Structure):

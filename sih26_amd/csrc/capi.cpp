@@ -68,6 +68,7 @@ sk::EngineOptions to_engine(const sk_pdhg_options& o) {
   e.stage = o.spmv_stage;
   e.tile_nnz = o.tile_nnz;
   e.streams = o.streams;
+  e.column_order = o.column_order == 1 || o.column_order == 2 ? o.column_order : 0;
   return e;
 }
 
@@ -368,7 +369,7 @@ void sk_pdhg_default_options(sk_pdhg_options* o) {
   o->compute_quality = 1;
   o->eval_stride = 20;
   o->streams = 0;
-  o->reserved1 = 0;
+  o->column_order = 0;
 }
 
 int32_t sk_pdhg_set_option(sk_pdhg_options* o, const char* key_in, const char* value_in) {

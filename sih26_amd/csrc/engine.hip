@@ -88,6 +88,34 @@ void build_row_tiles(const std::vector<int32_t>& ptr, int32_t row0, int32_t rows
   if (out->size() == before) emit(row0, row0, kTileShort);
 }
 
+// Column order on the device. When an instance's iterate vector exceeds one XCD's L2
+// (kRelabelBytes; mode 0) or always (mode 1), its kHotBytes / w columns with the most nonzeros (the x-bar entries that
+// the gathers of A x-bar touch most often) come first, largest first with ties in index order,
+// and the other columns follow in their own order: that head of x-bar stays resident in every
+// L2 while the rest streams. Returns new -> old, or nothing for the identity. The relabelling
+// depends on the instance alone, so an instance gets the same order alone or in a batch.
+constexpr size_t kRelabelBytes = size_t{4} << 20;
+constexpr size_t kHotBytes = size_t{2} << 20;
+std::vector<int32_t> relabel_columns(const HostLp& lp, size_t word, int32_t mode) {
+  const auto n = static_cast<size_t>(lp.num_cols);
+  if (mode == 2 || (mode == 0 && n * word <= kRelabelBytes)) return {};
+  const size_t hot = std::min(n, kHotBytes / word);
+  std::vector<int32_t> order(n);
+  for (size_t j = 0; j < n; ++j) order[j] = static_cast<int32_t>(j);
+  auto degree = [&](int32_t j) { return lp.col_starts[static_cast<size_t>(j) + 1] - lp.col_starts[static_cast<size_t>(j)]; };
+  std::partial_sort(order.begin(), order.begin() + static_cast<std::ptrdiff_t>(hot), order.end(),
+                    [&](int32_t a, int32_t b) { return degree(a) != degree(b) ? degree(a) > degree(b) : a < b; });
+  std::vector<uint8_t> taken(n, 0);
+  for (size_t k = 0; k < hot; ++k) taken[static_cast<size_t>(order[k])] = 1;
+  size_t next = hot;
+  for (size_t j = 0; j < n; ++j)
+    if (!taken[j]) order[next++] = static_cast<int32_t>(j);
+  bool identity = true;
+  for (size_t j = 0; j < n && identity; ++j) identity = order[j] == static_cast<int32_t>(j);
+  if (identity) order.clear();
+  return order;
+}
+
 void build_chunks(int32_t begin, int32_t count, int32_t inst, std::vector<Tile>* out) {
   const size_t before = out->size();
   for (int32_t k = 0; k < count; k += kChunk)
@@ -275,6 +303,8 @@ class EngineImpl final : public Engine {
   // results
   std::vector<InstState> final_state_;
   std::vector<double> bestx_, besty_, act_, red_;
+  // per instance, device column -> the LP's column (relabel_columns); empty for the identity
+  std::vector<std::vector<int32_t>> col_perm_;
   EngineTimings timings_;
 };
 
@@ -315,6 +345,7 @@ int EngineImpl<T>::upload() {
   trivial_norm_.assign(static_cast<size_t>(K_), 0);
   v0_.assign(N, 0.0);
   std::vector<Tile> ta, tt, tc, tr;
+  col_perm_.assign(static_cast<size_t>(K_), {});
   int32_t row0 = 0, col0 = 0, nz0 = 0;
   const int64_t limit = opt_.iteration_limit < 0 ? 1000000 : opt_.iteration_limit;
   for (int32_t k = 0; k < K_; ++k) {
@@ -322,22 +353,28 @@ int EngineImpl<T>::upload() {
     const int32_t m = lp.num_rows, n = lp.num_cols;
     // counts per row
     for (int32_t p = 0; p < lp.nnz(); ++p) ++a_ptr[static_cast<size_t>(row0 + lp.row_indices[p]) + 1];
+    std::vector<int32_t>& perm = col_perm_[static_cast<size_t>(k)];
+    perm = relabel_columns(lp, sizeof(T), opt_.column_order);
+    auto lp_col = [&](int32_t j) { return perm.empty() ? j : perm[static_cast<size_t>(j)]; };
+    int32_t nz = nz0;
+    t_ptr[static_cast<size_t>(col0)] = nz0;
     for (int32_t j = 0; j < n; ++j) {
-      t_ptr[static_cast<size_t>(col0 + j) + 1] = nz0 + lp.col_starts[static_cast<size_t>(j) + 1];
-      for (int32_t p = lp.col_starts[j]; p < lp.col_starts[j + 1]; ++p) {
-        t_idx[static_cast<size_t>(nz0 + p)] = row0 + lp.row_indices[p];
-        t_orig[static_cast<size_t>(nz0 + p)] = lp.values[p];
+      const int32_t o = lp_col(j);
+      for (int32_t p = lp.col_starts[o]; p < lp.col_starts[o + 1]; ++p, ++nz) {
+        t_idx[static_cast<size_t>(nz)] = row0 + lp.row_indices[p];
+        t_orig[static_cast<size_t>(nz)] = lp.values[p];
       }
+      t_ptr[static_cast<size_t>(col0 + j) + 1] = nz;
     }
-    if (n == 0) t_ptr[static_cast<size_t>(col0)] = nz0;
     const double sense = static_cast<double>(lp.sense);
     sense_[static_cast<size_t>(k)] = lp.sense;
     double cost_square = 0.0, bound_square = 0.0;
+    for (int32_t j = 0; j < n; ++j) cost_square += (sense * lp.cost[j]) * (sense * lp.cost[j]);
     for (int32_t j = 0; j < n; ++j) {
-      cmin[static_cast<size_t>(col0 + j)] = sense * lp.cost[j];
-      cost_square += cmin[static_cast<size_t>(col0 + j)] * cmin[static_cast<size_t>(col0 + j)];
-      cl[static_cast<size_t>(col0 + j)] = lp.col_lower[j];
-      cu[static_cast<size_t>(col0 + j)] = lp.col_upper[j];
+      const int32_t o = lp_col(j);
+      cmin[static_cast<size_t>(col0 + j)] = sense * lp.cost[o];
+      cl[static_cast<size_t>(col0 + j)] = lp.col_lower[o];
+      cu[static_cast<size_t>(col0 + j)] = lp.col_upper[o];
     }
     for (int32_t i = 0; i < m; ++i) {
       rl[static_cast<size_t>(row0 + i)] = lp.row_lower[i];
@@ -349,9 +386,12 @@ int EngineImpl<T>::upload() {
     }
     // power-iteration start (scaling.cpp:168-171), the same libstdc++ engine and distribution
     if (m > 0 && n > 0 && lp.nnz() > 0) {
+      // drawn in the LP's column order, placed in the device order
       std::mt19937 rng(static_cast<unsigned>(opt_.random_seed) + 1u);
       std::uniform_real_distribution<double> spread(-1.0, 1.0);
-      for (int32_t j = 0; j < n; ++j) v0_[static_cast<size_t>(col0 + j)] = spread(rng);
+      std::vector<double> draw(static_cast<size_t>(n));
+      for (int32_t j = 0; j < n; ++j) draw[static_cast<size_t>(j)] = spread(rng);
+      for (int32_t j = 0; j < n; ++j) v0_[static_cast<size_t>(col0 + j)] = draw[static_cast<size_t>(lp_col(j))];
     } else {
       trivial_norm_[static_cast<size_t>(k)] = 1;
       init_state_[static_cast<size_t>(k)].norm_trivial = 1;
@@ -1050,9 +1090,15 @@ int EngineImpl<T>::result(int32_t index, InstanceResult* out) const {
   out->dual_bound = verifiable ? sense * f.dual_objective + (lp ? lp->objective_offset : 0.0)
                                : (sense < 0 ? INFINITY : -INFINITY);
   const auto c0 = static_cast<size_t>(d.col0), r0 = static_cast<size_t>(d.row0);
-  out->col_value.assign(bestx_.begin() + static_cast<std::ptrdiff_t>(c0), bestx_.begin() + static_cast<std::ptrdiff_t>(c0 + static_cast<size_t>(d.cols)));
+  // back to the LP's column order
+  const std::vector<int32_t>& perm = col_perm_[static_cast<size_t>(index)];
+  out->col_value.resize(static_cast<size_t>(d.cols));
   out->col_dual.resize(static_cast<size_t>(d.cols));
-  for (int32_t j = 0; j < d.cols; ++j) out->col_dual[static_cast<size_t>(j)] = sense * red_[c0 + static_cast<size_t>(j)];
+  for (int32_t j = 0; j < d.cols; ++j) {
+    const auto o = static_cast<size_t>(perm.empty() ? j : perm[static_cast<size_t>(j)]);
+    out->col_value[o] = bestx_[c0 + static_cast<size_t>(j)];
+    out->col_dual[o] = sense * red_[c0 + static_cast<size_t>(j)];
+  }
   out->row_dual.resize(static_cast<size_t>(d.rows));
   for (int32_t i = 0; i < d.rows; ++i) out->row_dual[static_cast<size_t>(i)] = sense * (-besty_[r0 + static_cast<size_t>(i)]);
   out->row_activity.assign(act_.begin() + static_cast<std::ptrdiff_t>(r0), act_.begin() + static_cast<std::ptrdiff_t>(r0 + static_cast<size_t>(d.rows)));

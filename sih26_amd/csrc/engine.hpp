@@ -32,6 +32,7 @@ struct EngineOptions {
   int32_t stage = -1;             // staging order of SpMV tiles: -1 default (1), 0 CSR, 1 sorted
   int32_t tile_nnz = 0;           // row-block capacity; 0 picks per instance
   int32_t streams = 0;            // concurrent sub-batches: 0 automatic, 1 one stream, n >= 2
+  int32_t column_order = 0;       // 0 automatic, 1 most-referenced columns first, 2 the LP's order
 };
 
 struct InstanceResult {

@@ -178,3 +178,48 @@ def test_split_batch_matches_one_stream(gpu):
     ms_one, bytes_one = one.time_kernel(1, 4)
     # the same bytes, up to the one extra row pointer each sub-batch's CSR holds
     assert ms_split > 0 and ms_one > 0 and 0 <= bytes_split - bytes_one <= 4 * 2 * 2
+
+
+def small_power_law(gpu, seed=5):
+    """A config-5-shaped LP (power-law column degrees, |a| in [0.1, 10]) at 2,000 x 5,000."""
+    return gpu.Lp.generate_params(**dict(gpu.Lp.preset(5, seed), num_rows=2000, num_cols=5000,
+                                         num_equality_rows=1200, target_nnz=25000))
+
+
+def test_relabelled_columns_return_the_lp_order(gpu, oracle_mod):
+    """Columns with the most nonzeros first on the device (column_order 1) against the LP's
+    own order (2): both answers, in the LP's order, meet 1e-4 relative KKT as the oracle
+    measures them on the original model, and agree in objective. The power-iteration start is
+    permuted with the columns, so the norm estimate agrees to rounding."""
+    lp = small_power_law(gpu)
+    d = lp.export()
+    deg = np.diff(d["col_starts"])
+    assert deg.max() > 4 * deg.mean()  # the relabelling is not the identity here
+    kw = dict(pdhg_tolerance=1e-4, pdhg_stop_at_request=True, pdhg_absolute_primal=False)
+    own = gpu.solve(lp, column_order=2, **kw)
+    hot = gpu.solve(lp, column_order=1, **kw)
+    for r in (own, hot):
+        assert r.converged, r.info["message"]
+        assert kkt_of(oracle_mod, d, r)["worst"] <= 1e-4 * 1.000001
+    assert abs(hot.objective - own.objective) <= 1e-3 * max(1.0, abs(own.objective))
+    own64 = gpu.solve(lp, column_order=2, pdhg_precision="fp64", iteration_limit=1)
+    hot64 = gpu.solve(lp, column_order=1, pdhg_precision="fp64", iteration_limit=1)
+    assert abs(hot64.spectral_norm - own64.spectral_norm) <= 1e-12 * own64.spectral_norm
+
+
+def test_relabelled_instance_in_a_batch_equals_its_solo_run(gpu):
+    kw = dict(pdhg_tolerance=1e-4, pdhg_stop_at_request=True, pdhg_absolute_primal=False,
+              column_order=1)
+    lps = [gpu.Lp.generate(3, 7), small_power_law(gpu, 6), gpu.Lp.generate(3, 8)]
+    b = gpu.Batch(**kw)
+    for lp in lps:
+        b.add(lp)
+    b.upload()
+    b.solve()
+    for k in range(3):
+        solo = gpu.solve(lps[k], **kw)
+        mixed = b.result(k)
+        assert solo.converged and solo.iterations == mixed.iterations
+        np.testing.assert_array_equal(solo.col_value, mixed.col_value)
+        np.testing.assert_array_equal(solo.col_dual, mixed.col_dual)
+        np.testing.assert_array_equal(solo.row_dual, mixed.row_dual)
