@@ -349,9 +349,18 @@ __device__ __forceinline__ void finish(InstState& s, int status, uint32_t* done_
   atomicAdd(done_count, 1u);
 }
 
-// The adaptive step-size rule and acceptance (pdhg.cpp:410-503).
+// The growth and shrink factors of the step-size rule at the instance's iteration count
+// (pdhg.cpp:472-474).
+__device__ __forceinline__ void step_rule_factors(const InstState& s, double* shrink, double* grow) {
+  const double exponent = static_cast<double>(s.iteration + 1 > 2 ? s.iteration + 1 : 2);
+  *shrink = 1.0 - pow(exponent, -0.3);
+  *grow = 1.0 + pow(exponent, -0.6);
+}
+
+// The adaptive step-size rule and acceptance (pdhg.cpp:410-503); shrink and grow from
+// step_rule_factors.
 __device__ void decide_step(InstState& s, double dx2, double dy2, double inter_sum, int has_rows,
-                            uint32_t* done_count) {
+                            double shrink, double grow, uint32_t* done_count) {
   const double omega = s.omega;
   const double movement = 0.5 * omega * dx2 + 0.5 * dy2 / omega;
   const double interaction = has_rows ? fabs(inter_sum) : 0.0;
@@ -367,9 +376,6 @@ __device__ void decide_step(InstState& s, double dx2, double dy2, double inter_s
   }
   const bool no_information = interaction <= 0.0;
   const double limit = no_information ? INFINITY : movement / interaction;
-  const double exponent = static_cast<double>(s.iteration + 1 > 2 ? s.iteration + 1 : 2);
-  const double shrink = 1.0 - pow(exponent, -0.3);
-  const double grow = 1.0 + pow(exponent, -0.6);
   const double proposed = dev_min(shrink * limit, grow * s.eta);
   if (s.eta <= limit) {
     s.parity ^= 1;
@@ -430,18 +436,24 @@ __global__ __launch_bounds__(kStepLanes) void k_step_aty(Args<T> a) {
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_step_finalize(Args<T> a) {
   __shared__ double red[3 * (kBlock / kWave)];
+  __shared__ double rule[2];
+  // the instance record, its tile ranges and the tiles' partials are loaded together (the
+  // partials of a stopped instance are read and dropped), so the chain is one round trip
   InstState& s = a.state[blockIdx.x];
-  if (!stepping(s)) return;
   const InstDesc d = a.desc[blockIdx.x];
+  const bool go = stepping(s);
   // ||x'-x||^2, ||y'-y||^2 and the interaction: lane sums over the tiles, then one reduction
   double v[3] = {0.0, 0.0, 0.0};
   for (int k = threadIdx.x; k < d.tc1 - d.tc0; k += kBlock) v[0] += a.ps_c[d.tc0 + k];
   for (int k = threadIdx.x; k < d.ta1 - d.ta0; k += kBlock) v[1] += a.ps_a[d.ta0 + k];
   for (int k = threadIdx.x; k < d.tt1 - d.tt0; k += kBlock) v[2] += a.ps_t[d.tt0 + k];
+  if (!go) return;
+  // the rule's powers in the second wavefront, beside the reduction
+  if (threadIdx.x == kWave) step_rule_factors(s, &rule[0], &rule[1]);
   block_reduce_n<3, 0u>(v, red);
   __shared__ uint32_t flags;
   if (threadIdx.x == 0) {
-    decide_step(s, v[0], v[1], v[2], d.rows > 0, a.done_count);
+    decide_step(s, v[0], v[1], v[2], d.rows > 0, rule[0], rule[1], a.done_count);
     flags = step_flags(s);
   }
   __syncthreads();
