@@ -164,23 +164,38 @@ __global__ __launch_bounds__(kBlock) void k_step_primal(Args<T> a) {
   const int p = (f & kFlagParity) ? 1 : 0;
   const bool acc = a.force || (f & kFlagAccept);
   const InstState& s = a.state[t.inst];
-  const T tau = static_cast<T>(s.eta / s.omega);
+  const double eta = s.eta, omega = s.omega;
   const T* __restrict__ x = p ? a.x1 : a.x0;
   T* __restrict__ xn = p ? a.x0 : a.x1;
   const T* __restrict__ aty = p ? a.aty1 : a.aty0;
-  double d2 = 0.0;
+  // every operand load is issued before the step size (which waits for the instance record)
+  // is used
   constexpr int U = kChunk / kBlock;
+  T xj[U], c[U], at[U], lo[U], hi[U], sm[U];
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const int j = t.begin + u * kBlock + threadIdx.x;
+    if (j < t.end) {
+      xj[u] = x[j];
+      c[u] = a.chat[j];
+      at[u] = aty[j];
+      lo[u] = a.lhat[j];
+      hi[u] = a.uhat[j];
+      if (acc) sm[u] = a.xsum[j];
+    }
+  }
+  const T tau = static_cast<T>(eta / omega);
+  double d2 = 0.0;
 #pragma unroll
   for (int u = 0; u < U; ++u) {
     const int j = t.begin + u * kBlock + threadIdx.x;
     if (j >= t.end) continue;
-    const T xj = x[j];
-    if (acc) a.xsum[j] += xj;
-    const T g = a.chat[j] + aty[j];
-    const T v = dev_project<T>(xj - tau * g, a.lhat[j], a.uhat[j]);
+    if (acc) a.xsum[j] = sm[u] + xj[u];
+    const T g = c[u] + at[u];
+    const T v = dev_project<T>(xj[u] - tau * g, lo[u], hi[u]);
     xn[j] = v;
-    a.xbar[j] = static_cast<T>(2) * v - xj;
-    const double d = static_cast<double>(v - xj);
+    a.xbar[j] = static_cast<T>(2) * v - xj[u];
+    const double d = static_cast<double>(v - xj[u]);
     d2 += d * d;
   }
   d2 = block_sum(d2, red);
