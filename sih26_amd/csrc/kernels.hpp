@@ -317,8 +317,10 @@ __global__ __launch_bounds__(kStepLanes) void k_step_dual(Args<T> a) {
   if (!a.force && !(f & kFlagStepping)) return;
   const int p = (f & kFlagParity) ? 1 : 0;
   const bool acc = a.force || (f & kFlagAccept);
+  // the step size is formed in the epilogue, so waiting for the instance record does not
+  // hold back the staging loads
   const InstState& s = a.state[t.inst];
-  const T sigma = static_cast<T>(s.eta * s.omega);
+  const double eta = s.eta, omega = s.omega;
   const T* __restrict__ y = p ? a.y1 : a.y0;
   T* __restrict__ yn = p ? a.y0 : a.y1;
   const T* __restrict__ xbar = a.xbar;
@@ -336,6 +338,7 @@ __global__ __launch_bounds__(kStepLanes) void k_step_dual(Args<T> a) {
         if (acc) sum[k] = a.ysum[i];
       },
       [&](int i, int k, T r) {
+        const T sigma = static_cast<T>(eta * omega);
         if (acc) a.ysum[i] = sum[k] + yi[k];
         const T v = yi[k] + sigma * r;
         const T ynew = v - sigma * dev_project<T>(v / sigma, lo[k], hi[k]);
