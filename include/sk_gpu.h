@@ -199,11 +199,11 @@ typedef struct {
   int32_t streams;           /* a batch runs as this many sub-batches side by side, each on
                                 its own stream: 0 automatic (3 for batches of 2^20 nonzeros and
                                 more, else 1), 1 one block-diagonal launch per kernel */
-  int32_t column_order;      /* device column order: 0 automatic (default: an instance whose
-                                iterate vector exceeds 4 MiB puts its 2 MiB of columns with the
-                                most nonzeros first, so the most-gathered part of x stays in
-                                L2), 1 always so, 2 the LP's own order. Answers are returned in
-                                the LP's order either way. */
+  int32_t column_order;      /* device column order: 0 automatic (default: an instance of at
+                                least 65,536 columns puts its 2 MiB of columns with the most
+                                nonzeros first, so the most-gathered part of x stays in L2),
+                                1 every instance so, 2 the LP's own order. Answers are returned
+                                in the LP's order either way. */
 } sk_pdhg_options;
 
 void sk_pdhg_default_options(sk_pdhg_options* options);
