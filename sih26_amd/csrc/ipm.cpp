@@ -452,16 +452,6 @@ int Method::run(IpmOutcome* out) {
   const auto t0 = Clock::now();
   auto elapsed = [&] { return std::chrono::duration<double>(Clock::now() - t0).count(); };
   build();
-  // the polish declines a factor above its cap before anything is built (ipm.cpp:509-518);
-  // the dense factor holds m (m + 1) / 2 nonzeros
-  const int64_t factor_nonzeros = static_cast<int64_t>(m_) * (m_ + 1) / 2;
-  if (warm_ && opt_.max_factor_nonzeros >= 0 && factor_nonzeros > opt_.max_factor_nonzeros) {
-    finish(kStNotSolved,
-           format("declined: the factor would hold %lld nonzeros, above polish_max_factor_nonzeros = %lld",
-                  static_cast<long long>(factor_nonzeros), static_cast<long long>(opt_.max_factor_nonzeros)),
-           out);
-    return kStatusSuccess;
-  }
   int st = ne_.init(m_, n_, p_.rs, p_.ci, p_.rv, p_.cs, p_.ri, p_.cv);
   if (st) return st;
   int64_t iterations = 0;
@@ -637,17 +627,29 @@ void equilibrate(const HostLp& lp, int32_t ruiz_iterations, std::vector<double>*
 // ipm.cpp:897-988 (solve_scaled): scale, map the warm start in (x / Dc, y / Dr, d * Dc), run,
 // map the answer back (x * Dc, y * Dr, d / Dc)
 int solve_ipm(const HostLp& lp, const IpmOptions& options, const WarmPoint* warm, IpmOutcome* out) {
+  const size_t n = static_cast<size_t>(lp.num_cols), m = static_cast<size_t>(lp.num_rows);
+  // a non-finite start is ignored and the solve is cold (ipm.cpp:388-402)
+  bool finite = warm && warm->col_value && warm->row_dual && warm->col_dual;
+  for (size_t j = 0; j < n && finite; ++j)
+    finite = std::isfinite(warm->col_value[j]) && std::isfinite(warm->col_dual[j]);
+  for (size_t i = 0; i < m && finite; ++i) finite = std::isfinite(warm->row_dual[i]);
+  // the polish declines a factor above its cap before anything is built, scaling included
+  // (ipm.cpp:509-518); the dense factor holds m (m + 1) / 2 nonzeros
+  const int64_t factor_nonzeros = static_cast<int64_t>(m) * static_cast<int64_t>(m + 1) / 2;
+  if (finite && options.max_factor_nonzeros >= 0 && factor_nonzeros > options.max_factor_nonzeros) {
+    *out = IpmOutcome{};
+    out->status = kStNotSolved;
+    out->message = format("declined: the factor would hold %lld nonzeros, above polish_max_factor_nonzeros = %lld",
+                          static_cast<long long>(factor_nonzeros),
+                          static_cast<long long>(options.max_factor_nonzeros));
+    return kStatusSuccess;
+  }
   const Problem p = make_problem(lp, options.scaling != 0);
-  const size_t n = static_cast<size_t>(p.n), m = static_cast<size_t>(p.m);
   std::vector<double> wx, wy, wd;
   WarmPoint scaled_warm;
   const WarmPoint* use = nullptr;
-  if (warm && warm->col_value && warm->row_dual && warm->col_dual) {
-    bool finite = true;
-    for (size_t j = 0; j < n && finite; ++j)
-      finite = std::isfinite(warm->col_value[j]) && std::isfinite(warm->col_dual[j]);
-    for (size_t i = 0; i < m && finite; ++i) finite = std::isfinite(warm->row_dual[i]);
-    if (finite) {  // a non-finite start is ignored and the solve is cold (ipm.cpp:388-402)
+  {
+    if (finite) {
       wx.resize(n);
       wd.resize(n);
       wy.resize(m);
