@@ -127,7 +127,8 @@ int32_t sk_lp_create(int32_t num_rows, int32_t num_cols, int32_t sense, double o
                      const double* values, const double* col_cost, const double* col_lower,
                      const double* col_upper, const double* row_lower, const double* row_upper,
                      sk_lp** out);
-/* Seeded generator. config 1..5 are the five workloads: 1 = 500x1000 blending LP (seed =
+/* Seeded generator (no counterpart in the library; the nearest in the reference is the Python
+ * script bench/runners/generate_large_lp.py:49-206). config 1..5 are the five workloads: 1 = 500x1000 blending LP (seed =
  * instance), 2 = 200k x 500k power-law LP with 2.5M nnz (seed = instance), 3 = member
  * `instance` of the 256-LP batch, 4 = serving request `instance`, 5 = 2M x 5M power-law LP
  * with 25M nnz (seed = instance). */
@@ -152,9 +153,11 @@ typedef struct {
   double cost_lo, cost_hi;
   int32_t fix_empty_rows;
 } sk_gen_params;
+/* The generator's parameters for a workload member, and a generation from any parameters. */
 int32_t sk_gen_preset(int32_t config, uint64_t instance, sk_gen_params* out);
 int32_t sk_lp_generate_params(const sk_gen_params* params, sk_lp** out);
 
+/* The Model's sizes and arrays (model.hpp:131-225). */
 int32_t sk_lp_dims(const sk_lp* lp, int32_t* num_rows, int32_t* num_cols, int64_t* nnz);
 /* Copies the model out; every pointer may be NULL to skip that array. */
 int32_t sk_lp_export(const sk_lp* lp, int32_t* sense, double* objective_offset,
@@ -172,7 +175,8 @@ void sk_lp_free(sk_lp* lp);
  * marks integer; an LP solve relaxes them. */
 int32_t sk_lp_read_mps(const char* path, sk_lp** out, int32_t* integer_columns);
 /* The name of row (kind 0) or column (kind 1) `index` of an LP read from a file, or of the
- * model (kind 2, index ignored); NULL otherwise. Valid until sk_lp_free. */
+ * model (kind 2, index ignored); NULL otherwise (Model::row_names, col_names, name:
+ * model.hpp:135-173). Valid until sk_lp_free. */
 const char* sk_lp_name(const sk_lp* lp, int32_t kind, int32_t index);
 
 /* ---- PDHG options (the options solve_pdhg reads: pdhg.cpp:304-315) ------------------------ */
@@ -206,6 +210,7 @@ typedef struct {
                                 in the LP's order either way. */
 } sk_pdhg_options;
 
+/* The defaults listed above (options.cpp:83-106, 284-290, 328-334, 395-418). */
 void sk_pdhg_default_options(sk_pdhg_options* options);
 /* String-keyed setter over the reference's option names (src/util/options.cpp:78-455):
  * algorithm (must be "pdhg"), backend ("gpu"|"auto"), gpu, pdhg_tolerance, iteration_limit,
@@ -282,6 +287,7 @@ typedef struct {
   int32_t reserved;
 } sk_ipm_options;
 
+/* The defaults listed above (options.cpp:342-393, 183). */
 void sk_ipm_default_options(sk_ipm_options* options);
 /* ipm::solve_ipm (include/sankhya/ipm.hpp:17, 33-34): Mehrotra's predictor-corrector on the
  * normal equations, assembled and factorised (dense Cholesky, pivots floored at 1e-10) on the
@@ -306,7 +312,9 @@ int32_t sk_pdhg_polish(const sk_lp* lp, const sk_ipm_options* options,
                        sk_pdhg_result* result, double* col_value, double* row_dual,
                        double* col_dual, double* row_activity);
 
-/* ---- batches: many LPs of uneven size in one block-diagonal solve -------------------------- */
+/* ---- batches: many LPs of uneven size in one block-diagonal solve --------------------------
+ * No counterpart in the reference, which solves one LP per call (src/core/solve.cpp:385-399);
+ * every instance of a batch runs the engine of sk_pdhg_solve and gets its answer bit for bit. */
 typedef struct sk_batch sk_batch;
 
 typedef struct {
