@@ -174,6 +174,20 @@ void sk_lp_free(sk_lp* lp);
  * "path:line: reason". integer_columns (may be NULL) receives the number of columns the file
  * marks integer; an LP solve relaxes them. */
 int32_t sk_lp_read_mps(const char* path, sk_lp** out, int32_t* integer_columns);
+/* Free-format MPS of the LP, which read_mps (above, and the reference's, io.hpp:45-46) reads
+ * back to the same arrays: values with 17 significant digits, names from the file the LP was
+ * read from when they are blank-free and distinct, else R<i> / C<j>; a free row becomes an L
+ * row with right-hand side 1e30. Entries below 1e-11 in magnitude do not survive (readers drop
+ * them), and a two-sided row whose bound difference rounds comes back within one rounding of
+ * its range. No counterpart in the reference library: its generator writes MPS from Python
+ * (bench/runners/generate_large_lp.py:49-206) for the solver's CLI; this lets the reference's
+ * binary be run on the generated workloads. */
+int32_t sk_lp_write_mps(const sk_lp* lp, const char* path);
+/* Instance cache (no counterpart in the reference): the LP's arrays in host byte order after
+ * an 8-byte magic, with an FNV-1a checksum. sk_lp_load checks the size, the checksum and the
+ * model, and refuses a file that fails any of them with SK_GPU_INVALID_ARGUMENT. */
+int32_t sk_lp_save(const sk_lp* lp, const char* path);
+int32_t sk_lp_load(const char* path, sk_lp** out);
 /* The name of row (kind 0) or column (kind 1) `index` of an LP read from a file, or of the
  * model (kind 2, index ignored); NULL otherwise (Model::row_names, col_names, name:
  * model.hpp:135-173). Valid until sk_lp_free. */

@@ -124,6 +124,9 @@ SIGNATURES = {
     "sk_ipm_default_options": (None, [P(IpmOptions)]),
     "sk_lp_read_mps": (i32, [cstr, P(vp), P(i32)]),
     "sk_lp_name": (cstr, [vp, i32, i32]),
+    "sk_lp_write_mps": (i32, [vp, cstr]),
+    "sk_lp_save": (i32, [vp, cstr]),
+    "sk_lp_load": (i32, [cstr, P(vp)]),
     "sk_ipm_solve": (i32, [vp, P(IpmOptions), vp, vp, vp, P(PdhgResult), vp, vp, vp, vp]),
     "sk_pdhg_polish": (i32, [vp, P(IpmOptions), P(PdhgResult), vp, vp, vp, P(PdhgResult), vp, vp,
                              vp, vp]),

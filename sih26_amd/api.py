@@ -101,6 +101,21 @@ class Lp:
         lp.integer_columns = k.value
         return lp
 
+    def write_mps(self, path) -> None:
+        """Free-format MPS that read_mps gives back as the same arrays (sk_lp_write_mps)."""
+        check(lib().sk_lp_write_mps(self._h, str(path).encode()), "sk_lp_write_mps")
+
+    def save(self, path) -> None:
+        """The binary instance cache (sk_lp_save)."""
+        check(lib().sk_lp_save(self._h, str(path).encode()), "sk_lp_save")
+
+    @classmethod
+    def load(cls, path) -> "Lp":
+        """An LP from the binary instance cache (sk_lp_load)."""
+        h = C.c_void_p()
+        check(lib().sk_lp_load(str(path).encode(), C.byref(h)), "sk_lp_load")
+        return cls(h)
+
     def names(self) -> tuple[str, list[str], list[str]]:
         """(model name, row names, column names) of an LP read from a file."""
         m, n, _ = self.dims

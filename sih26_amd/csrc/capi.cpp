@@ -20,6 +20,7 @@
 #include "../../include/sk_gpu.h"
 #include "engine.hpp"
 #include "lp.hpp"
+#include "lpfile.hpp"
 #include "quality.hpp"
 #include "server.hpp"
 #include "status.hpp"
@@ -338,6 +339,45 @@ int32_t sk_lp_read_mps(const char* path, sk_lp** out, int32_t* integer_columns) 
       return SK_GPU_INVALID_ARGUMENT;
     }
     if (integer_columns) *integer_columns = h->file.integer_columns;
+    *out = h.release();
+    return SK_GPU_SUCCESS;
+  });
+}
+
+int32_t sk_lp_write_mps(const sk_lp* lp, const char* path) {
+  if (!lp || !path) return SK_GPU_INVALID_ARGUMENT;
+  return guarded([&]() -> int32_t {
+    std::string error;
+    if (!sk::write_mps(lp->lp, &lp->file, path, &error)) {
+      sk::set_last_error(error);
+      return SK_GPU_INVALID_ARGUMENT;
+    }
+    return SK_GPU_SUCCESS;
+  });
+}
+
+int32_t sk_lp_save(const sk_lp* lp, const char* path) {
+  if (!lp || !path) return SK_GPU_INVALID_ARGUMENT;
+  return guarded([&]() -> int32_t {
+    std::string error;
+    if (!sk::save_lp(lp->lp, path, &error)) {
+      sk::set_last_error(error);
+      return SK_GPU_INVALID_ARGUMENT;
+    }
+    return SK_GPU_SUCCESS;
+  });
+}
+
+int32_t sk_lp_load(const char* path, sk_lp** out) {
+  if (!path || !out) return SK_GPU_INVALID_ARGUMENT;
+  *out = nullptr;
+  return guarded([&]() -> int32_t {
+    auto h = std::make_unique<sk_lp>();
+    std::string error;
+    if (!sk::load_lp(path, &h->lp, &error)) {
+      sk::set_last_error(error);
+      return SK_GPU_INVALID_ARGUMENT;
+    }
     *out = h.release();
     return SK_GPU_SUCCESS;
   });
