@@ -291,7 +291,9 @@ def main() -> int:
     ap.add_argument("--kernel-reps", type=int, default=50)
     ap.add_argument("--spmv-stage", type=int, default=-1,
                     help="staging order: -1 default, 0 CSR order, 1 sorted by gather index")
-    ap.add_argument("--tile-nnz", type=int, default=0, help="row-block capacity, 0 auto")
+    ap.add_argument("--tile-nnz", type=int, default=-1,
+                    help="row-block capacity: -1 (default) picked by the autotuner at upload, "
+                         "0 the library default, n > 0 that capacity")
     ap.add_argument("--steps-per-block", type=int, default=40,
                     help="PDHG steps per captured graph")
     ap.add_argument("--streams", type=int, default=0,
@@ -301,7 +303,7 @@ def main() -> int:
     ap.add_argument("--requests", type=int, default=0,
                     help="serving stream length (0: 10000 for --workload c4, 1000 otherwise)")
     ap.add_argument("--rate", type=float, default=200.0, help="serving arrivals per second")
-    ap.add_argument("--slots", type=int, default=4, help="serving: concurrent batches")
+    ap.add_argument("--slots", type=int, default=2, help="serving: concurrent batches")
     ap.add_argument("--max-batch", type=int, default=256, help="serving: instances per batch")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--polish-sample", type=int, default=16,
@@ -356,7 +358,8 @@ def main() -> int:
     # dominant kernel: time every step kernel with HIP events on the engine's stream, over the
     # whole workload as one block-diagonal batch (one launch covers every instance)
     batch.close()
-    whole = sk.Batch(solve_options(sk, streams=1, **layout))
+    # (with the capacity the solve ran with, also when the autotuner picked it)
+    whole = sk.Batch(solve_options(sk, streams=1, **dict(layout, tile_nnz=stats["tile_nnz"])))
     for lp in lps:
         whole.add(lp)
     whole.upload()
@@ -391,7 +394,8 @@ def main() -> int:
         "data": "synthetic (seeded generator in sih26_amd/csrc/generator.cpp)",
         "config": {"workload": WORKLOADS[args.workload], "instances": n,
                    "rows": stats["total_rows"], "cols": stats["total_cols"],
-                   "nnz": stats["total_nnz"], "sub_batches": stats["streams"], "tolerance": 1e-4,
+                   "nnz": stats["total_nnz"], "sub_batches": stats["streams"],
+                   "tile_nnz": stats["tile_nnz"], "tolerance": 1e-4,
                    "stop": "relative KKT (primal, dual, gap) <= 1e-4, FP64 on the original model"},
         "pdhg_iters_per_s": iterations * args.steps / total,
         "solved": solved,

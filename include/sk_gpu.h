@@ -209,7 +209,10 @@ typedef struct {
   int32_t steps_per_block;   /* PDHG steps per captured graph, default 40 */
   int32_t use_graph;         /* 1: hipGraph-captured step blocks (default) */
   int32_t spmv_stage;        /* SpMV staging order: -1 default (1), 0 CSR order, 1 sorted by gather index */
-  int32_t tile_nnz;          /* row-block capacity in nonzeros, 0 = automatic */
+  int32_t tile_nnz;          /* row-block capacity in nonzeros: 0 the default (2,560), -1 picked
+                                by the autotuner at sk_batch_upload (each of 1,536, 2,048,
+                                2,560, 3,072 and 4,096 is uploaded and the two SpMV step
+                                kernels timed; the fastest stays), at most 4,096 */
   int32_t compute_quality;   /* 1: fill the recompute_quality measures (default) */
   int32_t eval_stride;       /* an evaluation stage follows every this many steps inside a
                                 captured block, so an instance due for evaluation waits at most
@@ -342,7 +345,8 @@ typedef struct {
   int32_t instances;
   int32_t spmv_stage;
   int32_t streams;  /* sub-batches the batch ran as (sk_pdhg_options.streams) */
-  int32_t reserved;
+  int32_t tile_nnz; /* row-block capacity the batch runs with (picked by the autotuner when
+                       sk_pdhg_options.tile_nnz is -1) */
 } sk_batch_stats;
 
 int32_t sk_batch_create(const sk_pdhg_options* options, sk_batch** out);
@@ -378,7 +382,9 @@ void sk_batch_destroy(sk_batch* batch);
 typedef struct sk_server sk_server;
 
 typedef struct {
-  int32_t slots;          /* concurrent batches (default 4) */
+  int32_t slots;          /* concurrent batches (default 2: each holds two streams, and
+                             more streams than the process's 4 hardware queues lengthen the
+                             latency tail) */
   int32_t max_batch;      /* instances per batch (default 256) */
   int64_t memory_budget;  /* device bytes all slots may hold; 0: 90% of free memory at create */
   int32_t keep_solutions; /* 1: keep the primal/dual vectors for sk_server_result (default 1) */

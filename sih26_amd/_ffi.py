@@ -70,7 +70,7 @@ class BatchStats(C.Structure):
     _fields_ = [("setup_ms", f64), ("loop_ms", f64), ("report_ms", f64),
                 ("graph_launches", i64), ("total_rows", i64), ("total_cols", i64),
                 ("total_nnz", i64), ("instances", i32), ("spmv_stage", i32),
-                ("streams", i32), ("reserved", i32)]
+                ("streams", i32), ("tile_nnz", i32)]
 
 
 class ServerOptions(C.Structure):

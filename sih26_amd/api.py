@@ -337,7 +337,7 @@ class Batch:
 class Server:
     """Continuous-batching request loop on the GPU (sk_server_* of include/sk_gpu.h)."""
 
-    def __init__(self, options: _ffi.PdhgOptions | None = None, *, slots: int = 4,
+    def __init__(self, options: _ffi.PdhgOptions | None = None, *, slots: int = 2,
                  max_batch: int = 256, memory_budget: int = 0, keep_solutions: bool = True,
                  **kw):
         self.options = options if options is not None else make_options(**kw)

@@ -31,6 +31,7 @@ struct sk_batch {
   sk_pdhg_options options;
   std::unique_ptr<sk::Engine> engine;
   std::vector<const sk_lp*> lps;
+  int32_t tile_nnz = 0;  // the row-block capacity the uploaded batch runs with
   bool uploaded = false;
   bool solved = false;
 };
@@ -530,10 +531,54 @@ int32_t sk_batch_add(sk_batch* b, const sk_lp* lp, int32_t* index) {
   });
 }
 
+namespace {
+
+// The capacity a batch uploaded with `tile_nnz` runs with (the engine's rule).
+int32_t capacity_for(int32_t tile_nnz) {
+  return tile_nnz > 0 ? std::min(tile_nnz, sk::kMaxTileNnz) : sk::kDefaultTileNnz;
+}
+
+// tile_nnz = -1: the batch is uploaded with each candidate capacity and the two SpMV step
+// kernels are timed with every instance stepping, as the solve schedules them (all sub-batch
+// streams at once); the engine with the least time of the two stays, the others are freed.
+// A capacity changes no row sum, only which tiles share a workgroup and a partial sum.
+int32_t autotune_upload(sk_batch* b) {
+  static const int32_t kCandidates[] = {1536, 2048, 2560, 3072, 4096};
+  constexpr int kReps = 10;
+  std::unique_ptr<sk::Engine> best;
+  double best_ms = INFINITY;
+  for (const int32_t cap : kCandidates) {
+    sk_pdhg_options o = b->options;
+    o.tile_nnz = cap;
+    auto e = sk::make_engine(to_engine(o));
+    for (const sk_lp* lp : b->lps) e->add(&lp->lp);
+    int32_t st = e->upload();
+    double dual_ms = 0.0, aty_ms = 0.0;
+    if (!st) st = e->time_kernel(1, kReps, &dual_ms);
+    if (!st) st = e->time_kernel(2, kReps, &aty_ms);
+    if (st) return st;
+    if (dual_ms + aty_ms < best_ms) {
+      best_ms = dual_ms + aty_ms;
+      best = std::move(e);
+      b->tile_nnz = cap;
+    }
+  }
+  b->engine = std::move(best);
+  return SK_GPU_SUCCESS;
+}
+
+}  // namespace
+
 int32_t sk_batch_upload(sk_batch* b) {
   if (!b) return SK_GPU_INVALID_ARGUMENT;
   return guarded([&]() -> int32_t {
-    const int32_t st = b->engine->upload();
+    int32_t st;
+    if (b->options.tile_nnz == -1) {
+      st = autotune_upload(b);
+    } else {
+      st = b->engine->upload();
+      b->tile_nnz = capacity_for(b->options.tile_nnz);
+    }
     if (!st) b->uploaded = true;
     return st;
   });
@@ -582,7 +627,7 @@ int32_t sk_batch_stats_get(const sk_batch* b, sk_batch_stats* out) {
   out->instances = b->engine->size();
   out->spmv_stage = b->engine->stage_choice();
   out->streams = b->engine->streams();
-  out->reserved = 0;
+  out->tile_nnz = b->tile_nnz;
   return SK_GPU_SUCCESS;
 }
 
@@ -664,7 +709,9 @@ extern "C" {
 void sk_server_default_options(sk_server_options* out) {
   if (!out) return;
   std::memset(out, 0, sizeof(*out));
-  out->slots = 4;
+  // two slots: each engine holds a compute and a read-back stream, and more than the
+  // process's four hardware queues' worth of streams doubled the p99 latency in measurement
+  out->slots = 2;
   out->max_batch = 256;
   out->memory_budget = 0;
   out->keep_solutions = 1;

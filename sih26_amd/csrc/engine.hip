@@ -88,12 +88,6 @@ void build_row_tiles(const std::vector<int32_t>& ptr, int32_t row0, int32_t rows
   if (out->size() == before) emit(row0, row0, kTileShort);
 }
 
-// Default row-block tile capacity. A step launch over a batch runs its tiles as a few rounds
-// of the workgroups the device holds at once (2,048: 8 per CU); measured on the 256-LP batch,
-// 2,560 nonzeros per tile gives ~1.9 rounds and step launches ~8% shorter than 2,048 (~2.4
-// rounds, a long partly-occupied last round).
-constexpr int32_t kDefaultTileNnz = 2560;
-
 // Column order on the device. For an instance of at least kRelabelColumns columns (mode 0) or
 // any instance (mode 1), its kHotBytes / w columns with the most nonzeros (the x-bar entries
 // that the gathers of A x-bar touch most often) come first, largest first with ties in index

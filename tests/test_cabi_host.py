@@ -158,7 +158,7 @@ def test_power_law_preset_hits_exact_nnz():
 def test_server_options_and_missing_device():
     so = _ffi.ServerOptions()
     sk.lib().sk_server_default_options(C.byref(so))
-    assert (so.slots, so.max_batch, so.memory_budget, so.keep_solutions) == (4, 256, 0, 1)
+    assert (so.slots, so.max_batch, so.memory_budget, so.keep_solutions) == (2, 256, 0, 1)
     if sk.gpu_available():
         pytest.skip("a device is visible; the missing-device path is for CPU-only hosts")
     h = C.c_void_p()

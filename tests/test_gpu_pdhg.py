@@ -223,3 +223,28 @@ def test_relabelled_instance_in_a_batch_equals_its_solo_run(gpu):
         np.testing.assert_array_equal(solo.col_value, mixed.col_value)
         np.testing.assert_array_equal(solo.col_dual, mixed.col_dual)
         np.testing.assert_array_equal(solo.row_dual, mixed.row_dual)
+
+
+def test_autotuned_batch_runs_with_the_capacity_it_picked(gpu, oracle_mod):
+    """tile_nnz = -1: the upload picks one of the candidate capacities by timing; the batch
+    then runs exactly as one uploaded with that capacity, and every answer meets 1e-4."""
+    lps = [gpu.Lp.generate(3, k) for k in range(40, 56)]
+    kw = dict(pdhg_tolerance=1e-4, pdhg_stop_at_request=True, pdhg_absolute_primal=False)
+    tuned = gpu.Batch(tile_nnz=-1, **kw)
+    for lp in lps:
+        tuned.add(lp)
+    tuned.upload()
+    tuned.solve()
+    cap = tuned.stats()["tile_nnz"]
+    assert cap in (1536, 2048, 2560, 3072, 4096)
+    fixed = gpu.Batch(tile_nnz=cap, **kw)
+    for lp in lps:
+        fixed.add(lp)
+    fixed.upload()
+    fixed.solve()
+    assert fixed.stats()["tile_nnz"] == cap
+    for k, lp in enumerate(lps):
+        a, c = tuned.result(k), fixed.result(k)
+        assert a.converged and a.iterations == c.iterations
+        np.testing.assert_array_equal(a.col_value, c.col_value)
+        assert kkt_of(oracle_mod, lp.export(), a)["worst"] <= 1e-4 * 1.000001
