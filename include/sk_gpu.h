@@ -225,6 +225,11 @@ typedef struct {
                                 nonzeros first, so the most-gathered part of x stays in L2),
                                 1 every instance so, 2 the LP's own order. Answers are returned
                                 in the LP's order either way. */
+  int32_t step_fusion;       /* 0 automatic (default): when every instance has at most 4 SpMV
+                                tiles and 2 column chunks, one workgroup per instance runs a
+                                whole PDHG step in one launch, with the same per-tile code and
+                                bit for bit the same answers as the four launches; 1 never */
+  int32_t reserved2;
 } sk_pdhg_options;
 
 /* The defaults listed above (options.cpp:83-106, 284-290, 328-334, 395-418). */
@@ -347,6 +352,8 @@ typedef struct {
   int32_t streams;  /* sub-batches the batch ran as (sk_pdhg_options.streams) */
   int32_t tile_nnz; /* row-block capacity the batch runs with (picked by the autotuner when
                        sk_pdhg_options.tile_nnz is -1) */
+  int32_t fused;    /* 1 when a PDHG step runs as one launch (sk_pdhg_options.step_fusion) */
+  int32_t reserved;
 } sk_batch_stats;
 
 int32_t sk_batch_create(const sk_pdhg_options* options, sk_batch** out);

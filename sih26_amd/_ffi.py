@@ -44,7 +44,7 @@ class PdhgOptions(C.Structure):
                 ("random_seed", i64), ("precision", i32), ("steps_per_block", i32),
                 ("use_graph", i32), ("spmv_stage", i32), ("tile_nnz", i32),
                 ("compute_quality", i32), ("eval_stride", i32), ("streams", i32),
-                ("column_order", i32)]
+                ("column_order", i32), ("step_fusion", i32), ("reserved2", i32)]
 
 
 class PdhgResult(C.Structure):
@@ -70,7 +70,8 @@ class BatchStats(C.Structure):
     _fields_ = [("setup_ms", f64), ("loop_ms", f64), ("report_ms", f64),
                 ("graph_launches", i64), ("total_rows", i64), ("total_cols", i64),
                 ("total_nnz", i64), ("instances", i32), ("spmv_stage", i32),
-                ("streams", i32), ("tile_nnz", i32)]
+                ("streams", i32), ("tile_nnz", i32),
+                ("fused", i32), ("reserved", i32)]
 
 
 class ServerOptions(C.Structure):

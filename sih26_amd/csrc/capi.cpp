@@ -71,6 +71,7 @@ sk::EngineOptions to_engine(const sk_pdhg_options& o) {
   e.tile_nnz = o.tile_nnz;
   e.streams = o.streams;
   e.column_order = o.column_order == 1 || o.column_order == 2 ? o.column_order : 0;
+  e.step_fusion = o.step_fusion == 1 ? 1 : 0;
   return e;
 }
 
@@ -628,6 +629,8 @@ int32_t sk_batch_stats_get(const sk_batch* b, sk_batch_stats* out) {
   out->spmv_stage = b->engine->stage_choice();
   out->streams = b->engine->streams();
   out->tile_nnz = b->tile_nnz;
+  out->fused = b->engine->fused() ? 1 : 0;
+  out->reserved = 0;
   return SK_GPU_SUCCESS;
 }
 

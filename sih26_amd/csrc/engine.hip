@@ -88,6 +88,11 @@ void build_row_tiles(const std::vector<int32_t>& ptr, int32_t row0, int32_t rows
   if (out->size() == before) emit(row0, row0, kTileShort);
 }
 
+// Step fusion (k_step_fused) applies when every instance has at most this many SpMV tiles
+// (A and A^T together) and column chunks.
+constexpr int32_t kFusedMaxTiles = 4;
+constexpr int32_t kFusedMaxChunks = 2;
+
 // Column order on the device. For an instance of at least kRelabelColumns columns (mode 0) or
 // any instance (mode 1), its kHotBytes / w columns with the most nonzeros (the x-bar entries
 // that the gathers of A x-bar touch most often) come first, largest first with ties in index
@@ -153,6 +158,7 @@ class EngineImpl final : public Engine {
   int64_t total_cols() const override { return n_; }
   int stage_choice() const override { return stage_; }
   int32_t streams() const override { return 1; }
+  bool fused() const override { return fused_; }
 
   int upload() override;
   int solve() override;
@@ -234,6 +240,7 @@ class EngineImpl final : public Engine {
   int32_t K_ = 0;
   int stage_ = 1;
   int32_t cap_ = kDefaultTileNnz;  // row-block tile capacity (nonzeros)
+  bool fused_ = false;  // one launch per step (k_step_fused): every instance has few tiles
   bool pack_ = true;    // packed index words in the staging order
   int32_t cus_ = 256;   // compute units of the device
 
@@ -457,6 +464,11 @@ int EngineImpl<T>::upload() {
   }
   n_ta_ = static_cast<int32_t>(ta.size());
   n_tt_ = static_cast<int32_t>(tt.size());
+  // A step of an instance with a few tiles is a chain of short dependent launches; one
+  // workgroup then runs its tiles one after another in a single launch
+  fused_ = opt_.step_fusion == 0 && K_ > 0;
+  for (const InstDesc& d : desc_)
+    fused_ = fused_ && (d.ta1 - d.ta0) + (d.tt1 - d.tt0) <= kFusedMaxTiles && d.tc1 - d.tc0 <= kFusedMaxChunks;
   n_tc_ = static_cast<int32_t>(tc.size());
   n_tr_ = static_cast<int32_t>(tr.size());
 
@@ -779,11 +791,21 @@ int EngineImpl<T>::enqueue_block() {
   const Args<T>& a = args_;
   hipStream_t s = stream_;
   for (int k = 0; k < opt_.steps_per_block; ++k) {
-    SK_LAUNCH(k_step_primal<T><<<n_tc_, kBlock, 0, s>>>(a));
-    int st = launch_spmv(1, a);
-    if (st) return st;
-    if ((st = launch_spmv(2, a))) return st;
-    SK_LAUNCH(k_step_finalize<T><<<K_, kBlock, 0, s>>>(a));
+    if (fused_) {
+      const int st = with_layout([&](auto cap, auto pack) -> int {
+        constexpr int C = decltype(cap)::value;
+        constexpr bool P = decltype(pack)::value;
+        SK_LAUNCH(k_step_fused<T, C, P><<<K_, kBlock, 0, s>>>(a));
+        return kStatusSuccess;
+      });
+      if (st) return st;
+    } else {
+      SK_LAUNCH(k_step_primal<T><<<n_tc_, kBlock, 0, s>>>(a));
+      int st = launch_spmv(1, a);
+      if (st) return st;
+      if ((st = launch_spmv(2, a))) return st;
+      SK_LAUNCH(k_step_finalize<T><<<K_, kBlock, 0, s>>>(a));
+    }
     // an instance that became due for evaluation waits at most eval_stride steps
     if ((k + 1) % opt_.eval_stride == 0 || k + 1 == opt_.steps_per_block) {
       const int st = enqueue_eval(0);

@@ -33,6 +33,7 @@ struct EngineOptions {
   int32_t tile_nnz = 0;           // row-block capacity; 0 picks per instance
   int32_t streams = 0;            // concurrent sub-batches: 0 automatic, 1 one stream, n >= 2
   int32_t column_order = 0;       // 0 automatic, 1 most-referenced columns first, 2 the LP's order
+  int32_t step_fusion = 0;        // 0 automatic (one launch per step for small instances), 1 never
 };
 
 struct InstanceResult {
@@ -93,6 +94,8 @@ class Engine {
   virtual int stage_choice() const = 0;
   // Sub-batches the batch runs as (each on its own stream).
   virtual int32_t streams() const = 0;
+  // whether the uploaded batch runs a PDHG step as one launch (k_step_fused)
+  virtual bool fused() const = 0;
 };
 
 // An engine for a batch: with options.streams other than 1 the batch is split into

@@ -154,10 +154,9 @@ __global__ __launch_bounds__(kBlock) void k_publish_flags(Args<T> a) {
 
 // ---- step kernels ---------------------------------------------------------------------------
 
+// The primal step of column chunk `tile` (pdhg.cpp:393-400) and its ||x'-x||^2 partial.
 template <class T>
-__global__ __launch_bounds__(kBlock) void k_step_primal(Args<T> a) {
-  __shared__ double red[kBlock / kWave];
-  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+__device__ __forceinline__ void primal_chunk(const Args<T>& a, const int tile, double* red) {
   const Tile t = a.tc[tile];
   const uint32_t f = a.fc[tile];
   if (!a.force && !(f & kFlagStepping)) return;
@@ -200,6 +199,12 @@ __global__ __launch_bounds__(kBlock) void k_step_primal(Args<T> a) {
   }
   d2 = block_sum(d2, red);
   if (threadIdx.x == 0) a.ps_c[tile] = d2;
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_step_primal(Args<T> a) {
+  __shared__ double red[kBlock / kWave];
+  primal_chunk(a, xcd_tile(blockIdx.x, gridDim.x), red);
 }
 
 // Gather index of staging position q: a packed word holds (local index << 12 | CSR slot),
@@ -307,11 +312,10 @@ __device__ __forceinline__ auto slot_stager(const int32_t* sidx, const uint16_t*
 constexpr int kStepLanes = kBlock;
 constexpr int kStepRows = kMaxTileRows / kStepLanes;
 
-template <class T, int CAP, bool PACK>
-__global__ __launch_bounds__(kStepLanes) void k_step_dual(Args<T> a) {
-  __shared__ T stage[CAP];
-  __shared__ double red[kStepLanes / kWave];
-  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+// A x-bar and the dual step on the rows of tile `tile` of A (pdhg.cpp:402-408), with its
+// ||y'-y||^2 partial.
+template <class T, bool PACK>
+__device__ __forceinline__ void dual_tile(const Args<T>& a, const int tile, T* stage, double* red) {
   const Tile t = a.ta[tile];
   const uint32_t f = a.fa[tile];
   if (!a.force && !(f & kFlagStepping)) return;
@@ -349,6 +353,13 @@ __global__ __launch_bounds__(kStepLanes) void k_step_dual(Args<T> a) {
       stage);
   d2 = block_sum(d2, red);
   if (threadIdx.x == 0) a.ps_a[tile] = d2;
+}
+
+template <class T, int CAP, bool PACK>
+__global__ __launch_bounds__(kStepLanes) void k_step_dual(Args<T> a) {
+  __shared__ T stage[CAP];
+  __shared__ double red[kStepLanes / kWave];
+  dual_tile<T, PACK>(a, xcd_tile(blockIdx.x, gridDim.x), stage, red);
 }
 
 // Reduce `count` per-tile partials starting at tile `base` with the whole workgroup in a
@@ -413,11 +424,9 @@ __device__ void decide_step(InstState& s, double dx2, double dy2, double inter_s
   }
 }
 
-template <class T, int CAP, bool PACK>
-__global__ __launch_bounds__(kStepLanes) void k_step_aty(Args<T> a) {
-  __shared__ T stage[CAP];
-  __shared__ double red[kStepLanes / kWave];
-  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+// A^T y' on the rows of tile `tile` of A^T and its share of the interaction (pdhg.cpp:410-438).
+template <class T, bool PACK>
+__device__ __forceinline__ void aty_tile(const Args<T>& a, const int tile, T* stage, double* red) {
   const Tile t = a.tt[tile];
   const uint32_t f = a.ft[tile];
   if (!a.force && !(f & kFlagStepping)) return;
@@ -448,17 +457,24 @@ __global__ __launch_bounds__(kStepLanes) void k_step_aty(Args<T> a) {
   if (threadIdx.x == 0) a.ps_t[tile] = inter;
 }
 
+template <class T, int CAP, bool PACK>
+__global__ __launch_bounds__(kStepLanes) void k_step_aty(Args<T> a) {
+  __shared__ T stage[CAP];
+  __shared__ double red[kStepLanes / kWave];
+  aty_tile<T, PACK>(a, xcd_tile(blockIdx.x, gridDim.x), stage, red);
+}
+
 // One workgroup per instance: reduce the step partials in tile order and decide the step.
 // A separate launch (rather than a last-arriving tile) makes the partials visible across
 // XCDs through the kernel boundary, with no per-tile device-scope fence.
+// `red` holds 3 * (kBlock / kWave) values, `rule` 2; `flags` is shared by the workgroup.
 template <class T>
-__global__ __launch_bounds__(kBlock) void k_step_finalize(Args<T> a) {
-  __shared__ double red[3 * (kBlock / kWave)];
-  __shared__ double rule[2];
+__device__ __forceinline__ void finalize_instance(const Args<T>& a, const int inst, double* red,
+                                                  double* rule, uint32_t* flags) {
   // the instance record, its tile ranges and the tiles' partials are loaded together (the
   // partials of a stopped instance are read and dropped), so the chain is one round trip
-  InstState& s = a.state[blockIdx.x];
-  const InstDesc d = a.desc[blockIdx.x];
+  InstState& s = a.state[inst];
+  const InstDesc d = a.desc[inst];
   const bool go = stepping(s);
   // ||x'-x||^2, ||y'-y||^2 and the interaction: lane sums over the tiles, then one reduction
   double v[3] = {0.0, 0.0, 0.0};
@@ -469,13 +485,42 @@ __global__ __launch_bounds__(kBlock) void k_step_finalize(Args<T> a) {
   // the rule's powers in the second wavefront, beside the reduction
   if (threadIdx.x == kWave) step_rule_factors(s, &rule[0], &rule[1]);
   block_reduce_n<3, 0u>(v, red);
-  __shared__ uint32_t flags;
   if (threadIdx.x == 0) {
     decide_step(s, v[0], v[1], v[2], d.rows > 0, rule[0], rule[1], a.done_count);
-    flags = step_flags(s);
+    *flags = step_flags(s);
   }
   __syncthreads();
-  publish_flags(a, d, flags);
+  publish_flags(a, d, *flags);
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_step_finalize(Args<T> a) {
+  __shared__ double red[3 * (kBlock / kWave)];
+  __shared__ double rule[2];
+  __shared__ uint32_t flags;
+  finalize_instance(a, blockIdx.x, red, rule, &flags);
+}
+
+// One workgroup per instance runs a whole step: the primal step of its column chunks, then
+// A x-bar and the dual step on its tiles of A, then A^T y' on its tiles of A^T, then the
+// decision, each with the per-tile code of the four step kernels above and tile after tile in
+// index order, with a barrier between the phases. The partials and so the answers are bit for
+// bit those of the four launches; for instances with only a few tiles the step is one launch
+// instead of four.
+template <class T, int CAP, bool PACK>
+__global__ __launch_bounds__(kBlock) void k_step_fused(Args<T> a) {
+  __shared__ T stage[CAP];
+  __shared__ double red[3 * (kBlock / kWave)];
+  __shared__ double rule[2];
+  __shared__ uint32_t flags;
+  const InstDesc d = a.desc[blockIdx.x];
+  for (int k = d.tc0; k < d.tc1; ++k) primal_chunk(a, k, red);
+  __syncthreads();
+  for (int k = d.ta0; k < d.ta1; ++k) dual_tile<T, PACK>(a, k, stage, red);
+  __syncthreads();
+  for (int k = d.tt0; k < d.tt1; ++k) aty_tile<T, PACK>(a, k, stage, red);
+  __syncthreads();
+  finalize_instance(a, blockIdx.x, red, rule, &flags);
 }
 
 // ---- evaluation kernels (FP64) ---------------------------------------------------------------

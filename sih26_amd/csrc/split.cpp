@@ -220,6 +220,11 @@ class SplitEngine final : public Engine {
   int64_t total_cols() const override { return sum(&Engine::total_cols); }
   int stage_choice() const override { return parts_.empty() ? 1 : parts_[0]->stage_choice(); }
   int32_t streams() const override { return static_cast<int32_t>(parts_.size()); }
+  bool fused() const override {
+    bool all = !parts_.empty();
+    for (const auto& e : parts_) all = all && e->fused();
+    return all;
+  }
 
  private:
   struct Place {

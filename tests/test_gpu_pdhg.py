@@ -248,3 +248,31 @@ def test_autotuned_batch_runs_with_the_capacity_it_picked(gpu, oracle_mod):
         assert a.converged and a.iterations == c.iterations
         np.testing.assert_array_equal(a.col_value, c.col_value)
         assert kkt_of(oracle_mod, lp.export(), a)["worst"] <= 1e-4 * 1.000001
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+def test_fused_step_equals_four_launches(gpu, precision):
+    """Instances with a few tiles run a whole step in one launch (k_step_fused) with the
+    per-tile code of the four step kernels: the answers are bit for bit those of the four
+    launches, alone and as a batch of small instances."""
+    kw = dict(pdhg_tolerance=1e-4, pdhg_stop_at_request=True, pdhg_absolute_primal=False,
+              pdhg_precision=precision)
+    groups = [[gpu.Lp.generate(1, 1)],
+              [gpu.Lp.generate(1, s) for s in (2, 3, 4)],
+              [to_lp(gpu, closed_form(name)[0]) for name in ("sum", "range", "maximise")]]
+    for lps in groups:
+        runs = {}
+        for fusion in (0, 1):
+            b = gpu.Batch(step_fusion=fusion, **kw)
+            for lp in lps:
+                b.add(lp)
+            b.upload()
+            b.solve()
+            runs[fusion] = (b.stats()["fused"], [b.result(k) for k in range(len(lps))])
+        assert runs[0][0] == 1 and runs[1][0] == 0
+        for fused, plain in zip(runs[0][1], runs[1][1]):
+            assert fused.converged and fused.iterations == plain.iterations
+            assert fused.status == plain.status
+            np.testing.assert_array_equal(fused.col_value, plain.col_value)
+            np.testing.assert_array_equal(fused.row_dual, plain.row_dual)
+            np.testing.assert_array_equal(fused.col_dual, plain.col_dual)
