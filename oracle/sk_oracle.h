@@ -8,10 +8,15 @@
  * Every function cites the reference file:line it restates (paths relative to the
  * Ajaysingh78/SIH26 repository root).
  *
- * Parity pins (what checks each function; see tests/test_oracle_*.py):
+ * Parity pins (what checks each function; see tests/test_oracle.py):
  *   ok_project, ok_cpu_primal_step ...... closed-form KATs of tests/unit/test_gpu.cpp:107-140,222-243
- *   ok_cpu_spmv_csc, ok_multiply ........ 2x3 KAT test_gpu.cpp:142-177; dense matmul brute force;
+ *   ok_cpu_spmv_csc, ok_multiply,
+ *   ok_transpose_multiply ............... 2x3 KAT test_gpu.cpp:142-177; dense matmul brute force;
  *                                         bit-exact against oracle/_ref (reference sparse.cpp)
+ *   ok_set_threads ...................... a solve on 1 and on 4 threads gives the same bits
+ *   ok_mt19937_raw ...................... the standard's 10,000th output (4123659995)
+ *   ok_mt19937_uniform .................. through ok_spectral_norm's bit-exact match with
+ *                                         oracle/_ref, whose start vector these draws are
  *   ok_cpu_dot, ok_cpu_axpy ............. KATs test_gpu.cpp:86-105; FP64 brute force
  *   ok_compare_vectors .................. KATs test_gpu.cpp:183-216
  *   ok_build_scaling .................... bit-exact against oracle/_ref (reference scaling.cpp);
