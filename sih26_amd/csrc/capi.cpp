@@ -574,11 +574,11 @@ int32_t sk_batch_upload(sk_batch* b) {
   if (!b) return SK_GPU_INVALID_ARGUMENT;
   return guarded([&]() -> int32_t {
     int32_t st;
-    if (b->options.tile_nnz == -1) {
+    if (b->options.tile_nnz == -1 && !b->lps.empty()) {
       st = autotune_upload(b);
     } else {
       st = b->engine->upload();
-      b->tile_nnz = capacity_for(b->options.tile_nnz);
+      b->tile_nnz = capacity_for(b->options.tile_nnz < 0 ? 0 : b->options.tile_nnz);
     }
     if (!st) b->uploaded = true;
     return st;
