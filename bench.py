@@ -258,7 +258,7 @@ def serve_stream(sk, requests: int, rate: float, slots: int, max_batch: int,
 
 def polish_sample(sk, lps, results, count: int) -> dict:
     """The interior-point finish (sk_pdhg_polish) of the first `count` batch answers: the
-    north-star's 1e-6 relative-objective step after the 1e-4 first-order solve."""
+    step from the 1e-4 first-order answer to an optimal one within 1e-6 relative objective."""
     rows = []
     for lp, first in list(zip(lps, results))[:count]:
         t0 = time.perf_counter()

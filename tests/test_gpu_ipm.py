@@ -68,7 +68,7 @@ def polish_and_check(gpu, tmp_path, lp, d, name, col_names=None, row_names=None)
 def test_polished_netlib_answers_pass_the_checker(gpu, tmp_path, name):
     d = read_mps(NETLIB / f"{name}.mps")
     _, r = polish_and_check(gpu, tmp_path, to_lp(gpu, d), d, name, d["col_names"], d["row_names"])
-    # 1e-6 relative objective against the published optimum, as the north-star finish demands
+    # the finish must reach 1e-6 relative objective against the published optimum
     assert rel(r.objective, OPTIMA[name]["published_optimal"]) <= 1e-6
 
 
