@@ -395,7 +395,8 @@ def main() -> int:
         "config": {"workload": WORKLOADS[args.workload], "instances": n,
                    "rows": stats["total_rows"], "cols": stats["total_cols"],
                    "nnz": stats["total_nnz"], "sub_batches": stats["streams"],
-                   "tile_nnz": stats["tile_nnz"], "tolerance": 1e-4,
+                   "tile_nnz": stats["tile_nnz"], "fused_step": bool(stats["fused"]),
+                   "tolerance": 1e-4,
                    "stop": "relative KKT (primal, dual, gap) <= 1e-4, FP64 on the original model"},
         "pdhg_iters_per_s": iterations * args.steps / total,
         "solved": solved,
