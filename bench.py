@@ -108,6 +108,7 @@ def cpu_baseline(lps, budget_s: float) -> dict:
     oracle.set_threads(threads)
     done_t, iters_t, setup_t, dt_t = run(exported)
     oracle.set_threads(1)
+    parallel = cpu_instance_parallel(exported[:done], threads)
     return {"value": done / dt, "unit": "LPs/s", "cores": 1, "kind": "port",
             "sample": f"first {done} of the {len(lps)} instances, FP64 oracle, 1 thread, "
                       f"{dt:.1f} s, {iters / dt:.0f} PDHG iterations/s",
@@ -118,7 +119,54 @@ def cpu_baseline(lps, budget_s: float) -> dict:
                             "iters_per_s": iters_t / dt_t,
                             "value_without_setup": done_t / (dt_t - setup_t),
                             "sample": f"the same {done_t} instances, A^T y gather on {threads} "
-                                      f"threads, {dt_t:.1f} s"}}
+                                      f"threads, {dt_t:.1f} s"},
+            "instance_parallel": parallel}
+
+
+_WORKER = r'''
+import json, sys, time
+import numpy as np
+sys.path.insert(0, sys.argv[1])
+from oracle import oracle
+oracle.lib()
+t0 = time.perf_counter()
+iters = 0
+for path in sys.argv[2:]:
+    z = np.load(path)
+    lp = {k: z[k] for k in z.files}
+    for k in ("num_rows", "num_cols", "sense"):
+        lp[k] = int(lp[k])
+    lp["offset"] = float(lp["offset"])
+    iters += oracle.solve_pdhg(lp, tolerance=1e-4, stop_at_request=True,
+                               absolute_primal=False)["iterations"]
+print(json.dumps({"iterations": iters, "seconds": time.perf_counter() - t0}))
+'''
+
+
+def cpu_instance_parallel(sample: list, procs: int) -> dict:
+    """The same instances spread over `procs` child processes (one oracle thread each), the
+    way a batch of independent LPs uses a multi-core host."""
+    import subprocess
+    import tempfile
+    import numpy as np
+    with tempfile.TemporaryDirectory() as tmp:
+        files = []
+        for k, lp in enumerate(sample):
+            f = Path(tmp) / f"lp{k}.npz"
+            np.savez(f, **{key: np.asarray(v) for key, v in lp.items()})
+            files.append(str(f))
+        t0 = time.perf_counter()
+        children = [subprocess.Popen([sys.executable, "-c", _WORKER, str(ROOT)] + files[r::procs],
+                                     stdout=subprocess.PIPE, text=True)
+                    for r in range(procs) if files[r::procs]]
+        outs = [c.communicate()[0] for c in children]
+        wall = time.perf_counter() - t0
+        if any(c.returncode for c in children):
+            return {"error": "a worker failed"}
+    iters = sum(json.loads(o)["iterations"] for o in outs)
+    return {"value": len(sample) / wall, "cores": len(children), "iters_per_s": iters / wall,
+            "sample": f"the same {len(sample)} instances over {len(children)} processes, one "
+                      f"oracle thread each, {wall:.1f} s wall (process start included)"}
 
 
 def cpu_baseline_single(oracle, lp: dict, budget_s: float) -> dict:
