@@ -339,9 +339,9 @@ def main() -> int:
     ap.add_argument("--kernel-reps", type=int, default=50)
     ap.add_argument("--spmv-stage", type=int, default=-1,
                     help="staging order: -1 default, 0 CSR order, 1 sorted by gather index")
-    ap.add_argument("--tile-nnz", type=int, default=-1,
-                    help="row-block capacity: -1 (default) picked by the autotuner at upload, "
-                         "0 the library default, n > 0 that capacity")
+    ap.add_argument("--tile-nnz", type=int, default=0,
+                    help="row-block capacity: 0 the library default (2,560), -1 picked by the "
+                         "autotuner at upload, n > 0 that capacity")
     ap.add_argument("--steps-per-block", type=int, default=40,
                     help="PDHG steps per captured graph")
     ap.add_argument("--streams", type=int, default=0,

@@ -541,11 +541,15 @@ int32_t capacity_for(int32_t tile_nnz) {
 
 // tile_nnz = -1: the batch is uploaded with each candidate capacity and the two SpMV step
 // kernels are timed with every instance stepping, as the solve schedules them (all sub-batch
-// streams at once); the engine with the least time of the two stays, the others are freed.
-// A capacity changes no row sum, only which tiles share a workgroup and a partial sum.
+// streams at once): the median of three timings of 10 launches each. The default capacity is
+// timed first and another one replaces it only when it is more than 3% faster, since
+// concurrent launches time with a few percent of noise. The engine kept is the chosen one;
+// the others are freed. A capacity changes no row sum, only which tiles share a workgroup and
+// a partial sum.
 int32_t autotune_upload(sk_batch* b) {
-  static const int32_t kCandidates[] = {1536, 2048, 2560, 3072, 4096};
-  constexpr int kReps = 10;
+  static const int32_t kCandidates[] = {sk::kDefaultTileNnz, 1536, 2048, 3072, 4096};
+  constexpr int kReps = 10, kTimings = 3;
+  constexpr double kMargin = 0.97;
   std::unique_ptr<sk::Engine> best;
   double best_ms = INFINITY;
   for (const int32_t cap : kCandidates) {
@@ -554,12 +558,18 @@ int32_t autotune_upload(sk_batch* b) {
     auto e = sk::make_engine(to_engine(o));
     for (const sk_lp* lp : b->lps) e->add(&lp->lp);
     int32_t st = e->upload();
-    double dual_ms = 0.0, aty_ms = 0.0;
-    if (!st) st = e->time_kernel(1, kReps, &dual_ms);
-    if (!st) st = e->time_kernel(2, kReps, &aty_ms);
+    double t[kTimings];
+    for (int k = 0; k < kTimings && !st; ++k) {
+      double dual_ms = 0.0, aty_ms = 0.0;
+      st = e->time_kernel(1, kReps, &dual_ms);
+      if (!st) st = e->time_kernel(2, kReps, &aty_ms);
+      t[k] = dual_ms + aty_ms;
+    }
     if (st) return st;
-    if (dual_ms + aty_ms < best_ms) {
-      best_ms = dual_ms + aty_ms;
+    std::sort(t, t + kTimings);
+    const double ms = t[kTimings / 2];
+    if (!best || ms < kMargin * best_ms) {
+      best_ms = ms;
       best = std::move(e);
       b->tile_nnz = cap;
     }
