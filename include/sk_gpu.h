@@ -389,9 +389,8 @@ void sk_batch_destroy(sk_batch* batch);
 typedef struct sk_server sk_server;
 
 typedef struct {
-  int32_t slots;          /* concurrent batches (default 2: each holds two streams, and
-                             more streams than the process's 4 hardware queues lengthen the
-                             latency tail) */
+  int32_t slots;          /* concurrent batches (default 2: more shorten the queue wait
+                             but slow every batch and lengthen the latency tail) */
   int32_t max_batch;      /* instances per batch (default 256) */
   int64_t memory_budget;  /* device bytes all slots may hold; 0: 90% of free memory at create */
   int32_t keep_solutions; /* 1: keep the primal/dual vectors for sk_server_result (default 1) */

@@ -722,8 +722,9 @@ extern "C" {
 void sk_server_default_options(sk_server_options* out) {
   if (!out) return;
   std::memset(out, 0, sizeof(*out));
-  // two slots: each engine holds a compute and a read-back stream, and more than the
-  // process's four hardware queues' worth of streams doubled the p99 latency in measurement
+  // two slots: more concurrent batches shorten the queue wait but slow every batch, and the
+  // largest requests' latency with them (p99 of a 2,000-request stream: 2 slots ~150 ms,
+  // 4 slots 180-350 ms)
   out->slots = 2;
   out->max_batch = 256;
   out->memory_budget = 0;

@@ -33,7 +33,7 @@ namespace sk {
 
 struct ServerOptions {
   EngineOptions engine;
-  int32_t slots = 2;           // concurrent batches (engines, each with its own two streams)
+  int32_t slots = 2;           // concurrent batches (engines, each with its own streams)
   int32_t max_batch = 256;     // instances per batch
   int64_t memory_budget = 0;   // device bytes all slots may hold; 0: 90% of free memory
 };
