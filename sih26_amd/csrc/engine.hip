@@ -776,8 +776,7 @@ int EngineImpl<T>::enqueue_eval(int report_mode) {
   const int st = with_layout([&](auto cap, auto pack) -> int {
     constexpr int C = decltype(cap)::value;
     constexpr bool P = decltype(pack)::value;
-    SK_LAUNCH(k_eval_rows<T, C, P><<<n_ta_, kBlock, 0, s>>>(a));
-    SK_LAUNCH(k_eval_cols<T, C, P><<<n_tt_, kBlock, 0, s>>>(a));
+    SK_LAUNCH(k_eval_products<T, C, P><<<n_ta_ + n_tt_, kBlock, 0, s>>>(a, n_ta_));
     return kStatusSuccess;
   });
   if (st) return st;

@@ -11,8 +11,9 @@
 // Every 40 accepted iterations (or on a step without information) an instance runs the
 // FP64 evaluation stage (pdhg.cpp:502-643):
 //   k_eval_prepare  unscale current and average candidates; ||x - x_restart||, ||y - y_restart||
-//   k_eval_rows     A x per candidate on the unscaled A (FP64): primal residual, row terms
-//   k_eval_cols     c + A^T y per candidate (FP64): dual residual, objectives
+//   k_eval_products eval_rows_tile: A x per candidate on the unscaled A (FP64): primal
+//                   residual, row terms; eval_cols_tile: c + A^T y per candidate: dual
+//                   residual, objectives (one launch for both)
 //   k_eval_finalize one workgroup per instance: picks the candidate, tests termination,
 //                   restarts
 //   k_eval_commit   copy the chosen candidate into `best`, reset sums on restart
@@ -574,14 +575,12 @@ __global__ __launch_bounds__(kBlock) void k_eval_prepare(Args<T> a) {
   if (threadIdx.x == 0) (is_col ? a.pe_c : a.pe_r)[tile * 2] = d2;
 }
 
-// k_eval_rows: activity = A x for both candidates on the original A, and every row term of
+// eval_rows_tile: activity = A x for both candidates on the original A, and every row term of
 // evaluate() (pdhg.cpp:154-170, 203-222, 227-238). Partials per tile, per candidate c:
 // [4c+0] sum violation^2, [4c+1] support, [4c+2] dual violation from y, [4c+3] complementarity.
-template <class T, int CAP, bool PACK>
-__global__ __launch_bounds__(kBlock) void k_eval_rows(Args<T> a) {
-  __shared__ D2 stage[CAP];
-  __shared__ double red[8 * (kBlock / kWave)];
-  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+template <class T, bool PACK>
+__device__ __forceinline__ void eval_rows_tile(const Args<T>& a, const int tile, D2* stage,
+                                               double* red) {
   if (!(a.fa[tile] & (a.report ? kFlagReport : kFlagEval))) return;
   const Tile t = a.ta[tile];
   const double* __restrict__ xa = a.report ? a.bestx : a.xu0;
@@ -725,15 +724,13 @@ __device__ void decide_evaluation(InstState& s, const Residuals& cur, const Resi
   if (s.iteration >= s.iteration_limit) finish(s, kStIterationLimit, done_count);
 }
 
-// k_eval_cols: reduced cost d = c + A^T y per candidate on the original A and every column
+// eval_cols_tile: reduced cost d = c + A^T y per candidate on the original A and every column
 // term of evaluate() (pdhg.cpp:175-199, 239-257). Partials per tile, per candidate c:
 // [4c+0] dual violation from d, [4c+1] bound contribution, [4c+2] complementarity,
 // [4c+3] primal objective.
-template <class T, int CAP, bool PACK>
-__global__ __launch_bounds__(kBlock) void k_eval_cols(Args<T> a) {
-  __shared__ D2 stage[CAP];
-  __shared__ double red[8 * (kBlock / kWave)];
-  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+template <class T, bool PACK>
+__device__ __forceinline__ void eval_cols_tile(const Args<T>& a, const int tile, D2* stage,
+                                               double* red) {
   if (!(a.ft[tile] & (a.report ? kFlagReport : kFlagEval))) return;
   const Tile t = a.tt[tile];
   const double* __restrict__ xa = a.report ? a.bestx : a.xu0;
@@ -791,6 +788,18 @@ __global__ __launch_bounds__(kBlock) void k_eval_cols(Args<T> a) {
 #pragma unroll
     for (int k = 0; k < 8; ++k) a.pe_t[(size_t)tile * 8 + k] = acc[k];
   }
+}
+
+// The row and the column products of an evaluation in one launch: workgroups [0, rows) take
+// the tiles of A, the others those of A^T. Both read only the candidates k_eval_prepare wrote
+// and write their own partials, so neither waits for the other.
+template <class T, int CAP, bool PACK>
+__global__ __launch_bounds__(kBlock) void k_eval_products(Args<T> a, int rows) {
+  __shared__ D2 stage[CAP];
+  __shared__ double red[8 * (kBlock / kWave)];
+  const int b = static_cast<int>(blockIdx.x);
+  if (b < rows) eval_rows_tile<T, PACK>(a, xcd_tile(b, rows), stage, red);
+  else eval_cols_tile<T, PACK>(a, xcd_tile(b - rows, static_cast<int>(gridDim.x) - rows), stage, red);
 }
 
 // One workgroup per evaluated instance: reduce every evaluation partial in tile order, form
@@ -855,7 +864,7 @@ __global__ __launch_bounds__(kBlock) void k_eval_finalize(Args<T> a) {
   publish_flags(a, d, flags);
 }
 
-// k_eval_commit: apply the decisions of k_eval_cols (pdhg.cpp:541-545, 592-594, 632-638).
+// k_eval_commit: apply the decisions of k_eval_finalize (pdhg.cpp:541-545, 592-594, 632-638).
 // Idempotent, so an instance that stopped may see it again without effect.
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_eval_commit(Args<T> a) {
