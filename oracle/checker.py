@@ -6,7 +6,7 @@ The reference's checker trusts nothing the solver says: it recomputes activities
 objective, reduced costs and the duality gap from (model, x, y) and holds each to the claim
 the file's status makes. This module does the same from the .sol text (written by
 sk_write_solution in the reference's format, src/io/writer.cpp:119-227) and the model arrays
-(the layout of sih26_amd.Lp.export() / sih26_amd.mps.read_mps), so it can run where the
+(the layout of sih26_amd.Lp.export() / tests/mps_ref.read_mps), so it can run where the
 reference cannot travel. Check names, thresholds and the order of checks follow
 verify_solution.py (line numbers cited per check); the model is taken as arrays rather than
 re-read from MPS. Integer columns, quadratic objectives and infeasibility certificates are

@@ -42,6 +42,15 @@ inline unsigned blocks_for(int64_t n) {
   return static_cast<unsigned>(std::max<int64_t>(1, (n + kBlock - 1) / kBlock));
 }
 
+// Waits for a stream when it goes out of scope: keeps host sources of queued asynchronous
+// copies alive until the copies are done, on every return path.
+struct StreamDrain {
+  hipStream_t stream;
+  ~StreamDrain() {
+    if (stream) (void)hipStreamSynchronize(stream);
+  }
+};
+
 // Device buffer owned by the engine.
 struct DevBuf {
   void* ptr = nullptr;
@@ -495,6 +504,10 @@ int EngineImpl<T>::upload() {
   }
   for (hipEvent_t& e : block_ev_)
     if (!e) SK_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming), kStatusInitFailed);
+  // The host vectors above are the sources of the asynchronous copies queued below and are
+  // destroyed when upload() returns; every exit from here on, an error return included, first
+  // waits for the stream (declared after the vectors, so destroyed before them).
+  const StreamDrain drain{stream_};
 
   Args<T>& a = args_;
   int st = 0;
@@ -641,7 +654,7 @@ int EngineImpl<T>::upload() {
   a.force = 0;
   a.report = 0;
 #undef SK_TRY
-  // the host vectors above are sources of asynchronous copies
+  // the host vectors above are sources of asynchronous copies (the drain also covers errors)
   SK_HIP(hipStreamSynchronize(stream_), kStatusSyncFailed);
   final_state_.assign(static_cast<size_t>(K_), InstState{});
   extracted_.assign(static_cast<size_t>(K_), 0);

@@ -115,6 +115,15 @@ __global__ __launch_bounds__(kLanes) void k_chol_block(double* __restrict__ M, i
   if (tid == 0) *floored += count;
 }
 
+// Waits for a stream when it goes out of scope, so host sources of queued asynchronous copies
+// outlive the copies on every return path.
+struct StreamDrain {
+  hipStream_t stream;
+  ~StreamDrain() {
+    if (stream) (void)hipStreamSynchronize(stream);
+  }
+};
+
 template <class U>
 int upload(const std::vector<U>& host, U** dev, hipStream_t s) {
   NE_HIP(hipMalloc(reinterpret_cast<void**>(dev), std::max<size_t>(1, host.size()) * sizeof(U)),
@@ -167,6 +176,8 @@ int NormalEquations::init(int32_t m, int32_t n, const std::vector<int32_t>& row_
     return kStatusInitFailed;
   }
   NE_BLAS(rocblas_set_stream(d.blas, d.stream));
+  // the caller's vectors are the sources of the queued copies: every return waits for them
+  const StreamDrain drain{d.stream};
   int st = 0;
   if ((st = upload(row_ptr, &d.rp, d.stream)) || (st = upload(row_idx, &d.ri, d.stream)) ||
       (st = upload(row_val, &d.rv, d.stream)) || (st = upload(col_ptr, &d.cp, d.stream)) ||
@@ -193,6 +204,7 @@ int NormalEquations::factor(const double* theta, const double* shift, double flo
   *floored = 0;
   if (m == 0) return kStatusSuccess;
   hipStream_t s = d.stream;
+  const StreamDrain drain{s};  // theta and shift are copied asynchronously from the caller
   NE_HIP(hipMemcpyAsync(d.theta, theta, static_cast<size_t>(d.n) * sizeof(double),
                         hipMemcpyHostToDevice, s), kStatusCopyFailed);
   NE_HIP(hipMemcpyAsync(d.shift, shift, static_cast<size_t>(m) * sizeof(double),
@@ -232,6 +244,7 @@ int NormalEquations::solve(double* rhs) {
   const int32_t m = d.m;
   if (m == 0) return kStatusSuccess;
   hipStream_t s = d.stream;
+  const StreamDrain drain{s};  // rhs is the source and the target of asynchronous copies
   NE_HIP(hipMemcpyAsync(d.x, rhs, static_cast<size_t>(m) * sizeof(double), hipMemcpyHostToDevice, s),
          kStatusCopyFailed);
   NE_BLAS(rocblas_dtrsv(d.blas, rocblas_fill_lower, rocblas_operation_none,

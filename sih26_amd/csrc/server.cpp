@@ -177,6 +177,22 @@ bool Server::take_batch(int index, std::vector<std::unique_ptr<Request>>* batch)
         stats_.max_batch_seen = std::max<int64_t>(stats_.max_batch_seen, static_cast<int64_t>(take));
         return true;
       }
+      // This slot's buffers grow per dimension, so the projection above takes the maxima of
+      // its earlier batches and the head (a row-heavy batch, then an nnz-heavy request):
+      // when the head fits on its own next to the other slots, the slot gives its buffers
+      // back and forms the batch from nothing.
+      if (me.held > 0) {
+        const HostLp& head = *queue_.front()->lp;
+        const int64_t alone = estimate_device_bytes(head.num_rows, head.num_cols, head.nnz(), 1,
+                                                    opt_.engine.precision);
+        if (others + alone <= opt_.memory_budget) {
+          me.engine->release();
+          me.held = 0;
+          me.extent = Extent{};
+          ++stats_.releases;
+          continue;
+        }
+      }
       // the head does not fit next to what the other slots hold: ask the idle ones to release
       if (!release_wanted_) {
         release_wanted_ = true;

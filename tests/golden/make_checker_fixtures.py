@@ -29,7 +29,7 @@ sys.path.insert(0, str(ROOT))
 
 import sih26_amd as sk  # noqa: E402
 from sih26_amd import _ffi  # noqa: E402
-from sih26_amd.mps import read_mps  # noqa: E402
+from tests.mps_ref import read_mps  # noqa: E402
 from oracle import oracle  # noqa: E402
 from tests.lp_cases import closed_form  # noqa: E402
 

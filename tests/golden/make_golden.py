@@ -17,7 +17,7 @@ ROOT = Path(__file__).resolve().parents[2]
 sys.path.insert(0, str(ROOT))
 
 from oracle import oracle  # noqa: E402
-from sih26_amd.mps import read_mps  # noqa: E402
+from tests.mps_ref import read_mps  # noqa: E402
 from tests.lp_cases import dense_to_lp  # noqa: E402
 
 

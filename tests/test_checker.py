@@ -8,7 +8,7 @@ from pathlib import Path
 import pytest
 
 from oracle import checker
-from sih26_amd.mps import read_mps
+from tests.mps_ref import read_mps
 
 GOLD = Path(__file__).resolve().parent / "golden"
 VERDICTS = json.loads((GOLD / "checker" / "verdicts.json").read_text())

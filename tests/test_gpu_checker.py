@@ -5,7 +5,7 @@ from pathlib import Path
 import pytest
 
 from oracle import checker
-from sih26_amd.mps import read_mps
+from tests.mps_ref import read_mps
 
 pytestmark = pytest.mark.gpu
 NETLIB = Path(__file__).resolve().parent / "golden" / "netlib"

@@ -8,7 +8,7 @@ import numpy as np
 import pytest
 
 from oracle import checker
-from sih26_amd.mps import read_mps
+from tests.mps_ref import read_mps
 from tests.lp_cases import CLOSED_FORM, closed_form
 
 pytestmark = pytest.mark.gpu

@@ -1,0 +1,93 @@
+"""The reference-side adapter (integration/pdhg_mi355x.cpp): it compiles against the
+reference's own headers, calls only entry points the library exports, and the option texts it
+forwards parse back to the same values. CPU only; the compile check needs /root/reference."""
+import ctypes as C
+import importlib.util
+import re
+import shutil
+import subprocess
+import sys
+from pathlib import Path
+
+import pytest
+
+ROOT = Path(__file__).resolve().parent.parent
+ADAPTER = ROOT / "integration" / "pdhg_mi355x.cpp"
+REFERENCE_INCLUDE = Path("/root/reference/include")
+
+
+def _fmt_include() -> Path | None:
+    """fmt headers (header-only) bundled with the installed torch, found without importing it."""
+    spec = importlib.util.find_spec("torch")
+    if spec is None or not spec.submodule_search_locations:
+        return None
+    inc = Path(list(spec.submodule_search_locations)[0]) / "include"
+    return inc if (inc / "fmt" / "format.h").exists() else None
+
+
+@pytest.fixture(scope="module")
+def adapter_object(tmp_path_factory):
+    if not REFERENCE_INCLUDE.exists():
+        pytest.skip("reference headers absent")
+    fmt = _fmt_include()
+    gxx = shutil.which("g++")
+    if fmt is None or gxx is None:
+        pytest.skip("g++ or the fmt headers are absent")
+    out = tmp_path_factory.mktemp("adapter") / "pdhg_mi355x.o"
+    cmd = [gxx, "-std=c++20", "-DFMT_HEADER_ONLY", f"-I{fmt}", f"-I{REFERENCE_INCLUDE}",
+           f"-I{ROOT / 'include'}", "-Wall", "-Wextra", "-Werror", "-c", str(ADAPTER), "-o", str(out)]
+    proc = subprocess.run(cmd, capture_output=True, text=True)
+    assert proc.returncode == 0, proc.stderr
+    return out
+
+
+def test_adapter_compiles_against_the_reference_headers(adapter_object):
+    syms = subprocess.run(["nm", "-C", str(adapter_object)], capture_output=True, text=True,
+                          check=True).stdout
+    assert re.search(r" T sankhya::gpu::solve_pdhg_mi355x\(sankhya::Model const&, "
+                     r"sankhya::Options const&, sankhya::Logger&\)", syms), syms
+
+
+def test_adapter_calls_only_exported_entry_points(adapter_object):
+    import sih26_amd
+    undefined = subprocess.run(["nm", "-u", str(adapter_object)], capture_output=True, text=True,
+                               check=True).stdout.split()
+    calls = sorted({s for s in undefined if s.startswith("sk_")})
+    assert "sk_pdhg_solve" in calls and "sk_lp_create" in calls
+    lib = sih26_amd.lib()
+    for name in calls:
+        assert hasattr(lib, name), name
+
+
+def test_integration_md_lists_the_compiled_file():
+    text = (ROOT / "INTEGRATION.md").read_text()
+    assert "```cpp\n" + ADAPTER.read_text() + "```" in text
+
+
+@pytest.mark.parametrize("key,value,field,expected", [
+    ("pdhg_tolerance", 1e-4, "tolerance", 1e-4),
+    ("pdhg_tolerance", 1.2345678901234567e-5, "tolerance", 1.2345678901234567e-5),
+    ("time_limit", sys.float_info.max, "time_limit", sys.float_info.max),  # kNoLimit
+    ("time_limit", 0.7 * 3.0, "time_limit", 0.7 * 3.0),  # solve.cpp:389-391's 70% share
+    ("iteration_limit", -1, "iteration_limit", -1),
+    ("iteration_limit", 123456789012, "iteration_limit", 123456789012),
+    ("pdhg_restart", False, "restart", 0),
+    ("pdhg_stop_at_request", True, "stop_at_request", 1),
+    ("random_seed", 7, "random_seed", 7),
+])
+def test_forwarded_option_texts_parse_back_exactly(key, value, field, expected):
+    """The adapter formats doubles with %.17g, integers in decimal and booleans as
+    true/false; sk_pdhg_set_option must read each back to the registry's value."""
+    import sih26_amd
+    from sih26_amd import _ffi
+    if isinstance(value, bool):
+        text = "true" if value else "false"
+    elif isinstance(value, int):
+        text = str(value)
+    else:
+        text = "%.17g" % value
+    o = _ffi.PdhgOptions()
+    lib = sih26_amd.lib()
+    lib.sk_pdhg_default_options(C.byref(o))
+    assert lib.sk_pdhg_set_option(C.byref(o), key.encode(), text.encode()) == 0
+    assert getattr(o, field) == expected

@@ -5,7 +5,7 @@ import numpy as np
 import pytest
 
 import sih26_amd as sk
-from sih26_amd.mps import read_mps
+from tests.mps_ref import read_mps
 from tests.lp_cases import dense_to_lp
 from tests.test_mps import ARRAYS, FIXTURES
 

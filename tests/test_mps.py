@@ -6,7 +6,7 @@ import numpy as np
 import pytest
 
 import sih26_amd as sk
-from sih26_amd.mps import read_mps
+from tests.mps_ref import read_mps
 
 GOLDEN = Path(__file__).resolve().parent / "golden"
 FIXTURES = sorted(GOLDEN.glob("netlib/*.mps")) + sorted(GOLDEN.glob("checker/*.mps"))

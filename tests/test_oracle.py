@@ -13,7 +13,7 @@ import numpy as np
 import pytest
 
 from oracle import oracle
-from sih26_amd.mps import read_mps
+from tests.mps_ref import read_mps
 from tests.lp_cases import CLOSED_FORM, closed_form, csc_to_dense, dense_to_lp, kkt_lp
 
 GOLDEN = Path(__file__).resolve().parent / "golden"
