@@ -377,6 +377,27 @@ int32_t sk_batch_stats_get(const sk_batch* batch, sk_batch_stats* out);
  * reports the wall time per repetition until the last is done. */
 int32_t sk_batch_time_kernel(sk_batch* batch, int32_t kernel, int32_t reps, double* mean_ms,
                              double* bytes);
+/* Probes of a solved batch, for parity tests: what the device computes, in the LP's own row
+ * and column order, for instance `index`.
+ * sk_batch_probe_spmv runs the step kernels' own sparse product (the per-tile staging,
+ * gathers and fixed per-row summation order of the PDHG step) on a caller's vector, in the
+ * iterate precision: which = 1 gives out (num_rows) = A-hat in (num_cols), the product the
+ * dual step forms as A-hat x-bar; which = 2 gives out (num_cols) = A-hat^T in (num_rows), the
+ * product the A^T step forms as A-hat^T y'. A-hat is the device-scaled matrix. The
+ * counterparts are SparseMatrix::multiply / transpose_multiply (src/la/sparse.cpp:202-239)
+ * applied to the reference's scaled matrix. */
+int32_t sk_batch_probe_spmv(sk_batch* batch, int32_t index, int32_t which, const double* in,
+                            double* out);
+/* The device scaling of instance `index`: D_r (num_rows), D_c (num_cols) and the scaled FP64
+ * values of A in the LP's CSC order (nnz); any pointer may be NULL. The counterpart is the
+ * reference's Scaling (include/sankhya/scaling.hpp:42-58; built by src/la/scaling.cpp:111-159):
+ * row_scale, col_scale and the scaled matrix's values. */
+int32_t sk_batch_probe_scaling(sk_batch* batch, int32_t index, double* row_scale,
+                               double* col_scale, double* scaled_values);
+/* The LP column each device column of instance `index` holds (num_cols entries): the device
+ * order the column_order option chose. No counterpart in the reference, whose engine keeps
+ * the model's order. */
+int32_t sk_batch_column_order(const sk_batch* batch, int32_t index, int32_t* device_to_lp);
 void sk_batch_destroy(sk_batch* batch);
 
 /* ---- serving: a continuous-batching request loop -------------------------------------------

@@ -138,6 +138,9 @@ SIGNATURES = {
     "sk_batch_result": (i32, [vp, i32, P(PdhgResult), vp, vp, vp, vp]),
     "sk_batch_stats_get": (i32, [vp, P(BatchStats)]),
     "sk_batch_time_kernel": (i32, [vp, i32, i32, P(f64), P(f64)]),
+    "sk_batch_probe_spmv": (i32, [vp, i32, i32, vp, vp]),
+    "sk_batch_probe_scaling": (i32, [vp, i32, vp, vp, vp]),
+    "sk_batch_column_order": (i32, [vp, i32, vp]),
     "sk_batch_destroy": (None, [vp]),
     "sk_server_default_options": (None, [P(ServerOptions)]),
     "sk_server_create": (i32, [P(PdhgOptions), P(ServerOptions), P(vp)]),

@@ -322,6 +322,33 @@ class Batch:
               "sk_batch_time_kernel")
         return ms.value, b.value
 
+    def probe_spmv(self, k: int, which: int, vec) -> np.ndarray:
+        """The step kernels' product on instance k after a solve: which 1 gives A-hat vec
+        (vec has num_cols entries), which 2 gives A-hat^T vec (num_rows entries)."""
+        m, n, _ = self._lps[k].dims
+        vec = np.ascontiguousarray(vec, np.float64)
+        if vec.shape != ((n,) if which == 1 else (m,)):
+            raise ValueError("probe_spmv: input length does not match the product")
+        out = np.zeros(m if which == 1 else n)
+        check(lib().sk_batch_probe_spmv(self._h, k, which, _ptr(vec), _ptr(out)),
+              "sk_batch_probe_spmv")
+        return out
+
+    def probe_scaling(self, k: int) -> dict[str, np.ndarray]:
+        """D_r, D_c and the scaled FP64 values (the LP's CSC order) the device set-up made."""
+        m, n, z = self._lps[k].dims
+        out = {"row": np.zeros(m), "column": np.zeros(n), "values": np.zeros(z)}
+        check(lib().sk_batch_probe_scaling(self._h, k, _ptr(out["row"]), _ptr(out["column"]),
+                                           _ptr(out["values"])), "sk_batch_probe_scaling")
+        return out
+
+    def column_order(self, k: int) -> np.ndarray:
+        """The LP column each device column of instance k holds."""
+        _, n, _ = self._lps[k].dims
+        out = np.zeros(n, np.int32)
+        check(lib().sk_batch_column_order(self._h, k, _ptr(out)), "sk_batch_column_order")
+        return out
+
     def close(self) -> None:
         if self._h:
             lib().sk_batch_destroy(self._h)

@@ -654,6 +654,34 @@ int32_t sk_batch_time_kernel(sk_batch* b, int32_t kernel, int32_t reps, double* 
   });
 }
 
+int32_t sk_batch_probe_spmv(sk_batch* b, int32_t index, int32_t which, const double* in,
+                            double* out) {
+  if (!b || !b->solved || !in || !out) {
+    sk::set_last_error("sk_batch_probe_spmv: needs a solved batch and both arrays");
+    return SK_GPU_INVALID_ARGUMENT;
+  }
+  return guarded([&]() -> int32_t { return b->engine->probe_spmv(index, which, in, out); });
+}
+
+int32_t sk_batch_probe_scaling(sk_batch* b, int32_t index, double* row_scale, double* col_scale,
+                               double* scaled_values) {
+  if (!b || !b->solved) {
+    sk::set_last_error("sk_batch_probe_scaling: needs a solved batch");
+    return SK_GPU_INVALID_ARGUMENT;
+  }
+  return guarded([&]() -> int32_t {
+    return b->engine->probe_scaling(index, row_scale, col_scale, scaled_values);
+  });
+}
+
+int32_t sk_batch_column_order(const sk_batch* b, int32_t index, int32_t* device_to_lp) {
+  if (!b || !b->uploaded || !device_to_lp) {
+    sk::set_last_error("sk_batch_column_order: needs an uploaded batch and an array");
+    return SK_GPU_INVALID_ARGUMENT;
+  }
+  return guarded([&]() -> int32_t { return b->engine->column_order(index, device_to_lp); });
+}
+
 void sk_batch_destroy(sk_batch* b) { delete b; }
 
 }  // extern "C"

@@ -148,6 +148,10 @@ __device__ __forceinline__ void rowblock_product(const Tile t, const int32_t* __
   const int p0 = t.p0;
   const int p1 = t.p1;
   if (t.kind == kTileVector) {
+    // Each product is rounded before it is added, as in the staged paths (products go through
+    // LDS there): without this the compiler fuses a product into the running sum (one FMA per
+    // unrolled batch), and a row would sum differently as a vector tile than as a long row.
+#pragma clang fp contract(off)
     if (tid < kWave) {
       if (tid == 0) pre(r0, 0);
       A acc = zero_of<A>();

@@ -178,6 +178,9 @@ class EngineImpl final : public Engine {
   int result(int32_t index, InstanceResult* out) const override;
   int time_kernel(int kernel, int reps, double* mean_ms) override;
   double kernel_bytes(int kernel) const override;
+  int probe_spmv(int32_t index, int which, const double* in, double* out) override;
+  int probe_scaling(int32_t index, double* row_scale, double* col_scale, double* values) override;
+  int column_order(int32_t index, int32_t* device_to_lp) const override;
 
  private:
   // Grow-only device buffers: a reloaded engine (a serving slot taking its next batch) keeps
@@ -261,6 +264,7 @@ class EngineImpl final : public Engine {
   std::vector<uint8_t> trivial_norm_;
   int32_t n_ta_ = 0, n_tt_ = 0, n_tc_ = 0, n_tr_ = 0;
   uint32_t initially_done_ = 0;
+  bool setup_done_ = false;  // the device set-up of the uploaded batch has been enqueued
 
   hipStream_t stream_ = nullptr;
   hipGraphExec_t graph_exec_ = nullptr;
@@ -334,6 +338,7 @@ int EngineImpl<T>::upload() {
   if (setup_exec_) (void)hipGraphExecDestroy(setup_exec_);
   graph_exec_ = setup_exec_ = nullptr;
   inflight_ = 0;
+  setup_done_ = false;
   K_ = static_cast<int32_t>(lps_.size());
   m_ = n_ = nnz_ = 0;
   initially_done_ = 0;
@@ -444,12 +449,25 @@ int EngineImpl<T>::upload() {
   }
   for (size_t i = 0; i < M; ++i) a_ptr[i + 1] += a_ptr[i];
   {
+    // Each row of CSR(A) holds its entries in the LP's column order, also where the device
+    // holds the columns in another order: that is the order in which the reference's column
+    // scatter (sparse.cpp:202-212) and its row statistics (scaling.cpp:44-100) add a row's
+    // terms, so row sums (scaling, A x-bar, the evaluation's activities) do not depend on the
+    // device order. The staging order sorts by device column anyway, for the gathers.
     std::vector<int32_t> next(a_ptr.begin(), a_ptr.end() - 1);
-    for (size_t j = 0; j < N; ++j) {
-      for (int32_t p = t_ptr[j]; p < t_ptr[j + 1]; ++p) {
-        const auto slot = static_cast<size_t>(next[static_cast<size_t>(t_idx[static_cast<size_t>(p)])]++);
-        a_idx[slot] = static_cast<int32_t>(j);
-        a_orig[slot] = t_orig[static_cast<size_t>(p)];
+    std::vector<int32_t> dev_of;
+    for (int32_t k = 0; k < K_; ++k) {
+      const InstDesc& d = desc_[static_cast<size_t>(k)];
+      const std::vector<int32_t>& perm = col_perm_[static_cast<size_t>(k)];  // device -> LP
+      dev_of.assign(perm.size(), 0);
+      for (size_t j = 0; j < perm.size(); ++j) dev_of[static_cast<size_t>(perm[j])] = static_cast<int32_t>(j);
+      for (int32_t o = 0; o < d.cols; ++o) {
+        const auto j = static_cast<size_t>(d.col0 + (perm.empty() ? o : dev_of[static_cast<size_t>(o)]));
+        for (int32_t p = t_ptr[j]; p < t_ptr[j + 1]; ++p) {
+          const auto slot = static_cast<size_t>(next[static_cast<size_t>(t_idx[static_cast<size_t>(p)])]++);
+          a_idx[slot] = static_cast<int32_t>(j);
+          a_orig[slot] = t_orig[static_cast<size_t>(p)];
+        }
       }
     }
   }
@@ -777,6 +795,7 @@ int EngineImpl<T>::setup_device() {
     const int st = enqueue_setup();
     if (st) return st;
   }
+  setup_done_ = true;
   return kStatusSuccess;
 }
 
@@ -1213,7 +1232,119 @@ void EngineImpl<T>::release() {
   }
   args_ = Args<T>{};
   inflight_ = 0;
+  setup_done_ = false;
   K_ = 0;
+}
+
+template <class T>
+int EngineImpl<T>::column_order(int32_t index, int32_t* device_to_lp) const {
+  if (index < 0 || index >= K_ || !device_to_lp) {
+    set_last_error("column_order(): no such instance");
+    return kStatusInvalidArgument;
+  }
+  const std::vector<int32_t>& perm = col_perm_[static_cast<size_t>(index)];
+  const int32_t n = desc_[static_cast<size_t>(index)].cols;
+  for (int32_t j = 0; j < n; ++j) device_to_lp[j] = perm.empty() ? j : perm[static_cast<size_t>(j)];
+  return kStatusSuccess;
+}
+
+template <class T>
+int EngineImpl<T>::probe_spmv(int32_t index, int which, const double* in, double* out) {
+  if (index < 0 || index >= K_ || (which != 1 && which != 2) || !in || !out) {
+    set_last_error("probe_spmv(): bad instance, product or array");
+    return kStatusInvalidArgument;
+  }
+  if (!setup_done_) {
+    set_last_error("probe_spmv(): the device set-up has not run (solve first)");
+    return kStatusInvalidState;
+  }
+  SK_HIP(hipStreamSynchronize(stream_), kStatusSyncFailed);
+  const InstDesc& d = desc_[static_cast<size_t>(index)];
+  const std::vector<int32_t>& perm = col_perm_[static_cast<size_t>(index)];
+  auto lp_col = [&](int32_t j) { return static_cast<size_t>(perm.empty() ? j : perm[static_cast<size_t>(j)]); };
+  const size_t nin = static_cast<size_t>(which == 1 ? n_ : m_);
+  const size_t nout = static_cast<size_t>(which == 1 ? m_ : n_);
+  // the other instances' inputs are zero; only this instance's outputs are read back
+  std::vector<T> hin(nin, static_cast<T>(0)), hout(nout);
+  if (which == 1) {
+    for (int32_t j = 0; j < d.cols; ++j) hin[static_cast<size_t>(d.col0 + j)] = static_cast<T>(in[lp_col(j)]);
+  } else {
+    for (int32_t i = 0; i < d.rows; ++i) hin[static_cast<size_t>(d.row0 + i)] = static_cast<T>(in[i]);
+  }
+  DevBuf bin, bout;
+  T *din = nullptr, *dout = nullptr;
+  int st = alloc(bin, nin, &din);
+  if (!st) st = alloc(bout, nout, &dout);
+  if (st) return st;
+  SK_HIP(hipMemcpy(din, hin.data(), nin * sizeof(T), hipMemcpyHostToDevice), kStatusCopyFailed);
+  const Args<T>& a = args_;
+  hipStream_t s = stream_;
+  st = with_layout([&](auto cap, auto pack) -> int {
+    constexpr int C = decltype(cap)::value;
+    constexpr bool P = decltype(pack)::value;
+    if (which == 1)
+      SK_LAUNCH(k_probe_product<T, C, P><<<static_cast<unsigned>(n_ta_), kStepLanes, 0, s>>>(
+          a.ta, a.a_ptr, a.a_sidx, a.a_pos, a.a_val, din, dout));
+    else
+      SK_LAUNCH(k_probe_product<T, C, P><<<static_cast<unsigned>(n_tt_), kStepLanes, 0, s>>>(
+          a.tt, a.t_ptr, a.t_sidx, a.t_pos, a.t_val, din, dout));
+    return kStatusSuccess;
+  });
+  if (st) return st;
+  SK_HIP(hipStreamSynchronize(s), kStatusSyncFailed);
+  SK_HIP(hipMemcpy(hout.data(), dout, nout * sizeof(T), hipMemcpyDeviceToHost), kStatusCopyFailed);
+  if (which == 1) {
+    for (int32_t i = 0; i < d.rows; ++i) out[i] = static_cast<double>(hout[static_cast<size_t>(d.row0 + i)]);
+  } else {
+    for (int32_t j = 0; j < d.cols; ++j) out[lp_col(j)] = static_cast<double>(hout[static_cast<size_t>(d.col0 + j)]);
+  }
+  return kStatusSuccess;
+}
+
+template <class T>
+int EngineImpl<T>::probe_scaling(int32_t index, double* row_scale, double* col_scale,
+                                 double* values) {
+  if (index < 0 || index >= K_) {
+    set_last_error("probe_scaling(): no such instance");
+    return kStatusInvalidArgument;
+  }
+  if (!setup_done_) {
+    set_last_error("probe_scaling(): the device set-up has not run (solve first)");
+    return kStatusInvalidState;
+  }
+  SK_HIP(hipStreamSynchronize(stream_), kStatusSyncFailed);
+  const InstDesc& d = desc_[static_cast<size_t>(index)];
+  const std::vector<int32_t>& perm = col_perm_[static_cast<size_t>(index)];
+  const auto rows = static_cast<size_t>(d.rows), cols = static_cast<size_t>(d.cols);
+  if (row_scale && rows)
+    SK_HIP(hipMemcpy(row_scale, args_.dr + d.row0, rows * sizeof(double), hipMemcpyDeviceToHost), kStatusCopyFailed);
+  if (col_scale && cols) {
+    std::vector<double> dc(cols);
+    SK_HIP(hipMemcpy(dc.data(), args_.dc + d.col0, cols * sizeof(double), hipMemcpyDeviceToHost), kStatusCopyFailed);
+    for (size_t j = 0; j < cols; ++j) col_scale[perm.empty() ? j : static_cast<size_t>(perm[j])] = dc[j];
+  }
+  if (values && cols) {
+    // CSR(A^T) holds the CSC of A in the device column order: each device column's entries
+    // (rows ascending) go to the LP column it holds
+    std::vector<int32_t> tp(cols + 1);
+    SK_HIP(hipMemcpy(tp.data(), args_.t_ptr + d.col0, (cols + 1) * sizeof(int32_t), hipMemcpyDeviceToHost), kStatusCopyFailed);
+    const auto z = static_cast<size_t>(tp[cols] - tp[0]);
+    std::vector<double> st(z);
+    if (z) SK_HIP(hipMemcpy(st.data(), d_st_ + tp[0], z * sizeof(double), hipMemcpyDeviceToHost), kStatusCopyFailed);
+    std::vector<size_t> lp_start(cols + 1, 0);
+    for (size_t j = 0; j < cols; ++j) {
+      const size_t o = perm.empty() ? j : static_cast<size_t>(perm[j]);
+      lp_start[o + 1] = static_cast<size_t>(tp[j + 1] - tp[j]);
+    }
+    for (size_t o = 0; o < cols; ++o) lp_start[o + 1] += lp_start[o];
+    for (size_t j = 0; j < cols; ++j) {
+      const size_t o = perm.empty() ? j : static_cast<size_t>(perm[j]);
+      const auto b = static_cast<size_t>(tp[j] - tp[0]), e = static_cast<size_t>(tp[j + 1] - tp[0]);
+      std::copy(st.begin() + static_cast<std::ptrdiff_t>(b), st.begin() + static_cast<std::ptrdiff_t>(e),
+                values + lp_start[o]);
+    }
+  }
+  return kStatusSuccess;
 }
 
 }  // namespace

@@ -96,6 +96,20 @@ class Engine {
   virtual int32_t streams() const = 0;
   // whether the uploaded batch runs a PDHG step as one launch (k_step_fused)
   virtual bool fused() const = 0;
+
+  // Probes for tests, valid once a solve (or start()) has run the device set-up. They return
+  // what the device holds or computes, in the LP's own row and column order.
+  // probe_spmv: the step kernels' sparse product (step_product) on instance `index` alone:
+  // which 1: out (rows) = A-hat in (cols), as k_step_dual forms A-hat x-bar; which 2: out
+  // (cols) = A-hat^T in (rows), as k_step_aty forms A-hat^T y'. Inputs are rounded to the
+  // iterate precision first, as the iterates are.
+  virtual int probe_spmv(int32_t index, int which, const double* in, double* out) = 0;
+  // probe_scaling: the device scaling's D_r (rows), D_c (cols) and the scaled FP64 values in
+  // the LP's CSC order (nnz); any pointer may be null.
+  virtual int probe_scaling(int32_t index, double* row_scale, double* col_scale,
+                            double* values) = 0;
+  // The LP column held in each device column of instance `index` (cols entries).
+  virtual int column_order(int32_t index, int32_t* device_to_lp) const = 0;
 };
 
 // An engine for a batch: with options.streams other than 1 the batch is split into

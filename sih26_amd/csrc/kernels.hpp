@@ -313,6 +313,22 @@ __device__ __forceinline__ auto slot_stager(const int32_t* sidx, const uint16_t*
 constexpr int kStepLanes = kBlock;
 constexpr int kStepRows = kMaxTileRows / kStepLanes;
 
+// The sparse product of the two step kernels on one row-block tile: acc_r = sum over row r of
+// val[q] * in[gather index of q], with the tile's nonzeros staged in LDS in its staging order
+// and each row reduced in its fixed order (rowblock_product; pre and epi as there).
+// k_step_dual runs it on CSR(A) with in = x-bar, k_step_aty on CSR(A^T) with in = y', and
+// k_probe_product on either table with a caller's vector.
+template <class T, bool PACK, class Pre, class Epi>
+__device__ __forceinline__ void step_product(const Tile& t, const int32_t* __restrict__ ptr,
+                                             const int32_t* __restrict__ sidx,
+                                             const uint16_t* __restrict__ pos,
+                                             const T* __restrict__ val, const T* __restrict__ in,
+                                             Pre pre, Epi epi, T* stage) {
+  rowblock_product<kStepLanes, T>(
+      t, ptr, [&](int q) { return val[q] * in[gather_index<PACK>(t, sidx, q)]; },
+      gather_stager<kStepLanes, T, PACK>(sidx, pos, val, in), pre, epi, stage);
+}
+
 // A x-bar and the dual step on the rows of tile `tile` of A (pdhg.cpp:402-408), with its
 // ||y'-y||^2 partial.
 template <class T, bool PACK>
@@ -328,14 +344,10 @@ __device__ __forceinline__ void dual_tile(const Args<T>& a, const int tile, T* s
   const double eta = s.eta, omega = s.omega;
   const T* __restrict__ y = p ? a.y1 : a.y0;
   T* __restrict__ yn = p ? a.y0 : a.y1;
-  const T* __restrict__ xbar = a.xbar;
-  const T* __restrict__ val = a.a_val;
-  const int32_t* __restrict__ idx = a.a_sidx;
   double d2 = 0.0;
   T yi[kStepRows], lo[kStepRows], hi[kStepRows], sum[kStepRows];
-  rowblock_product<kStepLanes, T>(
-      t, a.a_ptr, [&](int q) { return val[q] * xbar[gather_index<PACK>(t, idx, q)]; },
-      gather_stager<kStepLanes, T, PACK>(idx, a.a_pos, val, xbar),
+  step_product<T, PACK>(
+      t, a.a_ptr, a.a_sidx, a.a_pos, a.a_val, a.xbar,
       [&](int i, int k) {
         yi[k] = y[i];
         lo[k] = a.rlhat[i];
@@ -361,6 +373,24 @@ __global__ __launch_bounds__(kStepLanes) void k_step_dual(Args<T> a) {
   __shared__ T stage[CAP];
   __shared__ double red[kStepLanes / kWave];
   dual_tile<T, PACK>(a, xcd_tile(blockIdx.x, gridDim.x), stage, red);
+}
+
+// Probe of the step kernels' product (tests): out[r] = step_product's sum for every row of one
+// row-block table, with the staging arrays the step kernels read (CSR(A) with in = x, or
+// CSR(A^T) with in = y) and nothing else computed.
+template <class T, int CAP, bool PACK>
+__global__ __launch_bounds__(kStepLanes) void k_probe_product(const Tile* __restrict__ tiles,
+                                                              const int32_t* __restrict__ ptr,
+                                                              const int32_t* __restrict__ sidx,
+                                                              const uint16_t* __restrict__ pos,
+                                                              const T* __restrict__ val,
+                                                              const T* __restrict__ in,
+                                                              T* __restrict__ out) {
+  __shared__ T stage[CAP];
+  const Tile t = tiles[xcd_tile(blockIdx.x, gridDim.x)];
+  step_product<T, PACK>(
+      t, ptr, sidx, pos, val, in, [](int, int) {}, [&](int r, int, T acc) { out[r] = acc; },
+      stage);
 }
 
 // Reduce `count` per-tile partials starting at tile `base` with the whole workgroup in a
@@ -437,13 +467,10 @@ __device__ __forceinline__ void aty_tile(const Args<T>& a, const int tile, T* st
   const T* __restrict__ yn = p ? a.y0 : a.y1;
   const T* __restrict__ aty = p ? a.aty1 : a.aty0;
   T* __restrict__ atyn = p ? a.aty0 : a.aty1;
-  const T* __restrict__ val = a.t_val;
-  const int32_t* __restrict__ idx = a.t_sidx;
   double inter = 0.0;
   T xj[kStepRows], xnj[kStepRows], atyj[kStepRows];
-  rowblock_product<kStepLanes, T>(
-      t, a.t_ptr, [&](int q) { return val[q] * yn[gather_index<PACK>(t, idx, q)]; },
-      gather_stager<kStepLanes, T, PACK>(idx, a.t_pos, val, yn),
+  step_product<T, PACK>(
+      t, a.t_ptr, a.t_sidx, a.t_pos, a.t_val, yn,
       [&](int j, int k) {
         xj[k] = x[j];
         xnj[k] = xn[j];

@@ -218,6 +218,19 @@ class SplitEngine final : public Engine {
   int64_t total_nnz() const override { return sum(&Engine::total_nnz); }
   int64_t total_rows() const override { return sum(&Engine::total_rows); }
   int64_t total_cols() const override { return sum(&Engine::total_cols); }
+  int probe_spmv(int32_t index, int which, const double* in, double* out) override {
+    const Place* w = place(index);
+    return w ? parts_[static_cast<size_t>(w->part)]->probe_spmv(w->local, which, in, out) : kStatusInvalidArgument;
+  }
+  int probe_scaling(int32_t index, double* row_scale, double* col_scale, double* values) override {
+    const Place* w = place(index);
+    return w ? parts_[static_cast<size_t>(w->part)]->probe_scaling(w->local, row_scale, col_scale, values)
+             : kStatusInvalidArgument;
+  }
+  int column_order(int32_t index, int32_t* device_to_lp) const override {
+    const Place* w = place(index);
+    return w ? parts_[static_cast<size_t>(w->part)]->column_order(w->local, device_to_lp) : kStatusInvalidArgument;
+  }
   int stage_choice() const override { return parts_.empty() ? 1 : parts_[0]->stage_choice(); }
   int32_t streams() const override { return static_cast<int32_t>(parts_.size()); }
   bool fused() const override {
@@ -231,6 +244,13 @@ class SplitEngine final : public Engine {
     int32_t part = 0;
     int32_t local = 0;
   };
+  const Place* place(int32_t index) const {
+    if (index < 0 || static_cast<size_t>(index) >= where_.size()) {
+      set_last_error("no such instance");
+      return nullptr;
+    }
+    return &where_[static_cast<size_t>(index)];
+  }
   int64_t sum(int64_t (Engine::*f)() const) const {
     int64_t v = 0;
     for (const auto& e : parts_) v += ((*e).*f)();
