@@ -85,6 +85,13 @@ int32_t sk_gpu_memset(void* dev_ptr, int32_t value, size_t bytes);
 int32_t sk_gpu_axpy(int32_t n, double alpha, const double* d_x, double* d_y);
 /* gpu_ops.hpp:55-57 launch_gpu_dot: synchronous, deterministic two-stage FP64 reduction */
 int32_t sk_gpu_dot(int32_t n, const double* d_x, const double* d_y, double* host_result);
+/* No counterpart in the reference. sk_gpu_dot for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc): the
+ * same two-stage reduction, partition and bits as sk_gpu_dot, but its partial sums live in a
+ * fixed 1 MiB device array of the library (one per device, held until the process ends) and
+ * it calls no hipMalloc, hipFree or allocating routine. n is at most 268,435,456; a larger n
+ * returns SK_GPU_INVALID_ARGUMENT. Calls that overlap on one device race on that array. */
+int32_t sk_gpu_dot_noalloc(int32_t n, const double* d_x, const double* d_y, double* host_result);
 /* gpu_ops.hpp:59-62 launch_gpu_pdhg_primal_step:
  *   x_next = proj_[lo,hi](x - tau (cost + at_y)), extrapolated = 2 x_next - x */
 int32_t sk_gpu_pdhg_primal_step(int32_t n, double tau, const double* d_x, const double* d_cost,

@@ -107,6 +107,7 @@ SIGNATURES = {
     "sk_gpu_memset": (i32, [vp, i32, C.c_size_t]),
     "sk_gpu_axpy": (i32, [i32, f64, vp, vp]),
     "sk_gpu_dot": (i32, [i32, vp, vp, P(f64)]),
+    "sk_gpu_dot_noalloc": (i32, [i32, vp, vp, P(f64)]),
     "sk_gpu_pdhg_primal_step": (i32, [i32, f64, vp, vp, vp, vp, vp, vp, vp]),
     "sk_gpu_spmv_csc": (i32, [i32, i32, vp, vp, vp, vp, vp, f64, i32]),
     "sk_gpu_spmv_csr": (i32, [i32, i32, vp, vp, vp, vp, vp, f64]),

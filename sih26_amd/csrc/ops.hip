@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <string>
 
 #include "../../include/sk_gpu.h"
@@ -33,6 +34,9 @@ unsigned grid_for(int64_t n, int per_block) {
 // Grow-only device scratch for the two-stage dot product.
 double* g_dot_scratch = nullptr;
 size_t g_dot_scratch_count = 0;
+
+// Stage-1 workgroups of the dot product cover this many elements each.
+constexpr int32_t kDotPerBlock = 8 * kBlock;
 
 __global__ __launch_bounds__(kBlock) void k_axpy(int32_t n, double alpha, const double* __restrict__ x,
                                                   double* __restrict__ y) {
@@ -106,6 +110,40 @@ __global__ __launch_bounds__(kBlock) void k_csc_scatter(int32_t cols, const int3
   const double xj = alpha * x[j];
   if (xj == 0.0) return;
   for (int p = ptr[j]; p < ptr[j + 1]; ++p) atomicAdd(&y[idx[p]], val[p] * xj);
+}
+
+// ---- the dot product without allocation (sk_gpu_dot_noalloc) ----------------------------
+// Its scratch is a fixed device array of the library: slot 0 the result, then one partial per
+// stage-1 workgroup. Each device holds its own copy until the process ends; every call on a
+// device shares that device's copy (calls that overlap on one device race on it). Nothing
+// needs clearing: stage 1 writes every partial stage 2 reads, and stage 2 writes the result.
+constexpr int32_t kNoAllocDotBlocks = 131072;  // n <= 131,072 * 2,048 = 268,435,456
+static __device__ __align__(256) double scratch[kNoAllocDotBlocks + 1];
+
+// `scratch` in host code names the host-side stand-in of the array, not device memory: its
+// device address is looked up once per device (with that device current) and kept here.
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+
+// The address of the current device's `scratch` (set-up on the first call on each device).
+int scratch_address(double** out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return fail(e, "hipGetDevice", kStatusNoDevice);
+  if (dev < 0 || dev >= kMaxDevices) {
+    set_last_error("sk_gpu_dot_noalloc: device index beyond the library's table");
+    return kStatusInvalidState;
+  }
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) return fail(e, "hipGetSymbolAddress(scratch)", kStatusInitFailed);
+    g_scratch_ptr[dev] = static_cast<double*>(ptr);
+  }
+  *out = g_scratch_ptr[dev];
+  return kStatusSuccess;
 }
 
 }  // namespace
@@ -229,7 +267,7 @@ int32_t sk_gpu_dot(int32_t n, const double* d_x, const double* d_y, double* host
   *host_result = 0.0;
   if (n <= 0) return kStatusSuccess;
   if (!d_x || !d_y) return kStatusInvalidState;
-  const int32_t per_block = 8 * kBlock;
+  const int32_t per_block = kDotPerBlock;
   const auto blocks = static_cast<int32_t>(grid_for(n, per_block));
   if (static_cast<size_t>(blocks) + 1 > g_dot_scratch_count) {
     if (g_dot_scratch) (void)hipFree(g_dot_scratch);
@@ -245,6 +283,30 @@ int32_t sk_gpu_dot(int32_t n, const double* d_x, const double* d_y, double* host
   k_dot_final<<<1, kBlock>>>(blocks, g_dot_scratch + 1, g_dot_scratch);
   if ((st = launch_status("k_dot_final"))) return st;
   const hipError_t e = hipMemcpy(host_result, g_dot_scratch, sizeof(double), hipMemcpyDeviceToHost);
+  return e == hipSuccess ? kStatusSuccess : fail(e, "hipMemcpy D2H", kStatusCopyFailed);
+}
+
+int32_t sk_gpu_dot_noalloc(int32_t n, const double* d_x, const double* d_y, double* host_result) {
+  if (!host_result) return kStatusInvalidArgument;
+  *host_result = 0.0;
+  if (n <= 0) return kStatusSuccess;
+  if (!d_x || !d_y) return kStatusInvalidState;
+  const auto blocks = static_cast<int32_t>(grid_for(n, kDotPerBlock));
+  if (blocks > kNoAllocDotBlocks) {
+    set_last_error("sk_gpu_dot_noalloc: n = " + std::to_string(n) + " exceeds the maximum of " +
+                   std::to_string(static_cast<int64_t>(kNoAllocDotBlocks) * kDotPerBlock) +
+                   " elements (its fixed scratch holds one partial per 2,048 elements)");
+    return kStatusInvalidArgument;
+  }
+  double* part = nullptr;
+  int st = scratch_address(&part);
+  if (st) return st;
+  // the same two kernels and partition as sk_gpu_dot, so the same bits
+  k_dot_partial<<<blocks, kBlock>>>(n, kDotPerBlock, d_x, d_y, part + 1);
+  if ((st = launch_status("k_dot_partial"))) return st;
+  k_dot_final<<<1, kBlock>>>(blocks, part + 1, part);
+  if ((st = launch_status("k_dot_final"))) return st;
+  const hipError_t e = hipMemcpy(host_result, part, sizeof(double), hipMemcpyDeviceToHost);
   return e == hipSuccess ? kStatusSuccess : fail(e, "hipMemcpy D2H", kStatusCopyFailed);
 }
 
