@@ -41,8 +41,8 @@ WORKLOADS = {
 KERNEL_NAMES = {0: "k_step_primal", 1: "k_step_dual", 2: "k_step_aty"}
 
 
-def solve_options(sk, precision: str = "fp32", **extra):
-    return sk.make_options(pdhg_tolerance=1e-4, pdhg_stop_at_request=True,
+def solve_options(sk, precision: str = "fp32", tolerance: float = 1e-4, **extra):
+    return sk.make_options(pdhg_tolerance=tolerance, pdhg_stop_at_request=True,
                            pdhg_absolute_primal=False, pdhg_precision=precision,
                            compute_quality=0, **extra)
 
@@ -304,6 +304,24 @@ def serve_stream(sk, requests: int, rate: float, slots: int, max_batch: int,
             "max_submit_lag_ms": 1e3 * late, "seconds": last - first}
 
 
+def precision_comparison(sk, lp, time_limit: float, use_graph: int) -> dict:
+    """The FP32/FP64 split engine against the plain FP64 engine on one LP, at relative KKT
+    1e-4 and 1e-6: iterations, solve time and the relative primal, dual and gap errors of
+    each answer as the engine's FP64 evaluation on the original model measures them (the
+    oracle's evaluate gives the same figures to 1e-7 relative: tests/test_gpu_configs.py)."""
+    out = {}
+    for tol in (1e-4, 1e-6):
+        for prec in ("fp32", "fp64"):
+            t0 = time.perf_counter()
+            r = sk.solve(lp, solve_options(sk, prec, tolerance=tol, time_limit=time_limit,
+                                           use_graph=use_graph))
+            out[f"{prec}@{tol:g}"] = {
+                "converged": bool(r.converged), "status": r.status_name,
+                "iterations": r.iterations, "seconds": time.perf_counter() - t0,
+                "rel_primal": r.rel_primal, "rel_dual": r.rel_dual, "rel_gap": r.rel_gap}
+    return out
+
+
 def polish_sample(sk, lps, results, count: int) -> dict:
     """The interior-point finish (sk_pdhg_polish) of the first `count` batch answers: the
     step from the 1e-4 first-order answer to an optimal one within 1e-6 relative objective."""
@@ -356,6 +374,14 @@ def main() -> int:
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--polish-sample", type=int, default=16,
                     help="batch answers handed to the interior-point finish (0: none)")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="launch every kernel from the host instead of replaying captured "
+                         "hipGraphs (for profiler runs; same kernels and answers)")
+    ap.add_argument("--iteration-limit", type=int, default=-1,
+                    help="cap every solve at this many PDHG iterations (counter-profiling runs "
+                         "only; capped solves are not the metric)")
+    ap.add_argument("--no-compare", action="store_true",
+                    help="skip the FP32/FP64 comparison of the single-LP workloads")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -372,7 +398,8 @@ def main() -> int:
         return main_serving(sk, args)
     lps = instances(sk, args.workload, args.instances)
     layout = dict(spmv_stage=args.spmv_stage, tile_nnz=args.tile_nnz, eval_stride=args.eval_stride,
-                  steps_per_block=args.steps_per_block)
+                  steps_per_block=args.steps_per_block, use_graph=0 if args.no_graph else 1,
+                  iteration_limit=args.iteration_limit)
     batch = sk.Batch(solve_options(sk, streams=args.streams, **layout))
     for lp in lps:
         batch.add(lp)
@@ -471,6 +498,8 @@ def main() -> int:
         f = fp64_single(sk, lps, args.fp64_budget)
         line["fp64_single"] = f
         line["speedup_vs_fp64_single"] = line["value"] / f["lps_per_s"]
+    if args.workload == "c2" and not args.no_compare:
+        line["precision_comparison"] = precision_comparison(sk, lps[0], 120.0, layout["use_graph"])
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(lps, args.cpu_budget)
     if not args.no_serve and args.workload == "c3":

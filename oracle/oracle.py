@@ -154,6 +154,29 @@ def solve_pdhg(lp: dict[str, Any], tolerance: float = 1e-4, iteration_limit: int
     return out
 
 
+QUALITY_FIELDS = ("objective", "primal_infeasibility", "primal_infeasibility_scaled",
+                  "dual_infeasibility", "dual_infeasibility_scaled", "complementarity_violation",
+                  "absolute_gap", "relative_gap")
+
+
+def recompute_quality(lp: dict[str, Any], col_value: np.ndarray, row_dual: np.ndarray,
+                      col_dual: np.ndarray, dual_bound: float) -> dict[str, Any]:
+    """model.cpp:228-422 (LP case) on a given answer in the Solution's signs: the quality
+    measures and the recomputed row activities."""
+    mdl, a = model(lp)
+    x = np.ascontiguousarray(col_value, np.float64)
+    yd = np.ascontiguousarray(row_dual, np.float64)
+    d = np.ascontiguousarray(col_dual, np.float64)
+    act = np.zeros(a["num_rows"])
+    sol = Solution()
+    sol.col_value, sol.col_dual, sol.row_dual, sol.row_activity = _p(x), _p(d), _p(yd), _p(act)
+    sol.dual_bound = dual_bound
+    lib().ok_recompute_quality(C.byref(mdl), C.byref(sol))
+    out = {f: getattr(sol, f) for f in QUALITY_FIELDS}
+    out["row_activity"] = act
+    return out
+
+
 def evaluate(lp: dict[str, Any], x: np.ndarray, y: np.ndarray) -> dict[str, Any]:
     """pdhg.cpp:143-263 on the unscaled model; y uses the engine's sign (row_dual = -sense y)."""
     mdl, a = model(lp)

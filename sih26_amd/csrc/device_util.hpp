@@ -157,11 +157,14 @@ __device__ __forceinline__ void rowblock_product(const Tile t, const int32_t* __
       A acc = zero_of<A>();
       constexpr int U = 4;
       for (int base = p0 + tid; base < p1; base += U * kWave) {
+        // the U products' loads are issued together: a position past the row reads its last
+        // nonzero and the product is replaced by zero
         A part[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
           const int p = base + u * kWave;
-          part[u] = p < p1 ? prod(p) : zero_of<A>();
+          const A v = prod(p < p1 ? p : p1 - 1);
+          part[u] = p < p1 ? v : zero_of<A>();
         }
         // in order: lane l adds nonzeros l, l + 64, l + 128, ... (adding a zero is exact)
 #pragma unroll
@@ -172,17 +175,21 @@ __device__ __forceinline__ void rowblock_product(const Tile t, const int32_t* __
     }
     return;
   }
+  if (nrows <= 0) return;  // an instance without rows has one empty tile (same for every lane)
   const bool long_rows = t.kind == kTileLong;
   const int wave = tid / kWave, lane = tid % kWave;
   // the lane that finalises a row loads its epilogue operands first, so those loads are in
-  // flight together with the product loads below
+  // flight together with the product loads below; the row pointers are loaded whether or not
+  // the lane has a row (a missing row reads the tile's last one), so no branch waits for them
   int b[R], e[R];
 #pragma unroll
   for (int k = 0; k < R; ++k) {
     const int row = long_rows ? wave + W * k : tid + NT * k;
     if (row < nrows && (!long_rows || lane == 0)) pre(r0 + row, k);
-    b[k] = row < nrows ? ptr[r0 + row] - p0 : 0;
-    e[k] = row < nrows ? ptr[r0 + row + 1] - p0 : 0;
+    const int rr = r0 + (row < nrows ? row : nrows - 1);
+    const int pb = ptr[rr], pe = ptr[rr + 1];
+    b[k] = row < nrows ? pb - p0 : 0;
+    e[k] = row < nrows ? pe - p0 : 0;
   }
   stager(t, stage);
   __syncthreads();

@@ -33,6 +33,14 @@
 
 namespace sk {
 
+// Every kernel of the engine rounds each product before it is added, as the reference's
+// arithmetic does (a multiply, then an add: pdhg.cpp, scaling.cpp, sparse.cpp): the compiler
+// may not fuse them into an FMA. It matters beyond the last bit: in the dual step
+// y' = v - sigma proj(v / sigma) an FMA keeps the division's rounding residual, so a row whose
+// bound is inactive gets a tiny nonzero y' where the reference gets exactly zero, and the
+// step-size rule then sees information where the reference sees none.
+#pragma clang fp contract(off)
+
 // ---- kernel arguments ----------------------------------------------------------------------
 
 // Everything a kernel of the engine may touch, passed by value. T is the iterate precision
@@ -235,15 +243,19 @@ __device__ __forceinline__ void load4(const uint16_t* __restrict__ p, int v[4]) 
 }
 
 // Stager of rowblock_product over the staging arrays, in groups of four stored positions: each
-// lane loads four index words (and, with HAS_VAL, four staged values) with one 16-byte load per
-// array, so a wavefront reads 1 KB of each contiguously, then stages f(c, slot, v) for each of
-// the four entries (c its gather index, slot its CSR-order slot, v its staged value). The
-// stored order (group_rank) makes the e-th gathers of a wavefront's lanes consecutive sorted
-// indices. Groups start at multiples of four, so a tile's first and last groups also cover
-// positions of the neighbouring tiles, which are loaded and ignored; the staging arrays hold
-// 16-byte aligned storage for at least kStagePad more elements than nonzeros, so the last group
-// of the last tile stays inside them.
-template <int NT, class A, bool PACK, bool HAS_VAL, class V, class F>
+// lane loads four index words and four staged values with one 16-byte load per array, so a
+// wavefront reads 1 KB of each contiguously, then stages f(c, slot, v) for each of the four
+// entries (c its gather index, slot its CSR-order slot, v its staged value). The stored order
+// (group_rank) makes the e-th gathers of a wavefront's lanes consecutive sorted indices.
+// Groups start at multiples of four, so a tile's first and last groups also cover positions of
+// the neighbouring tiles, which are loaded and ignored; the staging arrays hold 16-byte aligned
+// storage for at least kStagePad more elements than nonzeros, so the last group of the last
+// tile stays inside them.
+// The four gathers of a group are issued together: an entry outside the tile gathers at the
+// tile's base (the first gather index of its instance, which exists because the tile has a
+// nonzero) and its product is dropped, so no gather waits in a branch of its own for the one
+// before it.
+template <int NT, class A, bool PACK, class V, class F>
 struct GroupStager {
   const int32_t* __restrict__ sidx;
   const uint16_t* __restrict__ pos;
@@ -251,6 +263,7 @@ struct GroupStager {
   F f;
   __device__ __forceinline__ void operator()(const Tile& t, A* stage) const {
     const int p0 = t.p0, p1 = t.p1, qa = p0 & ~3;
+    if (p1 <= p0) return;
     const int groups = (p1 - qa + 3) >> 2;
     constexpr int U = NT >= kBlock ? 1 : 4;  // groups per lane in flight (measured best)
     for (int g0 = 0; g0 < groups; g0 += U * NT) {
@@ -259,34 +272,38 @@ struct GroupStager {
 #pragma unroll
       for (int u = 0; u < U; ++u) {
         const int g = g0 + u * NT + static_cast<int>(threadIdx.x);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          w[u][e] = 0;
+          sl[u][e] = 0;
+          v[u][e] = V{};
+        }
         if (g < groups) {
           load4(sidx + qa + 4 * g, w[u]);
-          if (HAS_VAL) load4(val + qa + 4 * g, v[u]);
+          load4(val + qa + 4 * g, v[u]);
           if (!PACK) load4(pos + qa + 4 * g, sl[u]);
         }
       }
       A prod[U][4];
+      bool own[U][4];
 #pragma unroll
       for (int u = 0; u < U; ++u) {
         const int g = g0 + u * NT + static_cast<int>(threadIdx.x);
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
           const int q = qa + 4 * g + e;
-          if (g < groups && q >= p0 && q < p1) {
-            const uint32_t word = static_cast<uint32_t>(w[u][e]);
-            const int c = PACK ? t.base + static_cast<int>(word >> kPackShift) : w[u][e];
-            if (PACK) sl[u][e] = static_cast<int>(word & 0xFFFu);
-            prod[u][e] = f(c, sl[u][e], v[u][e]);
-          }
+          own[u][e] = g < groups && q >= p0 && q < p1;
+          const uint32_t word = static_cast<uint32_t>(w[u][e]);
+          const int c = PACK ? t.base + static_cast<int>(word >> kPackShift) : w[u][e];
+          if (PACK) sl[u][e] = static_cast<int>(word & 0xFFFu);
+          prod[u][e] = f(own[u][e] ? c : t.base, sl[u][e], v[u][e]);
         }
       }
 #pragma unroll
       for (int u = 0; u < U; ++u) {
-        const int g = g0 + u * NT + static_cast<int>(threadIdx.x);
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
-          const int q = qa + 4 * g + e;
-          if (g < groups && q >= p0 && q < p1) stage[sl[u][e]] = prod[u][e];
+          if (own[u][e]) stage[sl[u][e]] = prod[u][e];
         }
       }
     }
@@ -297,13 +314,7 @@ template <int NT, class A, bool PACK, class V, class X>
 __device__ __forceinline__ auto gather_stager(const int32_t* sidx, const uint16_t* pos,
                                               const V* val, const X* x) {
   auto f = [x](int c, int, V v) { return static_cast<A>(v) * static_cast<A>(x[c]); };
-  return GroupStager<NT, A, PACK, true, V, decltype(f)>{sidx, pos, val, f};
-}
-// stages f(c, slot) (no staged values) with NT lanes
-template <int NT, class A, bool PACK, class F>
-__device__ __forceinline__ auto slot_stager(const int32_t* sidx, const uint16_t* pos, F g) {
-  auto f = [g](int c, int slot, int) { return g(c, slot); };
-  return GroupStager<NT, A, PACK, false, int, decltype(f)>{sidx, pos, nullptr, f};
+  return GroupStager<NT, A, PACK, V, decltype(f)>{sidx, pos, val, f};
 }
 
 // CAP: the tile capacity the tables were built for, which sizes the LDS staging buffer (and
@@ -628,7 +639,7 @@ __device__ __forceinline__ void eval_rows_tile(const Args<T>& a, const int tile,
   double lo_[R], hi_[R], ya_[R], yb_[R];
   rowblock_product<kBlock, D2>(
       t, a.a_ptr, prod,
-      GroupStager<kBlock, D2, PACK, true, double, decltype(f)>{sidx, a.a_pos, val, f},
+      GroupStager<kBlock, D2, PACK, double, decltype(f)>{sidx, a.a_pos, val, f},
       [&](int i, int k) {
         lo_[k] = a.rl[i];
         hi_[k] = a.ru[i];
@@ -778,7 +789,7 @@ __device__ __forceinline__ void eval_cols_tile(const Args<T>& a, const int tile,
   double lo_[R], hi_[R], c_[R], xa_[R], xb_[R];
   rowblock_product<kBlock, D2>(
       t, a.t_ptr, prod,
-      GroupStager<kBlock, D2, PACK, true, double, decltype(f)>{sidx, a.t_pos, val, f},
+      GroupStager<kBlock, D2, PACK, double, decltype(f)>{sidx, a.t_pos, val, f},
       [&](int j, int k) {
         lo_[k] = a.cl[j];
         hi_[k] = a.cu[j];
@@ -921,8 +932,6 @@ __global__ __launch_bounds__(kBlock) void k_eval_commit(Args<T> a) {
 }
 
 // ---- set-up kernels (FP64, no FMA contraction: bit-for-bit the reference's arithmetic) -------
-
-#pragma clang fp contract(off)
 
 // Ruiz (max) or Pock-Chambolle (sum) statistic of every row of a CSR matrix, as the pass
 // multiplier 1/sqrt(stat) (1 for an empty row). Sums run in stored order, which is the
@@ -1213,8 +1222,6 @@ __global__ __launch_bounds__(kBlock) void k_chunk_div(const Tile* __restrict__ t
   if (len == 0.0) return;
   for (int k = t.begin + threadIdx.x; k < t.end; k += kBlock) v[k] /= len;
 }
-
-#pragma clang fp contract(fast)
 
 // ---- initial state (pdhg.cpp:327-379) ----------------------------------------------------------
 

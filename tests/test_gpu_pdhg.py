@@ -105,6 +105,72 @@ def test_fp64_engine_tracks_the_oracle_on_c1(gpu, oracle_mod):
     assert ev["worst"] <= 1e-6 * 1.000001
 
 
+def trajectory_case(gpu, name):
+    if name == "c1":
+        return gpu.Lp.generate(1, 1).export()
+    if name.startswith("c3_"):
+        return gpu.Lp.generate(3, int(name[3:])).export()
+    return closed_form(name)[0]
+
+
+# cost multipliers from one ulp to 1e-12 away from 1: the oracle run on each shows how far
+# rounding-size perturbations move its own trajectory (counts and iterates)
+ULP_MOVES = (1 + 2.0 ** -52, 1 - 2.0 ** -53, 1 + 2.0 ** -51, 1 - 2.0 ** -52, 1 + 1e-13,
+             1 - 1e-13, 1 + 1e-12, 1 - 1e-12)
+
+
+@pytest.mark.parametrize("limit", [100, 200, 400])
+@pytest.mark.parametrize("name", ["c1", "c3_5"] + [c[0] for c in CLOSED_FORM])
+def test_fp64_engine_follows_the_oracle_trajectory(gpu, oracle_mod, name, limit):
+    """Same method, FP64 both sides, restarts on, the same iteration limit: the step-size
+    rule (pdhg.cpp:454-496), the evaluation gate and the restart rule (:599-643) must make
+    the oracle's decisions. A different reduction order is a rounding-size perturbation, which
+    the method can amplify (a near-tie in an accept, a no-information step, a candidate
+    choice), so the bar is the oracle's own behaviour when its costs move by one ulp to 1e-12:
+    the engine's iterations, restarts and evaluations lie within the range those runs give
+    (one value where the trajectory is stable), its status is theirs, and where their reported
+    points agree to 1e-6 of scale the engine's is within 10x their spread (1e-9 where closer).
+    Where rounding alone moves the oracle's point further (the 500 x 1000 LP after ~300
+    iterations: ~1e-3), only the decisions are compared."""
+    d = trajectory_case(gpu, name)
+    ref = oracle_mod.solve_pdhg(d, tolerance=1e-12, iteration_limit=limit)
+    runs = [ref] + [oracle_mod.solve_pdhg(dict(d, cost=d["cost"] * f), tolerance=1e-12,
+                                          iteration_limit=limit) for f in ULP_MOVES]
+    r = gpu.solve(to_lp(gpu, d), pdhg_tolerance=1e-12, iteration_limit=limit,
+                  pdhg_precision="fp64")
+    for k in ("iterations", "restarts", "evaluations"):
+        lo, hi = min(x[k] for x in runs), max(x[k] for x in runs)
+        print(f"{name} {limit} {k}: gpu {getattr(r, k)}, oracle {ref[k]} (range {lo}..{hi})")
+        assert lo <= getattr(r, k) <= hi, (k, getattr(r, k), lo, hi)
+    assert r.status in {x["status"] for x in runs}, r.info["message"]
+    for k, got in (("col_value", r.col_value), ("row_dual", r.row_dual), ("col_dual", r.col_dual)):
+        want = ref[k]
+        if not want.size:
+            continue
+        spread = max(float(np.max(np.abs(x[k] - want))) for x in runs[1:])
+        scale = max(1.0, float(np.max(np.abs(want))))
+        diff = float(np.max(np.abs(got - want)))
+        print(f"{name} {limit} {k}: |gpu - oracle| {diff:.3e}, oracle spread {spread:.3e}")
+        if spread <= 1e-6 * scale:
+            assert diff <= max(10.0 * spread, 1e-9 * scale), (k, diff, spread)
+    if ref["status"] == 1:
+        assert abs(r.objective - ref["objective"]) <= 1e-9 * max(1.0, abs(ref["objective"]))
+    assert abs(r.spectral_norm - ref["spectral_norm"]) <= 1e-12 * ref["spectral_norm"]
+
+
+@pytest.mark.parametrize("name", ["c1", "c3_5", "maximise", "offset"])
+def test_quality_measures_are_the_oracle_recompute_quality(gpu, oracle_mod, name):
+    """sk_pdhg_result's Solution quality measures (model.cpp:228-422) on the reported point
+    are the oracle's recompute_quality of the same point and the same dual bound."""
+    d = trajectory_case(gpu, name)
+    r = gpu.solve(to_lp(gpu, d), pdhg_tolerance=1e-6, pdhg_stop_at_request=True,
+                  pdhg_absolute_primal=False, pdhg_precision="fp64", iteration_limit=200000)
+    q = oracle_mod.recompute_quality(d, r.col_value, r.row_dual, r.col_dual, r.dual_bound)
+    for f in oracle_mod.QUALITY_FIELDS:
+        assert r.info[f] == pytest.approx(q[f], rel=1e-12, abs=1e-300), f
+    np.testing.assert_allclose(r.row_activity, q["row_activity"], rtol=1e-12, atol=1e-12)
+
+
 def test_fp32_split_meets_1e4_on_c1(gpu, oracle_mod):
     d = gpu.Lp.generate(1, 1).export()
     tight = oracle_mod.solve_pdhg(d, tolerance=1e-8, stop_at_request=True, absolute_primal=False)

@@ -305,3 +305,16 @@ def test_recompute_quality_by_hand():
     # complementarity: |d0| * distance(x0, 0) = 1 * 5; row: |y| * (4.5 - 2) = 0.625
     assert sol.complementarity_violation == 5.0
     assert sol.absolute_gap == 2.5 and sol.relative_gap == 2.5 / 4.5
+
+
+def test_recompute_quality_wrapper_reproduces_the_solve_report(oracle_mod):
+    """oracle.recompute_quality on the oracle's own answer gives the quality fields its
+    solve_pdhg reported (both model.cpp:228-422)."""
+    import sih26_amd as sk
+    d = sk.Lp.generate(1, 2).export()
+    r = oracle_mod.solve_pdhg(d, tolerance=1e-5, stop_at_request=True, absolute_primal=False)
+    q = oracle_mod.recompute_quality(d, r["col_value"], r["row_dual"], r["col_dual"],
+                                     r["dual_bound"])
+    for f in oracle_mod.QUALITY_FIELDS:
+        assert q[f] == r[f], f
+    np.testing.assert_array_equal(q["row_activity"], r["row_activity"])
