@@ -70,6 +70,15 @@ int32_t sk_gpu_set_device(int32_t device_id);
 int32_t sk_gpu_synchronize(void);
 /* gpu_runtime.hpp:85-86 get_last_error_string (thread-local; valid until the next call) */
 const char* sk_gpu_last_error(void);
+/* Fault injection, for tests of the failure paths from the host (no counterpart in the
+ * reference, whose tests force failures only through a missing device and the limits). kind 1:
+ * the n-th device allocation an engine makes from now on fails with SK_GPU_OUT_OF_MEMORY; kind
+ * 2: the n-th kernel launch an engine makes from now on fails with SK_GPU_LAUNCH_FAILED (the
+ * launch is not made, so no kernel runs in a bad state); kind 3: instance n of the next batch
+ * upload gets a NaN cost in its first column, so its first step is non-finite and it ends with
+ * SK_SOLVE_NUMERICAL_ERROR while the other instances run on; kind 0 clears every injection.
+ * Process-wide; each injection fires once. */
+int32_t sk_fault_inject(int32_t kind, int64_t n);
 /* gpu_runtime.hpp:90-105 allocate/free/copy/memset */
 int32_t sk_gpu_malloc(void** dev_ptr, size_t bytes);
 int32_t sk_gpu_free(void* dev_ptr);
@@ -98,6 +107,19 @@ int32_t sk_gpu_pdhg_primal_step(int32_t n, double tau, const double* d_x, const 
                                 const double* d_at_y, const double* d_col_lower,
                                 const double* d_col_upper, double* d_x_next,
                                 double* d_extrapolated);
+/* gpu_ops.hpp:85-91 execute_pdhg_primal_step, the validated dispatcher (gpu_ops_cpu.cpp:126-207),
+ * on HOST arrays (synchronous): device buffers, copies in, the primal step, a device
+ * synchronisation, copies out, then the reference's check that every output is finite. Each
+ * failure class returns its status as the reference's does: SK_GPU_UNAVAILABLE without a
+ * device, SK_GPU_OUT_OF_MEMORY, SK_GPU_COPY_FAILED, SK_GPU_LAUNCH_FAILED, SK_GPU_SYNC_FAILED,
+ * SK_GPU_NON_FINITE. Unlike the reference it does not then compute the step on the CPU: this
+ * library has no CPU path, so on any status but success the outputs are unspecified and a
+ * caller that wants the fallback runs its own CPU step (the reference keeps
+ * cpu_pdhg_primal_step, gpu_ops_cpu.cpp:37-49, for exactly that). */
+int32_t sk_gpu_execute_pdhg_primal_step(int32_t n, double tau, const double* x, const double* cost,
+                                        const double* at_y, const double* col_lower,
+                                        const double* col_upper, double* x_next,
+                                        double* extrapolated);
 /* gpu_ops.hpp:64-67 launch_gpu_spmv_csc on a CSC matrix.
  *   transpose = 1: y = alpha A^T x, one wavefront per column (deterministic).
  *   transpose = 0: y = alpha A x by column scatter with FP64 atomics, as the reference's
@@ -236,19 +258,88 @@ typedef struct {
                                 tiles and 2 column chunks, one workgroup per instance runs a
                                 whole PDHG step in one launch, with the same per-tile code and
                                 bit for bit the same answers as the four launches; 1 never */
-  int32_t reserved2;
+  int32_t backend;           /* backend (options.cpp:228-236): SK_BACKEND_AUTO (default), _CPU,
+                                _GPU or _HYBRID. Read by sk_select_backend; a solve, batch or
+                                server of this library refuses _CPU (the CPU engine is the
+                                reference's) and _HYBRID (unsupported, as in the reference) */
+  int32_t gpu;               /* the legacy gpu flag (options.cpp:220-227), default 0 */
 } sk_pdhg_options;
 
-/* The defaults listed above (options.cpp:83-106, 284-290, 328-334, 395-418). */
+/* The defaults listed above (options.cpp:83-106, 220-236, 284-290, 328-334, 395-418). */
 void sk_pdhg_default_options(sk_pdhg_options* options);
 /* String-keyed setter over the reference's option names (src/util/options.cpp:78-455):
- * algorithm (must be "pdhg"), backend ("gpu"|"auto"), gpu, pdhg_tolerance, iteration_limit,
+ * algorithm (must be "pdhg"), backend (auto, cpu, gpu or cuda, hybrid, as parse_backend_mode,
+ * adaptive.hpp:43-62), gpu, pdhg_tolerance, iteration_limit,
  * time_limit, pdhg_restart, pdhg_stop_at_request, random_seed, pdhg_polish (validated; as in
  * the reference, the polish is a separate step after the first-order solve: sk_pdhg_polish),
  * plus pdhg_precision ("fp32"|"fp64"),
  * pdhg_absolute_primal and pdhg_steps_per_block. Unknown keys and malformed values are
  * rejected with SK_GPU_INVALID_ARGUMENT, as options.cpp:481-565 rejects them. */
 int32_t sk_pdhg_set_option(sk_pdhg_options* options, const char* key, const char* value);
+
+/* ---- backend selection (include/sankhya/adaptive.hpp:22-121; src/core/adaptive.cpp:32-232) --
+ * The reference's adaptive CPU/GPU decision, restated for callers of this library: the same
+ * modes, the same order of tests and thresholds, the same validity rules (an explicit GPU
+ * request without a device is invalid, never a silent fallback; the legacy gpu flag falls
+ * back to the CPU; hybrid is unsupported). A CPU decision means the reference's CPU engine:
+ * this library only runs the GPU engine. */
+#define SK_BACKEND_AUTO 0
+#define SK_BACKEND_CPU 1
+#define SK_BACKEND_GPU 2
+#define SK_BACKEND_HYBRID 3
+
+/* ProblemCharacteristics (adaptive.hpp:64-83) */
+typedef struct {
+  char problem_class[8]; /* "LP", "MILP", "QP" or "MIQP" */
+  int32_t num_rows;
+  int32_t num_cols;
+  int64_t num_nonzeros;
+  double density;        /* nnz / (rows cols) */
+  double sparsity_ratio; /* 1 - density */
+  int32_t num_integer_cols;
+  int32_t has_quadratic_objective;
+  double estimated_matvec_flops;  /* 2 nnz */
+  double estimated_memory_bytes;  /* CSC (int32 index + FP64 value) + 4 FP64 vectors */
+  int32_t gpu_available;
+  int32_t gpu_device_count;
+  uint64_t gpu_memory_bytes;
+  char gpu_name[256];
+} sk_problem_characteristics;
+
+/* AdaptiveConfig (adaptive.hpp:85-93) */
+typedef struct {
+  int64_t min_gpu_nonzeros;       /* adaptive_min_nnz, default 50,000 */
+  int64_t min_gpu_dimension;      /* adaptive_min_dim, default 1,000 */
+  double min_gpu_density;         /* adaptive_min_density, default 1e-6 (read, as in the
+                                     reference, but no test uses it) */
+  double max_gpu_memory_headroom; /* 0.8 */
+  int32_t explain_decision;       /* adaptive_explain, default 1 */
+  int32_t reserved;
+} sk_adaptive_config;
+
+/* BackendDecision (adaptive.hpp:96-104) and its format_explanation (adaptive.cpp:15-29) */
+typedef struct {
+  int32_t selected_backend; /* SK_BACKEND_CPU, _GPU or _HYBRID */
+  int32_t requested_mode;   /* SK_BACKEND_* */
+  int32_t is_valid;         /* 0: the requested backend cannot run; the reference then returns
+                               kNotSolved with primary_reason as the message (solve.cpp:359-366) */
+  int32_t detail_count;
+  char primary_reason[512];
+  char explanation[2048];
+} sk_backend_decision;
+
+void sk_adaptive_default_config(sk_adaptive_config* config);
+/* load_adaptive_config's keys (options.cpp:237-264): adaptive_min_nnz, adaptive_min_dim,
+ * adaptive_min_density, adaptive_explain, with the registry's bounds. */
+int32_t sk_adaptive_set_option(sk_adaptive_config* config, const char* key, const char* value);
+/* analyze_problem (adaptive.cpp:32-81) of an LP; integer_columns and has_quadratic_objective
+ * are what the model's source says (sk_lp_read_mps reports the integer columns), the device
+ * fields are queried from the HIP runtime (device 0). */
+int32_t sk_analyze_problem(const sk_lp* lp, int32_t integer_columns,
+                           int32_t has_quadratic_objective, sk_problem_characteristics* out);
+/* select_backend (adaptive.cpp:98-232), reading options->backend and options->gpu. */
+int32_t sk_select_backend(const sk_problem_characteristics* chars, const sk_pdhg_options* options,
+                          const sk_adaptive_config* config, sk_backend_decision* out);
 
 /* Residuals (pdhg.cpp:81-132) of the reported point and Solution quality measures
  * (model.cpp:228-422). */

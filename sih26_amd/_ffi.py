@@ -44,7 +44,27 @@ class PdhgOptions(C.Structure):
                 ("random_seed", i64), ("precision", i32), ("steps_per_block", i32),
                 ("use_graph", i32), ("spmv_stage", i32), ("tile_nnz", i32),
                 ("compute_quality", i32), ("eval_stride", i32), ("streams", i32),
-                ("column_order", i32), ("step_fusion", i32), ("reserved2", i32)]
+                ("column_order", i32), ("step_fusion", i32), ("backend", i32), ("gpu", i32)]
+
+
+class ProblemCharacteristics(C.Structure):
+    _fields_ = [("problem_class", C.c_char * 8), ("num_rows", i32), ("num_cols", i32),
+                ("num_nonzeros", i64), ("density", f64), ("sparsity_ratio", f64),
+                ("num_integer_cols", i32), ("has_quadratic_objective", i32),
+                ("estimated_matvec_flops", f64), ("estimated_memory_bytes", f64),
+                ("gpu_available", i32), ("gpu_device_count", i32), ("gpu_memory_bytes", u64),
+                ("gpu_name", C.c_char * 256)]
+
+
+class AdaptiveConfig(C.Structure):
+    _fields_ = [("min_gpu_nonzeros", i64), ("min_gpu_dimension", i64), ("min_gpu_density", f64),
+                ("max_gpu_memory_headroom", f64), ("explain_decision", i32), ("reserved", i32)]
+
+
+class BackendDecision(C.Structure):
+    _fields_ = [("selected_backend", i32), ("requested_mode", i32), ("is_valid", i32),
+                ("detail_count", i32), ("primary_reason", C.c_char * 512),
+                ("explanation", C.c_char * 2048)]
 
 
 class PdhgResult(C.Structure):
@@ -100,6 +120,7 @@ SIGNATURES = {
     "sk_gpu_set_device": (i32, [i32]),
     "sk_gpu_synchronize": (i32, []),
     "sk_gpu_last_error": (cstr, []),
+    "sk_fault_inject": (i32, [i32, i64]),
     "sk_gpu_malloc": (i32, [P(vp), C.c_size_t]),
     "sk_gpu_free": (i32, [vp]),
     "sk_gpu_copy_h2d": (i32, [vp, vp, C.c_size_t]),
@@ -109,6 +130,7 @@ SIGNATURES = {
     "sk_gpu_dot": (i32, [i32, vp, vp, P(f64)]),
     "sk_gpu_dot_noalloc": (i32, [i32, vp, vp, P(f64)]),
     "sk_gpu_pdhg_primal_step": (i32, [i32, f64, vp, vp, vp, vp, vp, vp, vp]),
+    "sk_gpu_execute_pdhg_primal_step": (i32, [i32, f64, vp, vp, vp, vp, vp, vp, vp]),
     "sk_gpu_spmv_csc": (i32, [i32, i32, vp, vp, vp, vp, vp, f64, i32]),
     "sk_gpu_spmv_csr": (i32, [i32, i32, vp, vp, vp, vp, vp, f64]),
     "sk_gpu_compare_vectors": (i32, [i32, vp, vp, f64, P(Comparison)]),
@@ -121,6 +143,11 @@ SIGNATURES = {
     "sk_lp_free": (None, [vp]),
     "sk_pdhg_default_options": (None, [P(PdhgOptions)]),
     "sk_pdhg_set_option": (i32, [P(PdhgOptions), cstr, cstr]),
+    "sk_adaptive_default_config": (None, [P(AdaptiveConfig)]),
+    "sk_adaptive_set_option": (i32, [P(AdaptiveConfig), cstr, cstr]),
+    "sk_analyze_problem": (i32, [vp, i32, i32, P(ProblemCharacteristics)]),
+    "sk_select_backend": (i32, [P(ProblemCharacteristics), P(PdhgOptions), P(AdaptiveConfig),
+                                P(BackendDecision)]),
     "sk_pdhg_solve": (i32, [vp, P(PdhgOptions), P(PdhgResult), vp, vp, vp, vp]),
     "sk_write_solution": (i32, [cstr, vp, P(PdhgResult), vp, vp, vp, vp, vp, vp, cstr]),
     "sk_ipm_default_options": (None, [P(IpmOptions)]),

@@ -191,6 +191,45 @@ def make_options(**kw) -> _ffi.PdhgOptions:
     return o
 
 
+def adaptive_config(**kw) -> _ffi.AdaptiveConfig:
+    """AdaptiveConfig (sk_adaptive_config) with the reference defaults; keywords are struct
+    fields or the registry's adaptive_* option names."""
+    c = _ffi.AdaptiveConfig()
+    lib().sk_adaptive_default_config(C.byref(c))
+    fields = {f for f, _ in c._fields_}
+    for k, v in kw.items():
+        if k in fields:
+            setattr(c, k, v)
+        else:
+            val = ("true" if v else "false") if isinstance(v, bool) else str(v)
+            check(lib().sk_adaptive_set_option(C.byref(c), k.encode(), val.encode()), f"option {k}")
+    return c
+
+
+def analyze_problem(lp: "Lp", integer_columns: int = 0,
+                    quadratic: bool = False) -> _ffi.ProblemCharacteristics:
+    """analyze_problem (src/core/adaptive.cpp:32-81) of an LP, with the HIP device's facts."""
+    ch = _ffi.ProblemCharacteristics()
+    check(lib().sk_analyze_problem(lp._h, integer_columns, int(quadratic), C.byref(ch)),
+          "sk_analyze_problem")
+    return ch
+
+
+BACKENDS = {0: "auto", 1: "cpu", 2: "gpu", 3: "hybrid"}
+
+
+def select_backend(chars: _ffi.ProblemCharacteristics, options: _ffi.PdhgOptions | None = None,
+                   config: _ffi.AdaptiveConfig | None = None) -> dict[str, Any]:
+    """select_backend (src/core/adaptive.cpp:98-232): which engine should run."""
+    d = _ffi.BackendDecision()
+    check(lib().sk_select_backend(C.byref(chars), C.byref(options) if options else None,
+                                  C.byref(config) if config else None, C.byref(d)),
+          "sk_select_backend")
+    return {"selected": BACKENDS[d.selected_backend], "requested": BACKENDS[d.requested_mode],
+            "valid": bool(d.is_valid), "reason": d.primary_reason.decode(),
+            "explanation": d.explanation.decode(), "details": d.detail_count}
+
+
 @dataclass
 class SolveResult:
     status: int

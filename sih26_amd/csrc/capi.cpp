@@ -75,6 +75,25 @@ sk::EngineOptions to_engine(const sk_pdhg_options& o) {
   return e;
 }
 
+// A solve, batch or server of this library runs the GPU engine: backend=cpu selects the
+// reference's CPU engine and backend=hybrid is unsupported (adaptive.cpp:141-166).
+int32_t refuse_backend(const sk_pdhg_options& o) {
+  if (o.backend == SK_BACKEND_CPU) {
+    sk::set_last_error("backend=cpu selects the CPU engine, which is not part of this library "
+                       "(it is the GPU engine; see sk_select_backend)");
+    return SK_GPU_INVALID_ARGUMENT;
+  }
+  if (o.backend == SK_BACKEND_HYBRID) {
+    sk::set_last_error("backend=hybrid is unsupported in this version");
+    return SK_GPU_INVALID_ARGUMENT;
+  }
+  if (o.backend != SK_BACKEND_AUTO && o.backend != SK_BACKEND_GPU) {
+    sk::set_last_error("backend must be one of SK_BACKEND_*");
+    return SK_GPU_INVALID_ARGUMENT;
+  }
+  return SK_GPU_SUCCESS;
+}
+
 std::string lower(const char* s) {
   std::string r(s ? s : "");
   for (auto& c : r) c = static_cast<char>(std::tolower(static_cast<unsigned char>(c)));
@@ -430,11 +449,17 @@ int32_t sk_pdhg_set_option(sk_pdhg_options* o, const char* key_in, const char* v
     return SK_GPU_SUCCESS;
   }
   if (key == "backend") {
-    if (v != "gpu" && v != "auto") return bad("this library is the gpu backend; cannot take");
+    // parse_backend_mode (adaptive.hpp:43-62); the registry's values (options.cpp:228-236)
+    if (v == "auto") o->backend = SK_BACKEND_AUTO;
+    else if (v == "cpu") o->backend = SK_BACKEND_CPU;
+    else if (v == "gpu" || v == "cuda") o->backend = SK_BACKEND_GPU;
+    else if (v == "hybrid") o->backend = SK_BACKEND_HYBRID;
+    else return bad("expected auto, cpu, gpu or hybrid, got");
     return SK_GPU_SUCCESS;
   }
   if (key == "gpu") {
     if (!parse_bool(v, &b)) return bad("expected a boolean, got");
+    o->gpu = b;
     return SK_GPU_SUCCESS;
   }
   if (key == "pdhg_polish") {
@@ -492,6 +517,7 @@ int32_t sk_pdhg_solve(const sk_lp* lp, const sk_pdhg_options* options, sk_pdhg_r
     sk_pdhg_options o;
     if (options) o = *options;
     else sk_pdhg_default_options(&o);
+    if (const int32_t refused = refuse_backend(o)) return refused;
     const auto t0 = std::chrono::steady_clock::now();
     auto engine = sk::make_engine(to_engine(o));
     engine->add(&lp->lp);
@@ -512,6 +538,7 @@ int32_t sk_batch_create(const sk_pdhg_options* options, sk_batch** out) {
     auto b = std::make_unique<sk_batch>();
     if (options) b->options = *options;
     else sk_pdhg_default_options(&b->options);
+    if (const int32_t refused = refuse_backend(b->options)) return refused;
     b->engine = sk::make_engine(to_engine(b->options));
     *out = b.release();
     return SK_GPU_SUCCESS;
@@ -767,6 +794,7 @@ int32_t sk_server_create(const sk_pdhg_options* options, const sk_server_options
     auto s = std::make_unique<sk_server>();
     if (options) s->options = *options;
     else sk_pdhg_default_options(&s->options);
+    if (const int32_t refused = refuse_backend(s->options)) return refused;
     if (server_options) s->server_options = *server_options;
     else sk_server_default_options(&s->server_options);
     const sk_server_options& so = s->server_options;

@@ -17,8 +17,6 @@ namespace sk {
 
 namespace {
 
-thread_local std::string g_last_error;
-
 #define SK_HIP(call, code)                                                              \
   do {                                                                                  \
     const hipError_t e_ = (call);                                                       \
@@ -30,6 +28,10 @@ thread_local std::string g_last_error;
 
 #define SK_LAUNCH(...)                                                                  \
   do {                                                                                  \
+    if (fault_launch_fails()) {                                                         \
+      set_last_error("kernel launch: injected failure (sk_fault_inject)");              \
+      return kStatusLaunchFailed;                                                       \
+    }                                                                                   \
     __VA_ARGS__;                                                                        \
     const hipError_t e_ = hipGetLastError();                                            \
     if (e_ != hipSuccess) {                                                             \
@@ -194,6 +196,10 @@ class EngineImpl final : public Engine {
       b.bytes = 0;
     }
     if (!b.ptr) {
+      if (fault_allocation_fails()) {
+        set_last_error("hipMalloc: injected failure (sk_fault_inject)");
+        return kStatusOutOfMemory;
+      }
       SK_HIP(hipMalloc(&b.ptr, need), kStatusOutOfMemory);
       b.bytes = need;
     }
@@ -446,6 +452,13 @@ int EngineImpl<T>::upload() {
     row0 += m;
     col0 += n;
     nz0 += static_cast<int32_t>(lp.nnz());
+  }
+  {
+    // an injected fault: a NaN cost in the first column of one instance (its first step is
+    // non-finite, the others run on)
+    const int64_t bad = fault_take_nan_instance();
+    if (bad >= 0 && bad < K_ && desc_[static_cast<size_t>(bad)].cols > 0)
+      cmin[static_cast<size_t>(desc_[static_cast<size_t>(bad)].col0)] = std::nan("");
   }
   for (size_t i = 0; i < M; ++i) a_ptr[i + 1] += a_ptr[i];
   {
@@ -1349,16 +1362,6 @@ int EngineImpl<T>::probe_scaling(int32_t index, double* row_scale, double* col_s
 
 }  // namespace
 
-// Per-element byte counts of the buffers upload() allocates (w = iterate bytes):
-//   nonzero: in both layouts, the CSR-order index, the FP64 original and FP64 scaled value,
-//            and the staging-order index, value (w) and 16-bit slot; about 1 B of tile tables
-//   row:     pointer, bounds, scaling, iterates (y, y', restart point: w each, 2 scaled
-//            bounds), sums and candidates, best, activity, power-iteration image
-//   column:  pointer, cost, bounds, scaling, 3 scaled vectors + FP64 bounds, iterates (x, x',
-//            A^T y twice, xbar, restart point), sums, candidates, best, reduced costs, power
-//            iteration vectors
-//   instance: state record, descriptor and minimum tile entries
-// plus a fixed allowance for the allocation granularity of some 60 buffers.
 int query_device_memory(int64_t* free_bytes, int64_t* total_bytes) {
   int count = 0;
   if (hipGetDeviceCount(&count) != hipSuccess || count == 0) {
@@ -1371,22 +1374,6 @@ int query_device_memory(int64_t* free_bytes, int64_t* total_bytes) {
   *total_bytes = static_cast<int64_t>(t);
   return kStatusSuccess;
 }
-
-int64_t estimate_device_bytes(int64_t rows, int64_t cols, int64_t nnz, int64_t instances,
-                              int32_t precision) {
-  const double w = precision == 1 ? 8.0 : 4.0;
-  const double per_nnz = 56.0 + 2.0 * w + 12.0 + 1.1;
-  // each includes the iterate-precision running sum and its FP64 total
-  const double per_row = 84.0 + 6.0 * w + 0.5;
-  const double per_col = 116.0 + 10.0 * w + 0.5;
-  const double bytes = per_nnz * static_cast<double>(nnz) + per_row * static_cast<double>(rows) +
-                       per_col * static_cast<double>(cols) + 1024.0 * static_cast<double>(instances) +
-                       262144.0;
-  return static_cast<int64_t>(std::ceil(bytes));
-}
-
-void set_last_error(const std::string& message) { g_last_error = message; }
-const char* last_error() { return g_last_error.c_str(); }
 
 std::unique_ptr<Engine> make_single_engine(const EngineOptions& options) {
   if (options.precision == 1) return std::make_unique<EngineImpl<double>>(options);

@@ -136,6 +136,13 @@ int64_t estimate_device_bytes(int64_t rows, int64_t cols, int64_t nnz, int64_t i
 // Free and total memory of the current HIP device.
 int query_device_memory(int64_t* free_bytes, int64_t* total_bytes);
 
+// Injected faults (sk_fault_inject), process-wide, each firing once: the engine's next device
+// allocation or kernel launch that the countdown selects fails, and the next upload puts a NaN
+// into the cost of one instance's first column.
+bool fault_allocation_fails();
+bool fault_launch_fails();
+int64_t fault_take_nan_instance();
+
 // Thread-local description of the last failure inside the library.
 void set_last_error(const std::string& message);
 const char* last_error();

@@ -17,6 +17,10 @@
 namespace sk {
 namespace {
 
+// products are rounded before they are added, as in the reference's CPU operators
+// (gpu_ops_cpu.cpp:13-124): no FMA contraction in these kernels
+#pragma clang fp contract(off)
+
 int fail(hipError_t e, const char* what, int code) {
   set_last_error(std::string(what) + ": " + hipGetErrorString(e));
   return code;
@@ -153,23 +157,6 @@ using namespace sk;
 
 extern "C" {
 
-const char* sk_gpu_status_string(int32_t status) {
-  switch (status) {
-    case SK_GPU_SUCCESS: return "success";
-    case SK_GPU_UNAVAILABLE: return "hip unavailable";
-    case SK_GPU_NO_DEVICE: return "no compatible hip device";
-    case SK_GPU_INIT_FAILED: return "initialization failed";
-    case SK_GPU_OUT_OF_MEMORY: return "out of device memory";
-    case SK_GPU_LAUNCH_FAILED: return "kernel launch failed";
-    case SK_GPU_COPY_FAILED: return "memory copy failed";
-    case SK_GPU_SYNC_FAILED: return "device synchronization failed";
-    case SK_GPU_INVALID_STATE: return "invalid device state";
-    case SK_GPU_NON_FINITE: return "non-finite numerical output detected";
-    case SK_GPU_INVALID_ARGUMENT: return "invalid argument";
-    default: return "unknown status";
-  }
-}
-
 int32_t sk_gpu_device_count(void) {
   int count = 0;
   if (hipGetDeviceCount(&count) != hipSuccess) return 0;
@@ -221,8 +208,6 @@ int32_t sk_gpu_synchronize(void) {
   const hipError_t e = hipDeviceSynchronize();
   return e == hipSuccess ? kStatusSuccess : fail(e, "hipDeviceSynchronize", kStatusSyncFailed);
 }
-
-const char* sk_gpu_last_error(void) { return last_error(); }
 
 int32_t sk_gpu_malloc(void** dev_ptr, size_t bytes) {
   if (!dev_ptr) return kStatusInvalidArgument;
@@ -319,6 +304,61 @@ int32_t sk_gpu_pdhg_primal_step(int32_t n, double tau, const double* d_x, const 
   return launch_status("k_primal_step");
 }
 
+int32_t sk_gpu_execute_pdhg_primal_step(int32_t n, double tau, const double* x,
+                                        const double* cost, const double* at_y,
+                                        const double* col_lower, const double* col_upper,
+                                        double* x_next, double* extrapolated) {
+  // execute_pdhg_primal_step (gpu_ops_cpu.cpp:126-207) without its CPU fallback: the same
+  // sequence of failure classes, each returned as soon as it happens; the caller that wants
+  // the fallback runs its own CPU step on any status other than success.
+  if (n <= 0) return kStatusSuccess;
+  if (!x || !cost || !at_y || !col_lower || !col_upper || !x_next || !extrapolated) {
+    set_last_error("sk_gpu_execute_pdhg_primal_step: null host array");
+    return kStatusInvalidArgument;
+  }
+  if (sk_gpu_device_count() <= 0) {
+    set_last_error("sk_gpu_execute_pdhg_primal_step: no HIP device is available");
+    return kStatusUnavailable;
+  }
+  const size_t bytes = static_cast<size_t>(n) * sizeof(double);
+  struct Buffers {
+    double* p[7] = {};
+    ~Buffers() {
+      for (double* q : p)
+        if (q) (void)hipFree(q);
+    }
+  } d;
+  for (double*& q : d.p) {
+    const hipError_t e = hipMalloc(&q, bytes);
+    if (e != hipSuccess) {
+      q = nullptr;
+      return fail(e, "hipMalloc", kStatusOutOfMemory);
+    }
+  }
+  const double* host_in[5] = {x, cost, at_y, col_lower, col_upper};
+  for (int k = 0; k < 5; ++k) {
+    const hipError_t e = hipMemcpy(d.p[k], host_in[k], bytes, hipMemcpyHostToDevice);
+    if (e != hipSuccess) return fail(e, "hipMemcpy H2D", kStatusCopyFailed);
+  }
+  k_primal_step<<<grid_for(n, kBlock), kBlock>>>(n, tau, d.p[0], d.p[1], d.p[2], d.p[3], d.p[4],
+                                                 d.p[5], d.p[6]);
+  int st = launch_status("k_primal_step");
+  if (st) return st;
+  hipError_t e = hipDeviceSynchronize();
+  if (e != hipSuccess) return fail(e, "hipDeviceSynchronize", kStatusSyncFailed);
+  e = hipMemcpy(x_next, d.p[5], bytes, hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(extrapolated, d.p[6], bytes, hipMemcpyDeviceToHost);
+  if (e != hipSuccess) return fail(e, "hipMemcpy D2H", kStatusCopyFailed);
+  // the reference's numerical sanity check of the GPU output (gpu_ops_cpu.cpp:187-193)
+  for (int32_t i = 0; i < n; ++i) {
+    if (!std::isfinite(x_next[i]) || !std::isfinite(extrapolated[i])) {
+      set_last_error("sk_gpu_execute_pdhg_primal_step: non-finite output at column " + std::to_string(i));
+      return kStatusNonFinite;
+    }
+  }
+  return kStatusSuccess;
+}
+
 int32_t sk_gpu_spmv_csc(int32_t num_rows, int32_t num_cols, const int32_t* d_col_starts,
                         const int32_t* d_row_indices, const double* d_values, const double* d_x,
                         double* d_y, double alpha, int32_t transpose) {
@@ -344,34 +384,6 @@ int32_t sk_gpu_spmv_csr(int32_t num_rows, int32_t num_cols, const int32_t* d_row
   k_wave_rows<<<grid_for(num_rows, kBlock / kWave), kBlock>>>(num_rows, d_row_starts, d_col_indices,
                                                               d_values, d_x, d_y, alpha);
   return launch_status("k_wave_rows");
-}
-
-int32_t sk_gpu_compare_vectors(int32_t n, const double* t, const double* r, double tol,
-                               sk_gpu_comparison* out) {
-  if (!out) return kStatusInvalidArgument;
-  std::memset(out, 0, sizeof(*out));
-  out->passed = 1;
-  if (n <= 0) return kStatusSuccess;
-  if (!t || !r) {
-    out->passed = 0;
-    return kStatusSuccess;
-  }
-  double max_abs = 0.0, max_rel = 0.0;
-  for (int32_t i = 0; i < n; ++i) {
-    if (!std::isfinite(t[i]) || !std::isfinite(r[i])) {
-      out->contains_nan_or_inf = 1;
-      out->passed = 0;
-      return kStatusSuccess;
-    }
-    const double abs_diff = std::fabs(t[i] - r[i]);
-    max_abs = std::max(max_abs, abs_diff);
-    const double rel = abs_diff / std::max(1.0, std::max(std::fabs(t[i]), std::fabs(r[i])));
-    max_rel = std::max(max_rel, rel);
-    if (abs_diff > tol && rel > tol) out->passed = 0;
-  }
-  out->max_abs_diff = max_abs;
-  out->max_rel_diff = max_rel;
-  return kStatusSuccess;
 }
 
 }  // extern "C"

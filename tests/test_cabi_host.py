@@ -51,7 +51,7 @@ def test_option_registry_semantics():
     assert o.tolerance == 1e-6 and o.iteration_limit == 50 and o.precision == 1 and o.restart == 0
     for key, value in [("no_such_option", "1"), ("pdhg_tolerance", "abc"),
                        ("pdhg_tolerance", "0.5"), ("algorithm", "simplex"),
-                       ("backend", "cpu"), ("pdhg_polish", "maybe"), ("iteration_limit", "-2"),
+                       ("backend", "quantum"), ("pdhg_polish", "maybe"), ("iteration_limit", "-2"),
                        ("pdhg_precision", "fp16")]:
         with pytest.raises(sk.SkError) as e:
             sk.make_options(**{key: value})
@@ -190,3 +190,25 @@ def test_ipm_options_and_argument_checks():
         st = sk.lib().sk_ipm_solve(lp._h, C.byref(bad), None, None, None, C.byref(res),
                                    None, None, None, None)
         assert st == 10, field  # rejected before any device work
+
+
+def test_fault_injection_arguments_are_checked():
+    lib = sk.lib()
+    assert lib.sk_fault_inject(1, 3) == 0 and lib.sk_fault_inject(2, 1) == 0
+    assert lib.sk_fault_inject(3, 0) == 0 and lib.sk_fault_inject(0, 0) == 0
+    for kind, n in ((1, 0), (2, -1), (3, -2), (4, 1), (-1, 0)):
+        assert lib.sk_fault_inject(kind, n) == 10
+
+
+def test_validated_primal_step_without_a_device_reports_unavailable():
+    # no device in this container: the dispatcher returns the reference's kCudaUnavailable
+    # class (SK_GPU_UNAVAILABLE) and the caller runs its CPU step (gpu_ops_cpu.cpp:131-134)
+    n = 4
+    a = [np.ones(n) for _ in range(7)]
+    ptrs = [C.c_void_p(v.ctypes.data) for v in a]
+    lib = sk.lib()
+    assert lib.sk_gpu_execute_pdhg_primal_step(0, 1.0, *ptrs[:7]) == 0  # empty: success
+    if lib.sk_gpu_is_available():
+        pytest.skip("a device is present")
+    assert lib.sk_gpu_execute_pdhg_primal_step(n, 1.0, *ptrs[:7]) == 1
+    assert lib.sk_gpu_execute_pdhg_primal_step(n, 1.0, None, *ptrs[1:7]) == 10

@@ -91,3 +91,31 @@ def test_forwarded_option_texts_parse_back_exactly(key, value, field, expected):
     lib.sk_pdhg_default_options(C.byref(o))
     assert lib.sk_pdhg_set_option(C.byref(o), key.encode(), text.encode()) == 0
     assert getattr(o, field) == expected
+
+
+HAS_OPTION = ("\n  /// True when the registry holds this option (the query adaptive.cpp makes).\n"
+              "  [[nodiscard]] bool has_option(const std::string& name) const { return exists(name); }\n")
+
+
+def test_the_has_option_declaration_closes_the_selector_gap(tmp_path):
+    """The reference's selector calls Options::has_option, which options.hpp never declares:
+    it does not compile as shipped, and compiles with the one declaration INTEGRATION.md
+    gives (added to a scratch copy of the header, outside the repository)."""
+    if not REFERENCE_INCLUDE.exists():
+        pytest.skip("reference headers absent")
+    fmt = _fmt_include()
+    if fmt is None or shutil.which("g++") is None:
+        pytest.skip("g++ or the fmt headers are absent")
+    assert HAS_OPTION.strip() in (ROOT / "INTEGRATION.md").read_text()
+    header = (REFERENCE_INCLUDE / "sankhya" / "options.hpp").read_text()
+    anchor = "  [[nodiscard]] static bool exists(const std::string& name);\n"
+    assert anchor in header
+    (tmp_path / "sankhya").mkdir()
+    (tmp_path / "sankhya" / "options.hpp").write_text(header.replace(anchor, anchor + HAS_OPTION, 1))
+    src = REFERENCE_INCLUDE.parent / "src" / "core" / "adaptive.cpp"
+    base = ["g++", "-std=c++20", "-DFMT_HEADER_ONLY", f"-I{fmt}", "-fsyntax-only", str(src)]
+    plain = subprocess.run(base + [f"-I{REFERENCE_INCLUDE}"], capture_output=True, text=True)
+    assert plain.returncode != 0 and "has_option" in plain.stderr
+    fixed = subprocess.run(base + [f"-I{tmp_path}", f"-I{REFERENCE_INCLUDE}"], capture_output=True,
+                           text=True)
+    assert fixed.returncode == 0, fixed.stderr
