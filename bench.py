@@ -322,6 +322,21 @@ def precision_comparison(sk, lp, time_limit: float, use_graph: int) -> dict:
     return out
 
 
+def fp64_finish(sk, lp, first, tolerance: float, use_graph: int) -> dict:
+    """The finish of an LP above the interior point's factor cap (sk_pdhg_refine): the FP64
+    engine warm-started from the FP32 answer, to `tolerance` relative KKT. Its accuracy
+    against a known optimum is tests/test_gpu_finish.py's; here its cost."""
+    t0 = time.perf_counter()
+    r = sk.refine(lp, first, pdhg_tolerance=tolerance, pdhg_stop_at_request=True,
+                  pdhg_absolute_primal=False, use_graph=use_graph)
+    return {"tolerance": tolerance, "converged": bool(r.converged), "status": r.status_name,
+            "fp64_iterations": r.polish_iterations, "seconds": time.perf_counter() - t0,
+            "rel_primal": r.rel_primal, "rel_dual": r.rel_dual, "rel_gap": r.rel_gap,
+            "objective_change": abs(r.objective - first.objective) / max(1.0, abs(r.objective)),
+            "method": "FP64 PDHG warm-started from the FP32 answer (the dense interior point "
+                      "declines an LP of this size)"}
+
+
 def polish_sample(sk, lps, results, count: int) -> dict:
     """The interior-point finish (sk_pdhg_polish) of the first `count` batch answers: the
     step from the 1e-4 first-order answer to an optimal one within 1e-6 relative objective."""
@@ -500,6 +515,8 @@ def main() -> int:
         line["speedup_vs_fp64_single"] = line["value"] / f["lps_per_s"]
     if args.workload == "c2" and not args.no_compare:
         line["precision_comparison"] = precision_comparison(sk, lps[0], 120.0, layout["use_graph"])
+    if args.workload == "c5" and not args.no_compare:
+        line["finish"] = fp64_finish(sk, lps[0], results[0], 1e-7, layout["use_graph"])
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(lps, args.cpu_budget)
     if not args.no_serve and args.workload == "c3":

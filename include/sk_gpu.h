@@ -373,6 +373,9 @@ typedef struct {
   char algorithm[16]; /* "pdhg-gpu" (verify.cpp:86-89 audits on "gpu") */
   char message[256];
   int64_t polish_iterations; /* interior-point iterations of a polish (solve.cpp:225-242) */
+  double primal_weight;      /* the primal weight omega and step size eta the run ended with,
+                                to continue it (sk_batch_warm_start, sk_pdhg_refine) */
+  double step_size;
 } sk_pdhg_result;
 
 /* The drop-in for pdhg::solve_pdhg (include/sankhya/pdhg.hpp:15; src/pdhg/pdhg.cpp:267-726)
@@ -382,6 +385,19 @@ typedef struct {
 int32_t sk_pdhg_solve(const sk_lp* lp, const sk_pdhg_options* options, sk_pdhg_result* result,
                       double* col_value, double* row_dual, double* col_dual,
                       double* row_activity);
+
+/* The finish of an LP the interior point declines (its factor above the cap): the plain FP64
+ * engine, warm-started from a first answer (for example the FP32 engine's at 1e-4), run to the
+ * options' tolerance and stop rule (pdhg_precision is taken as fp64). `first` gives the primal
+ * weight and step size to continue with; first_col_value and first_row_dual are its arrays in
+ * the Solution's signs. The result's iterations are both phases together, polish_iterations
+ * the second, and algorithm reads "pdhg-gpu+fp64". No counterpart in the reference, whose
+ * PDHG always starts cold (pdhg.cpp:327-334) and whose polish declines such an LP
+ * (solve.cpp:184-243). */
+int32_t sk_pdhg_refine(const sk_lp* lp, const sk_pdhg_options* options,
+                       const sk_pdhg_result* first, const double* first_col_value,
+                       const double* first_row_dual, sk_pdhg_result* result, double* col_value,
+                       double* row_dual, double* col_dual, double* row_activity);
 
 /* writer.cpp:119-227 write_solution: the .sol file the reference's independent checker
  * (tools/verify_solution.py) reads. Header lines, then `columns` (name value reduced_cost
@@ -458,6 +474,12 @@ int32_t sk_batch_create(const sk_pdhg_options* options, sk_batch** out);
 /* The LP is borrowed: it must outlive sk_batch_upload. Returns the instance index. */
 int32_t sk_batch_add(sk_batch* batch, const sk_lp* lp, int32_t* index);
 /* Host CSR build, tiles and device allocation + upload (inputs resident afterwards). */
+/* Start instance `index` from col_value (num_cols) and row_dual (num_rows, the Solution's
+ * signs) instead of x = proj(0), y = 0 (pdhg.cpp:327-334); a NaN entry keeps that element's
+ * cold start; primal_weight and step_size > 0 replace omega = 1 and eta = 1 / ||A-hat||. Call
+ * before sk_batch_upload. No counterpart in the reference, whose PDHG always starts cold. */
+int32_t sk_batch_warm_start(sk_batch* batch, int32_t index, const double* col_value,
+                            const double* row_dual, double primal_weight, double step_size);
 int32_t sk_batch_upload(sk_batch* batch);
 /* One complete solve of every instance from the uploaded data, on the device. May be called
  * again; each call starts from scratch. */

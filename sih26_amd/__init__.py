@@ -5,10 +5,10 @@ Importing this package loads sih26_amd/lib/libsk_gpu.so and fails loudly if it i
 """
 from .api import (Batch, Lp, Server, SkError, SolveResult, adaptive_config, analyze_problem,
                   device_info, gpu_available, ipm_options, lib, make_options, polish,
-                  select_backend, solve, solve_ipm, write_solution)
+                  refine, select_backend, solve, solve_ipm, write_solution)
 
 lib()  # load now: a missing library is an import error, never a silent fallback
 
 __all__ = ["Batch", "Lp", "Server", "SkError", "SolveResult", "adaptive_config",
            "analyze_problem", "device_info", "gpu_available", "ipm_options", "lib",
-           "make_options", "polish", "select_backend", "solve", "solve_ipm", "write_solution"]
+           "make_options", "polish", "refine", "select_backend", "solve", "solve_ipm", "write_solution"]

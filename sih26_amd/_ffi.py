@@ -78,7 +78,8 @@ class PdhgResult(C.Structure):
                 ("dual_infeasibility", f64), ("dual_infeasibility_scaled", f64),
                 ("complementarity_violation", f64), ("absolute_gap", f64),
                 ("relative_gap", f64), ("algorithm", C.c_char * 16),
-                ("message", C.c_char * 256), ("polish_iterations", i64)]
+                ("message", C.c_char * 256), ("polish_iterations", i64),
+                ("primal_weight", f64), ("step_size", f64)]
 
 
 class IpmOptions(C.Structure):
@@ -161,7 +162,10 @@ SIGNATURES = {
                              vp, vp]),
     "sk_batch_create": (i32, [P(PdhgOptions), P(vp)]),
     "sk_batch_add": (i32, [vp, vp, P(i32)]),
+    "sk_batch_warm_start": (i32, [vp, i32, vp, vp, f64, f64]),
     "sk_batch_upload": (i32, [vp]),
+    "sk_pdhg_refine": (i32, [vp, P(PdhgOptions), P(PdhgResult), vp, vp, P(PdhgResult), vp, vp,
+                             vp, vp]),
     "sk_batch_solve": (i32, [vp]),
     "sk_batch_result": (i32, [vp, i32, P(PdhgResult), vp, vp, vp, vp]),
     "sk_batch_stats_get": (i32, [vp, P(BatchStats)]),

@@ -320,6 +320,21 @@ def polish(lp: Lp, first: SolveResult, **kw) -> SolveResult:
     return _result(res, x, yd, d, act)
 
 
+def refine(lp: Lp, first: SolveResult, **kw) -> SolveResult:
+    """sk_pdhg_refine: the FP64 engine warm-started from `first` (the finish of an LP above the
+    interior point's factor cap)."""
+    m, n, _ = lp.dims
+    o = make_options(**kw)
+    x, d, yd, act = np.zeros(n), np.zeros(n), np.zeros(m), np.zeros(m)
+    res = _ffi.PdhgResult()
+    check(lib().sk_pdhg_refine(lp._h, C.byref(o), C.byref(first.extra["raw"]),
+                               _ptr(np.ascontiguousarray(first.col_value, np.float64)),
+                               _ptr(np.ascontiguousarray(first.row_dual, np.float64)),
+                               C.byref(res), _ptr(x), _ptr(yd), _ptr(d), _ptr(act)),
+          "sk_pdhg_refine")
+    return _result(res, x, yd, d, act)
+
+
 class Batch:
     """Many LPs solved together in one block-diagonal device solve."""
 
@@ -335,6 +350,18 @@ class Batch:
         check(lib().sk_batch_add(self._h, lp._h, C.byref(k)), "sk_batch_add")
         self._lps.append(lp)
         return k.value
+
+    def warm_start(self, k: int, col_value, row_dual, primal_weight: float = 0.0,
+                   step_size: float = 0.0) -> None:
+        """Instance k starts from (col_value, row_dual) (Solution signs; NaN keeps an entry's
+        cold start); call before upload()."""
+        m, n, _ = self._lps[k].dims
+        x = np.ascontiguousarray(col_value, np.float64)
+        y = np.ascontiguousarray(row_dual, np.float64)
+        if x.shape != (n,) or y.shape != (m,):
+            raise ValueError("warm_start: lengths do not match the LP")
+        check(lib().sk_batch_warm_start(self._h, k, _ptr(x), _ptr(y), primal_weight, step_size),
+              "sk_batch_warm_start")
 
     def upload(self) -> None:
         check(lib().sk_batch_upload(self._h), "sk_batch_upload")

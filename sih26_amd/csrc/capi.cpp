@@ -31,6 +31,7 @@ struct sk_batch {
   sk_pdhg_options options;
   std::unique_ptr<sk::Engine> engine;
   std::vector<const sk_lp*> lps;
+  std::vector<sk::WarmStart> warm;  // per instance (engine sign), replayed into every engine
   int32_t tile_nnz = 0;  // the row-block capacity the uploaded batch runs with
   bool uploaded = false;
   bool solved = false;
@@ -199,6 +200,8 @@ int32_t fill_result(const sk::HostLp& lp, const sk::InstanceResult& r, const sk_
     res->absolute_gap = q.absolute_gap;
     res->relative_gap = q.relative_gap;
   }
+  res->primal_weight = r.omega;
+  res->step_size = r.eta;
   std::snprintf(res->algorithm, sizeof(res->algorithm), "pdhg-gpu");
   fill_message(r, o, res->message, sizeof(res->message));
   return SK_GPU_SUCCESS;
@@ -532,6 +535,68 @@ int32_t sk_pdhg_solve(const sk_lp* lp, const sk_pdhg_options* options, sk_pdhg_r
   });
 }
 
+int32_t sk_pdhg_refine(const sk_lp* lp, const sk_pdhg_options* options, const sk_pdhg_result* first,
+                       const double* first_col_value, const double* first_row_dual,
+                       sk_pdhg_result* result, double* col_value, double* row_dual,
+                       double* col_dual, double* row_activity) {
+  return guarded([&]() -> int32_t {
+    if (!lp || !first || !first_col_value || !first_row_dual) return SK_GPU_INVALID_ARGUMENT;
+    sk_pdhg_options o;
+    if (options) o = *options;
+    else sk_pdhg_default_options(&o);
+    if (const int32_t refused = refuse_backend(o)) return refused;
+    o.precision = 1;
+    const sk::HostLp& m = lp->lp;
+    // the engine's dual sign: row_dual = -sense y (pdhg.cpp:657-661)
+    std::vector<double> y(static_cast<size_t>(m.num_rows));
+    for (int32_t i = 0; i < m.num_rows; ++i) y[static_cast<size_t>(i)] = -static_cast<double>(m.sense) * first_row_dual[i];
+    const auto t0 = std::chrono::steady_clock::now();
+    auto engine = sk::make_engine(to_engine(o));
+    engine->add(&m);
+    int32_t st = engine->warm_start(0, first_col_value, y.data(), first->primal_weight, first->step_size);
+    if (!st) st = engine->upload();
+    if (!st) st = engine->solve();
+    sk::InstanceResult r;
+    if (!st) st = engine->result(0, &r);
+    if (st) return st;
+    const double seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    st = fill_result(m, r, o, seconds, result, col_value, row_dual, col_dual, row_activity);
+    if (st || !result) return st;
+    result->polish_iterations = result->iterations;
+    result->iterations += first->iterations;
+    result->solve_seconds += first->solve_seconds;
+    std::snprintf(result->algorithm, sizeof(result->algorithm), "pdhg-gpu+fp64");
+    return SK_GPU_SUCCESS;
+  });
+}
+
+int32_t sk_batch_warm_start(sk_batch* b, int32_t index, const double* col_value,
+                            const double* row_dual, double primal_weight, double step_size) {
+  if (!b || !col_value || !row_dual || index < 0 || index >= static_cast<int32_t>(b->lps.size())) {
+    sk::set_last_error("sk_batch_warm_start: no such instance, or a null start");
+    return SK_GPU_INVALID_ARGUMENT;
+  }
+  if (b->uploaded) {
+    sk::set_last_error("sk_batch_warm_start after sk_batch_upload");
+    return SK_GPU_INVALID_STATE;
+  }
+  return guarded([&]() -> int32_t {
+    const sk::HostLp& m = b->lps[static_cast<size_t>(index)]->lp;
+    std::vector<double> y(static_cast<size_t>(m.num_rows));
+    for (int32_t i = 0; i < m.num_rows; ++i) y[static_cast<size_t>(i)] = -static_cast<double>(m.sense) * row_dual[i];
+    const int32_t st = b->engine->warm_start(index, col_value, y.data(), primal_weight, step_size);
+    if (st) return st;
+    if (b->warm.size() < b->lps.size()) b->warm.resize(b->lps.size());
+    sk::WarmStart& w = b->warm[static_cast<size_t>(index)];
+    w.set = true;
+    w.x.assign(col_value, col_value + m.num_cols);
+    w.y = std::move(y);
+    w.omega = primal_weight;
+    w.eta = step_size;
+    return SK_GPU_SUCCESS;
+  });
+}
+
 int32_t sk_batch_create(const sk_pdhg_options* options, sk_batch** out) {
   return guarded([&]() -> int32_t {
     if (!out) return SK_GPU_INVALID_ARGUMENT;
@@ -584,7 +649,12 @@ int32_t autotune_upload(sk_batch* b) {
     o.tile_nnz = cap;
     auto e = sk::make_engine(to_engine(o));
     for (const sk_lp* lp : b->lps) e->add(&lp->lp);
-    int32_t st = e->upload();
+    int32_t st = SK_GPU_SUCCESS;
+    for (size_t k = 0; k < b->warm.size() && !st; ++k) {
+      const sk::WarmStart& w = b->warm[k];
+      if (w.set) st = e->warm_start(static_cast<int32_t>(k), w.x.data(), w.y.data(), w.omega, w.eta);
+    }
+    if (!st) st = e->upload();
     double t[kTimings];
     for (int k = 0; k < kTimings && !st; ++k) {
       double dual_ms = 0.0, aty_ms = 0.0;

@@ -107,6 +107,8 @@ struct InstState {
   double eta_ceiling;
   double restart_kkt;
   double best_worst;
+  // a warm start's step size (0: the reference's 1 / ||A-hat||, pdhg.cpp:368)
+  double eta_start;
   // termination options and problem norms (pdhg.cpp:285-315)
   double tolerance;
   double cost_norm;

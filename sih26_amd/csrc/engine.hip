@@ -159,9 +159,27 @@ class EngineImpl final : public Engine {
 
   int32_t add(const HostLp* lp) override {
     lps_.push_back(lp);
+    warm_.emplace_back();
     return static_cast<int32_t>(lps_.size()) - 1;
   }
-  void clear() override { lps_.clear(); }
+  int warm_start(int32_t index, const double* x, const double* y, double omega, double eta) override {
+    if (index < 0 || static_cast<size_t>(index) >= lps_.size() || !x || !y) {
+      set_last_error("warm_start(): no such instance, or a null start");
+      return kStatusInvalidArgument;
+    }
+    const HostLp& lp = *lps_[static_cast<size_t>(index)];
+    WarmStart& w = warm_[static_cast<size_t>(index)];
+    w.set = true;
+    w.x.assign(x, x + lp.num_cols);
+    w.y.assign(y, y + lp.num_rows);
+    w.omega = omega;
+    w.eta = eta;
+    return kStatusSuccess;
+  }
+  void clear() override {
+    lps_.clear();
+    warm_.clear();
+  }
   int32_t size() const override { return static_cast<int32_t>(lps_.size()); }
   const EngineTimings& timings() const override { return timings_; }
   int64_t total_nnz() const override { return nnz_; }
@@ -254,6 +272,8 @@ class EngineImpl final : public Engine {
 
   EngineOptions opt_;
   std::vector<const HostLp*> lps_;
+  std::vector<WarmStart> warm_;  // per instance, applied at upload
+  bool warm_any_ = false;        // the uploaded batch has a warm start
   int64_t m_ = 0, n_ = 0, nnz_ = 0;
   int32_t K_ = 0;
   int stage_ = 1;
@@ -267,6 +287,7 @@ class EngineImpl final : public Engine {
   std::vector<InstState> init_state_;
   std::vector<int32_t> sense_;
   std::vector<double> v0_;  // power-iteration start, all instances
+  std::vector<double> wx_host_, wy_host_;  // warm start in the device order (copy sources)
   std::vector<uint8_t> trivial_norm_;
   int32_t n_ta_ = 0, n_tt_ = 0, n_tc_ = 0, n_tr_ = 0;
   uint32_t initially_done_ = 0;
@@ -288,6 +309,7 @@ class EngineImpl final : public Engine {
   DevBuf b_ps_c_, b_ps_a_, b_ps_t_, b_pe_c_, b_pe_r_, b_pe_a_, b_pe_t_;
   DevBuf b_done_, b_v_, b_v0_, b_av_, b_len_, b_zero_, b_fa_, b_ft_, b_fc_, b_fr_;
   DevBuf b_a_orig_st_, b_t_orig_st_;
+  DevBuf b_wx_, b_wy_;  // warm start, device order, NaN where cold
   std::vector<DevBuf*> buffers() {
     return {&b_a_ptr_, &b_a_idx_, &b_a_orig_, &b_t_ptr_, &b_t_idx_, &b_t_orig_, &b_sa_, &b_st_,
             &b_a_val_, &b_t_val_, &b_a_sidx_, &b_t_sidx_, &b_a_pos_, &b_t_pos_, &b_cmin_, &b_cl_, &b_cu_, &b_rl_, &b_ru_, &b_dc_, &b_dr_,
@@ -297,7 +319,7 @@ class EngineImpl final : public Engine {
             &b_yu1_, &b_bestx_, &b_besty_, &b_act_, &b_red_, &b_tc_, &b_tr_, &b_ta_, &b_tt_,
             &b_desc_, &b_state_, &b_ps_c_, &b_ps_a_, &b_ps_t_, &b_pe_c_, &b_pe_r_, &b_pe_a_,
             &b_pe_t_, &b_done_, &b_v_, &b_v0_, &b_av_, &b_len_, &b_zero_, &b_fa_, &b_ft_,
-            &b_fc_, &b_fr_, &b_a_orig_st_, &b_t_orig_st_};
+            &b_fc_, &b_fr_, &b_a_orig_st_, &b_t_orig_st_, &b_wx_, &b_wy_};
   }
   std::vector<const DevBuf*> buffers() const {
     auto all = const_cast<EngineImpl<T>*>(this)->buffers();
@@ -316,6 +338,8 @@ class EngineImpl final : public Engine {
   double* d_av_ = nullptr;
   double* d_len_ = nullptr;
   int32_t* d_zero_seen_ = nullptr;
+  double* d_wx_ = nullptr;
+  double* d_wy_ = nullptr;
 
   // serving: blocks in flight (a ring of 2: event, finished count and state snapshot per
   // block), instances already read back, and a second stream for those read-backs
@@ -436,6 +460,11 @@ int EngineImpl<T>::upload() {
     s.cost_norm = std::sqrt(cost_square);
     s.bound_norm = std::sqrt(bound_square);
     s.omega = 1.0;
+    if (k < static_cast<int32_t>(warm_.size()) && warm_[static_cast<size_t>(k)].set) {
+      const WarmStart& w = warm_[static_cast<size_t>(k)];
+      if (w.omega > 0.0) s.omega = std::min(1e6, std::max(1e-6, w.omega));  // pdhg.cpp:627-629
+      if (w.eta > 0.0) s.eta_start = w.eta;
+    }
     s.restart_kkt = INFINITY;
     s.best_worst = INFINITY;
     s.iteration_limit = limit;
@@ -671,6 +700,24 @@ int EngineImpl<T>::upload() {
   SK_TRY(alloc(b_pe_t_, static_cast<size_t>(n_tt_) * 8, &a.pe_t));
   SK_TRY(alloc(b_done_, 1, &a.done_count));
   SK_TRY(put(b_v0_, v0_, &d_v0_));
+  warm_any_ = false;
+  for (int32_t k = 0; k < K_ && k < static_cast<int32_t>(warm_.size()); ++k) warm_any_ = warm_any_ || warm_[static_cast<size_t>(k)].set;
+  if (warm_any_) {
+    // in the device order; NaN marks the cold elements
+    wx_host_.assign(N, std::nan(""));
+    wy_host_.assign(M, std::nan(""));
+    for (int32_t k = 0; k < K_; ++k) {
+      const WarmStart& w = warm_[static_cast<size_t>(k)];
+      if (!w.set) continue;
+      const InstDesc& d = desc_[static_cast<size_t>(k)];
+      const std::vector<int32_t>& perm = col_perm_[static_cast<size_t>(k)];
+      for (int32_t j = 0; j < d.cols; ++j)
+        wx_host_[static_cast<size_t>(d.col0 + j)] = w.x[static_cast<size_t>(perm.empty() ? j : perm[static_cast<size_t>(j)])];
+      for (int32_t i = 0; i < d.rows; ++i) wy_host_[static_cast<size_t>(d.row0 + i)] = w.y[static_cast<size_t>(i)];
+    }
+    SK_TRY(put(b_wx_, wx_host_, &d_wx_));
+    SK_TRY(put(b_wy_, wy_host_, &d_wy_));
+  }
   SK_TRY(alloc(b_v_, N, &d_v_));
   SK_TRY(alloc(b_av_, M, &d_av_));
   SK_TRY(alloc(b_len_, static_cast<size_t>(K_), &d_len_));
@@ -760,6 +807,19 @@ int EngineImpl<T>::enqueue_setup() {
   SK_LAUNCH(k_inst_norm<<<K_, kBlock, 0, s>>>(a.desc, a.ps_c, d_len_, nullptr));
   SK_LAUNCH(k_power_finish<<<blocks_for(K_), kBlock, 0, s>>>(d_len_, d_zero_seen_, a.state, K_));
   SK_LAUNCH(k_init_iterates<T><<<blocks_for(MN), kBlock, 0, s>>>(a, d_lhat64_, d_uhat64_, N, M));
+  if (warm_any_) {
+    // warm iterates, then the carried A^T y of the warm y with the step kernels' product
+    SK_LAUNCH(k_warm_iterates<T><<<blocks_for(MN), kBlock, 0, s>>>(a, d_wx_, d_wy_, d_lhat64_, d_uhat64_, N, M));
+    const int st = with_layout([&](auto cap, auto pack) -> int {
+      constexpr int C = decltype(cap)::value;
+      constexpr bool P = decltype(pack)::value;
+      SK_LAUNCH(k_probe_product<T, C, P><<<static_cast<unsigned>(n_tt_), kStepLanes, 0, s>>>(
+          a.tt, a.t_ptr, a.t_sidx, a.t_pos, a.t_val, a.y0, a.aty0));
+      return kStatusSuccess;
+    });
+    if (st) return st;
+    SK_LAUNCH(k_copy_t<T><<<blocks_for(N), kBlock, 0, s>>>(a.aty0, a.aty1, N));
+  }
   if (K_ > 0) SK_LAUNCH(k_publish_flags<T><<<K_, kBlock, 0, s>>>(a));
   return kStatusSuccess;
 }

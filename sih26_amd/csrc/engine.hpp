@@ -36,6 +36,13 @@ struct EngineOptions {
   int32_t step_fusion = 0;        // 0 automatic (one launch per step for small instances), 1 never
 };
 
+// A warm start held by an engine until its next upload (Engine::warm_start).
+struct WarmStart {
+  bool set = false;
+  std::vector<double> x, y;  // LP order; NaN keeps an element's cold start
+  double omega = 0.0, eta = 0.0;
+};
+
 struct InstanceResult {
   int32_t status = 0;
   int32_t converged = 0;
@@ -63,6 +70,12 @@ class Engine {
   virtual ~Engine() = default;
   // Instances are borrowed: they must stay alive until upload() returns.
   virtual int32_t add(const HostLp* lp) = 0;
+  // Start instance `index` (already added, not yet uploaded) from x (cols, LP order) and y
+  // (rows, the engine's sign: row_dual = -sense y) instead of proj(0) and 0; a NaN entry keeps
+  // that element's cold start. omega and eta > 0 replace the initial primal weight and step
+  // size. Takes effect at the next upload().
+  virtual int warm_start(int32_t index, const double* x, const double* y, double omega,
+                         double eta) = 0;
   virtual int upload() = 0;  // host CSR/tiles + device allocation + H2D (not part of a solve)
   virtual int solve() = 0;   // everything on the device, from the uploaded originals
   virtual int result(int32_t index, InstanceResult* out) const = 0;

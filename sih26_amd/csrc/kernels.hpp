@@ -1195,6 +1195,8 @@ __global__ __launch_bounds__(kBlock) void k_power_finish(const double* __restric
   s.spectral_norm = norm;
   s.eta = norm > 0.0 ? 1.0 / norm : 1.0;
   s.eta_ceiling = 1.0e3 / dev_max(norm, 1e-12);
+  // a warm start keeps the step size of the run it continues, within the rule's clamp
+  if (s.eta_start > 0.0) s.eta = dev_min(dev_max(s.eta_start, 1e-12), s.eta_ceiling);
 }
 
 __global__ __launch_bounds__(kBlock) void k_fill(double* __restrict__ dst, double value, int64_t n) {
@@ -1254,6 +1256,44 @@ __global__ __launch_bounds__(kBlock) void k_init_iterates(Args<T> a, const doubl
     if constexpr (!std::is_same<T, double>::value) a.ytot[k] = 0.0;
     a.besty[k] = 0.0;
   }
+}
+
+// A warm start (sk_batch_warm_start): the columns and rows that have a start value (not NaN)
+// begin there instead of at proj(0) and 0, in the scaled space of the device set-up:
+// x-hat = proj(x / D_c), y-hat = y / D_r (pdhg.cpp:357-366 maps the other way). The running
+// sums stay zero and the restart point and best point start at the warm point, as the cold
+// start does at its own (pdhg.cpp:327-379).
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_warm_iterates(Args<T> a, const double* __restrict__ wx,
+                                                          const double* __restrict__ wy,
+                                                          const double* __restrict__ lhat64,
+                                                          const double* __restrict__ uhat64,
+                                                          int32_t ncols, int32_t nrows) {
+  const int k = blockIdx.x * kBlock + threadIdx.x;
+  if (k < ncols && !isnan(wx[k])) {
+    double v = wx[k] * (1.0 / a.dc[k]);
+    if (v < lhat64[k]) v = lhat64[k];
+    if (v > uhat64[k]) v = uhat64[k];
+    const T x = dev_project<T>(static_cast<T>(v), a.lhat[k], a.uhat[k]);
+    a.x0[k] = x;
+    a.x1[k] = x;
+    a.xrst[k] = x;
+    a.xbar[k] = x;
+    a.bestx[k] = v;
+  }
+  if (k < nrows && !isnan(wy[k])) {
+    const double v = wy[k] * (1.0 / a.dr[k]);
+    a.y0[k] = static_cast<T>(v);
+    a.y1[k] = static_cast<T>(v);
+    a.yrst[k] = static_cast<T>(v);
+    a.besty[k] = v;
+  }
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_copy_t(const T* __restrict__ src, T* __restrict__ dst, int64_t n) {
+  const int64_t k = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+  if (k < n) dst[k] = src[k];
 }
 
 }  // namespace sk

@@ -33,7 +33,22 @@ class SplitEngine final : public Engine {
 
   int32_t add(const HostLp* lp) override {
     lps_.push_back(lp);
+    warm_.emplace_back();
     return static_cast<int32_t>(lps_.size()) - 1;
+  }
+  int warm_start(int32_t index, const double* x, const double* y, double omega, double eta) override {
+    if (index < 0 || static_cast<size_t>(index) >= lps_.size() || !x || !y) {
+      set_last_error("warm_start(): no such instance, or a null start");
+      return kStatusInvalidArgument;
+    }
+    const HostLp& lp = *lps_[static_cast<size_t>(index)];
+    WarmStart& w = warm_[static_cast<size_t>(index)];
+    w.set = true;
+    w.x.assign(x, x + lp.num_cols);
+    w.y.assign(y, y + lp.num_rows);
+    w.omega = omega;
+    w.eta = eta;
+    return kStatusSuccess;
   }
 
   int upload() override {
@@ -69,6 +84,12 @@ class SplitEngine final : public Engine {
       const int32_t p = owner[static_cast<size_t>(i)];
       where_[static_cast<size_t>(i)] = Place{p, parts_[static_cast<size_t>(p)]->add(lps_[static_cast<size_t>(i)])};
       back_[static_cast<size_t>(p)].push_back(i);
+      const WarmStart& w = warm_[static_cast<size_t>(i)];
+      if (w.set) {
+        const int st = parts_[static_cast<size_t>(p)]->warm_start(where_[static_cast<size_t>(i)].local,
+                                                                  w.x.data(), w.y.data(), w.omega, w.eta);
+        if (st) return st;
+      }
     }
     for (auto& e : parts_) {
       const int st = e->upload();
@@ -122,6 +143,7 @@ class SplitEngine final : public Engine {
 
   void clear() override {
     lps_.clear();
+    warm_.clear();
     where_.clear();
     back_.clear();
     for (auto& e : parts_) e->clear();
@@ -270,6 +292,7 @@ class SplitEngine final : public Engine {
 
   EngineOptions opt_;
   std::vector<const HostLp*> lps_;
+  std::vector<WarmStart> warm_;
   std::vector<Place> where_;                 // batch index -> sub-batch and its index there
   std::vector<std::vector<int32_t>> back_;   // sub-batch index -> batch index
   std::vector<std::unique_ptr<Engine>> parts_;

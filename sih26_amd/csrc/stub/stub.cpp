@@ -29,6 +29,7 @@ int unavailable() {
 class StubEngine final : public Engine {
  public:
   int32_t add(const HostLp*) override { return count_++; }
+  int warm_start(int32_t, const double*, const double*, double, double) override { return unavailable(); }
   int upload() override { return unavailable(); }
   int solve() override { return unavailable(); }
   int result(int32_t, InstanceResult*) const override { return unavailable(); }

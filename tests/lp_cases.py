@@ -75,3 +75,31 @@ def csc_to_dense(lp: dict) -> np.ndarray:
         for p in range(cs[j], cs[j + 1]):
             a[ri[p], j] = v[p]
     return a
+
+
+def kkt_backwards_from(lp: dict, seed: int, multiply) -> tuple[dict, float, np.ndarray, np.ndarray]:
+    """The LP `lp` (its matrix and column bounds; rows with equal bounds stay equalities, the
+    others become <= rows) with costs and right-hand sides chosen so that a drawn point is
+    optimal (the construction of tests/oracles/lp_generator.hpp:48-53 on a given matrix):
+    x* at its lower bound (30%), upper bound (20%) or inside; row multipliers free on
+    equalities, <= 0 on active <= rows, 0 on inactive ones (slack U[0.5, 5]); reduced costs
+    >= 0 at lower bounds, <= 0 at upper bounds, 0 inside; c = A^T y* + d*. Returns the LP, its
+    optimum c'x*, x* and y* in the Solution's signs (row_dual). `multiply(lp, v, transpose)`
+    is the FP64 oracle product."""
+    rng = np.random.default_rng(seed)
+    m, n = lp["num_rows"], lp["num_cols"]
+    lo, up = np.asarray(lp["col_lower"], float), np.asarray(lp["col_upper"], float)
+    assert np.all(np.isfinite(lo)) and np.all(np.isfinite(up)) and np.all(up > lo)
+    r = rng.random(n)
+    at_lo, at_up = r < 0.3, (r >= 0.3) & (r < 0.5)
+    x = np.where(at_lo, lo, np.where(at_up, up, lo + (up - lo) * rng.uniform(0.2, 0.8, n)))
+    act = multiply(lp, x, False)
+    eq = np.asarray(lp["row_lower"]) == np.asarray(lp["row_upper"])
+    active = ~eq & (rng.random(m) < 0.5)
+    y = np.where(eq, rng.standard_normal(m), np.where(active, -np.abs(rng.standard_normal(m)), 0.0))
+    row_lower = np.where(eq, act, -INF)
+    row_upper = np.where(eq | active, act, act + rng.uniform(0.5, 5.0, m))
+    d = np.where(at_lo, rng.uniform(0.1, 1.0, n), np.where(at_up, -rng.uniform(0.1, 1.0, n), 0.0))
+    c = multiply(lp, y, True) + d
+    out = dict(lp, cost=c, row_lower=row_lower, row_upper=row_upper, sense=1, offset=0.0)
+    return out, float(c @ x), x, y

@@ -318,3 +318,21 @@ def test_recompute_quality_wrapper_reproduces_the_solve_report(oracle_mod):
     for f in oracle_mod.QUALITY_FIELDS:
         assert q[f] == r[f], f
     np.testing.assert_array_equal(q["row_activity"], r["row_activity"])
+
+
+def test_kkt_backwards_construction_is_optimal_at_its_point(oracle_mod):
+    """The construction the finish test of the 2M x 5M LP relies on, on a small LP of the same
+    generator: the oracle measures relative KKT ~ 0 at the drawn point (x*, y*), and the
+    optimum is c'x*."""
+    import sih26_amd as sk
+    from tests.lp_cases import kkt_backwards_from
+    base = sk.Lp.generate_params(**dict(sk.Lp.preset(5, 5), num_rows=300, num_cols=800,
+                                        num_equality_rows=180, target_nnz=4000)).export()
+    mult = lambda lp, v, t: oracle_mod.multiply(lp, v, transpose=t)  # noqa: E731
+    d, opt, x, y = kkt_backwards_from(base, 7, mult)
+    ev = oracle_mod.evaluate(d, x, -y)  # the engine's sign: y_engine = -sense row_dual
+    assert ev["worst"] <= 1e-13, ev
+    assert ev["primal_objective"] == pytest.approx(opt, rel=1e-14)
+    # and the oracle's PDHG finds that optimum
+    r = oracle_mod.solve_pdhg(d, tolerance=1e-8, stop_at_request=True, absolute_primal=False)
+    assert r["converged"] and abs(r["objective"] - opt) <= 1e-6 * abs(opt)
