@@ -36,6 +36,7 @@ extern "C" {
 /* ---- solve statuses (model.hpp:62-80) ---------------------------------------------------- */
 #define SK_SOLVE_NOT_SOLVED 0
 #define SK_SOLVE_OPTIMAL 1
+#define SK_SOLVE_INFEASIBLE 2
 #define SK_SOLVE_FEASIBLE 5
 #define SK_SOLVE_ITERATION_LIMIT 6
 #define SK_SOLVE_TIME_LIMIT 7
@@ -263,6 +264,11 @@ typedef struct {
                                 server of this library refuses _CPU (the CPU engine is the
                                 reference's) and _HYBRID (unsupported, as in the reference) */
   int32_t gpu;               /* the legacy gpu flag (options.cpp:220-227), default 0 */
+  int32_t presolve;          /* presolve (options.cpp:190-201): default 0 here, since the
+                                reference presolves in solve(), around solve_pdhg
+                                (solve.cpp:423-443); 1 makes sk_pdhg_solve do the same:
+                                sk_presolve, the engine on the reduced LP, sk_postsolve, and
+                                the answer measured on the original LP */
 } sk_pdhg_options;
 
 /* The defaults listed above (options.cpp:83-106, 220-236, 284-290, 328-334, 395-418). */
@@ -447,6 +453,80 @@ int32_t sk_pdhg_polish(const sk_lp* lp, const sk_ipm_options* options,
                        const double* first_row_dual, const double* first_col_dual,
                        sk_pdhg_result* result, double* col_value, double* row_dual,
                        double* col_dual, double* row_activity);
+
+/* ---- presolve and postsolve (src/presolve/presolve.hpp:36-127; presolve.cpp:204-1697) -------
+ * The reductions the reference's solve() applies ahead of every LP engine and undoes after it
+ * (src/core/solve.cpp:415-443), on the host: the same eight record kinds, the same column and
+ * row passes repeated to a fixed point (at most 20), the same tests and tolerances, so the
+ * reduced LP is the one the reference's engine is handed; and postsolve of the primal point,
+ * the row duals and the reduced costs back to the original LP (presolve.cpp:812-1697). */
+#define SK_PRESOLVE_EMPTY_ROW 0
+#define SK_PRESOLVE_REDUNDANT_ROW 1
+#define SK_PRESOLVE_FIXED_COLUMN 2
+#define SK_PRESOLVE_EMPTY_COLUMN 3
+#define SK_PRESOLVE_SINGLETON_ROW 4
+#define SK_PRESOLVE_FORCING_ROW 5
+#define SK_PRESOLVE_FREE_COLUMN_SINGLETON 6
+#define SK_PRESOLVE_DOUBLETON_EQUATION 7
+typedef struct sk_presolved sk_presolved; /* a presolve's result and its records */
+
+/* presolve::Result's counts (presolve.hpp:94-117), plus the records of each kind and the time */
+typedef struct {
+  int32_t original_rows;
+  int32_t original_cols;
+  int64_t original_nonzeros;
+  int32_t reduced_rows;
+  int32_t reduced_cols;
+  int64_t reduced_nonzeros;
+  int32_t records;
+  int32_t passes;
+  int32_t proved_infeasible;
+  int32_t by_kind[8]; /* indexed by SK_PRESOLVE_* */
+  int32_t reserved;
+  double seconds;
+} sk_presolve_info;
+
+/* presolve::Record (presolve.hpp:66-91) */
+typedef struct {
+  int32_t kind; /* SK_PRESOLVE_* */
+  int32_t index;
+  double value;
+  double coefficient;
+  double row_lower;
+  double row_upper;
+  int32_t column;
+  int32_t partner_column;
+  double substituted_rhs;
+  double partner_coefficient;
+  double eliminated_cost;
+} sk_presolve_record;
+
+/* presolve::presolve (presolve.hpp:123): `lp` is left untouched. integer_columns holds one flag
+ * per column or is NULL (every column continuous; integer columns get inward-rounded implied
+ * bounds and are never eliminated by a doubleton); primal_feasibility_tolerance is the option of
+ * that name (options.cpp:293-299, default 1e-7; <= 0 takes the default). A proof of
+ * infeasibility is a success: sk_presolve_get_info says proved_infeasible and
+ * sk_presolve_message names the row or column. */
+int32_t sk_presolve(const sk_lp* lp, const uint8_t* integer_columns,
+                    double primal_feasibility_tolerance, sk_presolved** out);
+int32_t sk_presolve_get_info(const sk_presolved* presolve, sk_presolve_info* info);
+/* Result::message: empty unless infeasibility was proved. Owned by the handle. */
+const char* sk_presolve_message(const sk_presolved* presolve);
+/* The reduced LP (Result::model) as a new handle the caller frees with sk_lp_free; names are
+ * carried over when the original was read from a file. */
+int32_t sk_presolve_reduced_lp(const sk_presolved* presolve, sk_lp** out);
+/* Result::col_to_original and row_to_original (reduced sizes; either may be NULL). */
+int32_t sk_presolve_maps(const sk_presolved* presolve, int32_t* col_to_original,
+                         int32_t* row_to_original);
+int32_t sk_presolve_get_record(const sk_presolved* presolve, int32_t k, sk_presolve_record* out);
+/* presolve::postsolve (presolve.hpp:130-131): a point of the reduced LP (sizes of the reduced
+ * LP; Solution signs, col_dual = c - A^T row_dual) back to the original LP `original` (the one
+ * sk_presolve was given), into arrays of the original sizes. The caller measures the result on
+ * the original LP, as the reference's postsolve does with recompute_quality. */
+int32_t sk_postsolve(const sk_presolved* presolve, const sk_lp* original, const double* col_value,
+                     const double* row_dual, const double* col_dual, double* out_col_value,
+                     double* out_row_dual, double* out_col_dual);
+void sk_presolve_free(sk_presolved* presolve);
 
 /* ---- batches: many LPs of uneven size in one block-diagonal solve --------------------------
  * No counterpart in the reference, which solves one LP per call (src/core/solve.cpp:385-399);

@@ -3,12 +3,13 @@ solver, as hand-written HIP kernels for gfx950 behind a C ABI (include/sk_gpu.h)
 
 Importing this package loads sih26_amd/lib/libsk_gpu.so and fails loudly if it is not built.
 """
-from .api import (Batch, Lp, Server, SkError, SolveResult, adaptive_config, analyze_problem,
-                  device_info, gpu_available, ipm_options, lib, make_options, polish,
-                  refine, select_backend, solve, solve_ipm, write_solution)
+from .api import (Batch, Lp, Presolved, Server, SkError, SolveResult, adaptive_config,
+                  analyze_problem, device_info, gpu_available, ipm_options, lib, make_options,
+                  polish, presolve, refine, select_backend, solve, solve_ipm, write_solution)
 
 lib()  # load now: a missing library is an import error, never a silent fallback
 
-__all__ = ["Batch", "Lp", "Server", "SkError", "SolveResult", "adaptive_config",
+__all__ = ["Batch", "Lp", "Presolved", "Server", "SkError", "SolveResult", "adaptive_config",
            "analyze_problem", "device_info", "gpu_available", "ipm_options", "lib",
-           "make_options", "polish", "refine", "select_backend", "solve", "solve_ipm", "write_solution"]
+           "make_options", "polish", "presolve", "refine", "select_backend", "solve", "solve_ipm",
+           "write_solution"]

@@ -44,7 +44,8 @@ class PdhgOptions(C.Structure):
                 ("random_seed", i64), ("precision", i32), ("steps_per_block", i32),
                 ("use_graph", i32), ("spmv_stage", i32), ("tile_nnz", i32),
                 ("compute_quality", i32), ("eval_stride", i32), ("streams", i32),
-                ("column_order", i32), ("step_fusion", i32), ("backend", i32), ("gpu", i32)]
+                ("column_order", i32), ("step_fusion", i32), ("backend", i32), ("gpu", i32),
+                ("presolve", i32)]
 
 
 class ProblemCharacteristics(C.Structure):
@@ -112,6 +113,20 @@ class ServerStats(C.Structure):
                 ("memory_budget", i64), ("peak_held", i64), ("releases", i64)]
 
 
+class PresolveInfo(C.Structure):
+    _fields_ = [("original_rows", i32), ("original_cols", i32), ("original_nonzeros", i64),
+                ("reduced_rows", i32), ("reduced_cols", i32), ("reduced_nonzeros", i64),
+                ("records", i32), ("passes", i32), ("proved_infeasible", i32),
+                ("by_kind", i32 * 8), ("reserved", i32), ("seconds", f64)]
+
+
+class PresolveRecord(C.Structure):
+    _fields_ = [("kind", i32), ("index", i32), ("value", f64), ("coefficient", f64),
+                ("row_lower", f64), ("row_upper", f64), ("column", i32),
+                ("partner_column", i32), ("substituted_rhs", f64),
+                ("partner_coefficient", f64), ("eliminated_cost", f64)]
+
+
 # name -> (restype, argtypes)
 SIGNATURES = {
     "sk_gpu_status_string": (cstr, [i32]),
@@ -160,6 +175,14 @@ SIGNATURES = {
     "sk_ipm_solve": (i32, [vp, P(IpmOptions), vp, vp, vp, P(PdhgResult), vp, vp, vp, vp]),
     "sk_pdhg_polish": (i32, [vp, P(IpmOptions), P(PdhgResult), vp, vp, vp, P(PdhgResult), vp, vp,
                              vp, vp]),
+    "sk_presolve": (i32, [vp, vp, f64, P(vp)]),
+    "sk_presolve_get_info": (i32, [vp, P(PresolveInfo)]),
+    "sk_presolve_message": (cstr, [vp]),
+    "sk_presolve_reduced_lp": (i32, [vp, P(vp)]),
+    "sk_presolve_maps": (i32, [vp, vp, vp]),
+    "sk_presolve_get_record": (i32, [vp, i32, P(PresolveRecord)]),
+    "sk_postsolve": (i32, [vp, vp, vp, vp, vp, vp, vp, vp]),
+    "sk_presolve_free": (None, [vp]),
     "sk_batch_create": (i32, [P(PdhgOptions), P(vp)]),
     "sk_batch_add": (i32, [vp, vp, P(i32)]),
     "sk_batch_warm_start": (i32, [vp, i32, vp, vp, f64, f64]),

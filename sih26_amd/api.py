@@ -335,6 +335,73 @@ def refine(lp: Lp, first: SolveResult, **kw) -> SolveResult:
     return _result(res, x, yd, d, act)
 
 
+PRESOLVE_KINDS = ("empty_row", "redundant_row", "fixed_column", "empty_column", "singleton_row",
+                  "forcing_row", "free_column_singleton", "doubleton_equation")
+
+
+class Presolved:
+    """presolve::presolve on one LP (sk_presolve, host code): the reduced LP, the records, and
+    postsolve of a reduced answer back to the original LP."""
+
+    def __init__(self, lp: Lp, integer_columns=None, feasibility: float = 1e-7):
+        self._h = C.c_void_p()
+        mask = None if integer_columns is None else np.ascontiguousarray(integer_columns, np.uint8)
+        check(lib().sk_presolve(lp._h, _ptr(mask), feasibility, C.byref(self._h)), "sk_presolve")
+        self.original = lp
+        info = _ffi.PresolveInfo()
+        check(lib().sk_presolve_get_info(self._h, C.byref(info)), "sk_presolve_get_info")
+        self.info = {f: getattr(info, f) for f, _ in info._fields_ if f != "by_kind"}
+        self.info["by_kind"] = {k: info.by_kind[i] for i, k in enumerate(PRESOLVE_KINDS)}
+        self.message = lib().sk_presolve_message(self._h).decode()
+        self.proved_infeasible = bool(info.proved_infeasible)
+
+    @property
+    def reduced(self) -> Lp:
+        h = C.c_void_p()
+        check(lib().sk_presolve_reduced_lp(self._h, C.byref(h)), "sk_presolve_reduced_lp")
+        return Lp(h)
+
+    def maps(self) -> tuple[np.ndarray, np.ndarray]:
+        cols = np.empty(self.info["reduced_cols"], np.int32)
+        rows = np.empty(self.info["reduced_rows"], np.int32)
+        check(lib().sk_presolve_maps(self._h, _ptr(cols), _ptr(rows)), "sk_presolve_maps")
+        return cols, rows
+
+    def records(self) -> list[dict[str, Any]]:
+        out = []
+        for k in range(self.info["records"]):
+            r = _ffi.PresolveRecord()
+            check(lib().sk_presolve_get_record(self._h, k, C.byref(r)), "sk_presolve_get_record")
+            d = {f: getattr(r, f) for f, _ in r._fields_}
+            d["kind"] = PRESOLVE_KINDS[r.kind]
+            out.append(d)
+        return out
+
+    def postsolve(self, col_value, row_dual, col_dual) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
+        """A reduced answer (Solution signs) back to the original LP: (x, row_dual, col_dual)."""
+        m, n, _ = self.original.dims
+        x, y, d = np.zeros(n), np.zeros(m), np.zeros(n)
+        arrs = [np.ascontiguousarray(a, np.float64) for a in (col_value, row_dual, col_dual)]
+        check(lib().sk_postsolve(self._h, self.original._h, *[_ptr(a) for a in arrs], _ptr(x),
+                                 _ptr(y), _ptr(d)), "sk_postsolve")
+        return x, y, d
+
+    def close(self) -> None:
+        if self._h:
+            lib().sk_presolve_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def presolve(lp: Lp, integer_columns=None, feasibility: float = 1e-7) -> Presolved:
+    return Presolved(lp, integer_columns, feasibility)
+
+
 class Batch:
     """Many LPs solved together in one block-diagonal device solve."""
 

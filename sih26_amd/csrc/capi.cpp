@@ -10,6 +10,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <deque>
+#include <limits>
 #include <memory>
 #include <mutex>
 #include <new>
@@ -21,6 +22,7 @@
 #include "engine.hpp"
 #include "lp.hpp"
 #include "lpfile.hpp"
+#include "presolve.hpp"
 #include "quality.hpp"
 #include "server.hpp"
 #include "status.hpp"
@@ -35,6 +37,12 @@ struct sk_batch {
   int32_t tile_nnz = 0;  // the row-block capacity the uploaded batch runs with
   bool uploaded = false;
   bool solved = false;
+};
+
+struct sk_presolved {
+  sk::presolve::Result result;
+  double seconds = 0.0;
+  std::vector<std::string> col_names, row_names;  // of the reduced LP, when the original had names
 };
 
 namespace {
@@ -204,6 +212,71 @@ int32_t fill_result(const sk::HostLp& lp, const sk::InstanceResult& r, const sk_
   res->step_size = r.eta;
   std::snprintf(res->algorithm, sizeof(res->algorithm), "pdhg-gpu");
   fill_message(r, o, res->message, sizeof(res->message));
+  return SK_GPU_SUCCESS;
+}
+
+// solve.cpp:423-443 around the engine: presolve, the engine on the reduced LP, postsolve, and
+// the answer measured on the original LP (fill_result recomputes its objective and quality
+// there; the residuals reported are the engine's, on the reduced LP it solved).
+int32_t solve_presolved(const sk::HostLp& lp, const sk_pdhg_options& o, sk_pdhg_result* res,
+                        double* col_value, double* row_dual, double* col_dual, double* row_activity) {
+  const auto t0 = std::chrono::steady_clock::now();
+  const sk::presolve::Result P = sk::presolve::run(lp, {}, 1e-7);
+  const auto n = static_cast<size_t>(lp.num_cols), m = static_cast<size_t>(lp.num_rows);
+  auto elapsed = [&]() { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); };
+  if (P.proved_infeasible) {  // solve.cpp:425-442: no engine runs, and there is no certificate
+    if (col_value) std::fill(col_value, col_value + n, 0.0);
+    if (col_dual) std::fill(col_dual, col_dual + n, 0.0);
+    if (row_dual) std::fill(row_dual, row_dual + m, 0.0);
+    if (row_activity) std::fill(row_activity, row_activity + m, 0.0);
+    if (!res) return SK_GPU_SUCCESS;
+    std::memset(res, 0, sizeof(*res));
+    res->status = SK_SOLVE_INFEASIBLE;
+    res->dual_bound = std::numeric_limits<double>::infinity();
+    res->solve_seconds = elapsed();
+    std::snprintf(res->algorithm, sizeof(res->algorithm), "presolve");
+    std::snprintf(res->message, sizeof(res->message),
+                  "%s; proved by presolve, which carries no Farkas certificate - re-run with "
+                  "--option presolve=false for one", P.message.c_str());
+    return SK_GPU_SUCCESS;
+  }
+  const sk::HostLp& red = P.reduced;
+  sk::InstanceResult rr;
+  if (red.num_cols > 0) {
+    auto engine = sk::make_engine(to_engine(o));
+    engine->add(&red);
+    int32_t st = engine->upload();
+    if (!st) st = engine->solve();
+    if (!st) st = engine->result(0, &rr);
+    if (st) return st;
+  } else {
+    // Presolve removed every column: what is left is a set of rows with no entries, which
+    // hold exactly when zero lies within their bounds. Nothing remains to iterate on.
+    bool holds = true;
+    for (int32_t i = 0; i < red.num_rows; ++i)
+      holds = holds && !(red.row_lower[static_cast<size_t>(i)] > 1e-7) && !(red.row_upper[static_cast<size_t>(i)] < -1e-7);
+    rr.status = holds ? SK_SOLVE_OPTIMAL : SK_SOLVE_INFEASIBLE;
+    rr.converged = holds ? 1 : 0;
+    rr.dual_bound = holds ? red.objective_offset : std::numeric_limits<double>::infinity();
+    rr.row_dual.assign(static_cast<size_t>(red.num_rows), 0.0);
+  }
+  sk::InstanceResult full = rr;
+  full.col_value.assign(n, 0.0);
+  full.row_dual.assign(m, 0.0);
+  full.col_dual.assign(n, 0.0);
+  sk::presolve::postsolve(P, lp, rr.col_value.data(), rr.row_dual.data(), rr.col_dual.data(),
+                          full.col_value.data(), full.row_dual.data(), full.col_dual.data());
+  full.row_activity.assign(m, 0.0);
+  for (size_t j = 0; j < n; ++j)
+    for (int32_t p = lp.col_starts[j]; p < lp.col_starts[j + 1]; ++p)
+      full.row_activity[static_cast<size_t>(lp.row_indices[static_cast<size_t>(p)])] += lp.values[static_cast<size_t>(p)] * full.col_value[j];
+  const int32_t st = fill_result(lp, full, o, elapsed(), res, col_value, row_dual, col_dual, row_activity);
+  if (st || !res) return st;
+  char head[96];
+  std::snprintf(head, sizeof head, "presolve %d x %d -> %d x %d; ", lp.num_rows, lp.num_cols,
+                red.num_rows, red.num_cols);
+  const std::string msg = std::string(head) + res->message;
+  std::snprintf(res->message, sizeof(res->message), "%s", msg.c_str());
   return SK_GPU_SUCCESS;
 }
 
@@ -465,6 +538,11 @@ int32_t sk_pdhg_set_option(sk_pdhg_options* o, const char* key_in, const char* v
     o->gpu = b;
     return SK_GPU_SUCCESS;
   }
+  if (key == "presolve") {
+    if (!parse_bool(v, &b)) return bad("expected a boolean, got");
+    o->presolve = b;
+    return SK_GPU_SUCCESS;
+  }
   if (key == "pdhg_polish") {
     // read by the caller, which then hands the answer to sk_pdhg_polish (solve.cpp:184-243);
     // the first-order solve itself is the same either way
@@ -521,6 +599,7 @@ int32_t sk_pdhg_solve(const sk_lp* lp, const sk_pdhg_options* options, sk_pdhg_r
     if (options) o = *options;
     else sk_pdhg_default_options(&o);
     if (const int32_t refused = refuse_backend(o)) return refused;
+    if (o.presolve) return solve_presolved(lp->lp, o, result, col_value, row_dual, col_dual, row_activity);
     const auto t0 = std::chrono::steady_clock::now();
     auto engine = sk::make_engine(to_engine(o));
     engine->add(&lp->lp);
@@ -569,6 +648,124 @@ int32_t sk_pdhg_refine(const sk_lp* lp, const sk_pdhg_options* options, const sk
     return SK_GPU_SUCCESS;
   });
 }
+
+int32_t sk_presolve(const sk_lp* lp, const uint8_t* integer_columns, double feasibility,
+                    sk_presolved** out) {
+  return guarded([&]() -> int32_t {
+    if (!lp || !out) return SK_GPU_INVALID_ARGUMENT;
+    *out = nullptr;
+    const std::string problem = lp->lp.validate();
+    if (!problem.empty()) {
+      sk::set_last_error("sk_presolve: " + problem);
+      return SK_GPU_INVALID_ARGUMENT;
+    }
+    if (!(feasibility > 0.0)) feasibility = 1e-7;
+    if (feasibility < 1e-12 || feasibility > 1e-3) {  // the option's range (options.cpp:293-299)
+      sk::set_last_error("sk_presolve: primal_feasibility_tolerance outside [1e-12, 1e-3]");
+      return SK_GPU_INVALID_ARGUMENT;
+    }
+    std::vector<uint8_t> integer;
+    if (integer_columns) integer.assign(integer_columns, integer_columns + lp->lp.num_cols);
+    auto h = std::make_unique<sk_presolved>();
+    const auto t0 = std::chrono::steady_clock::now();
+    h->result = sk::presolve::run(lp->lp, integer, feasibility);
+    h->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    const sk::MpsInfo& f = lp->file;
+    if (f.col_names.size() == static_cast<size_t>(lp->lp.num_cols))
+      for (const int32_t j : h->result.col_to_original) h->col_names.push_back(f.col_names[static_cast<size_t>(j)]);
+    if (f.row_names.size() == static_cast<size_t>(lp->lp.num_rows))
+      for (const int32_t i : h->result.row_to_original) h->row_names.push_back(f.row_names[static_cast<size_t>(i)]);
+    *out = h.release();
+    return SK_GPU_SUCCESS;
+  });
+}
+
+int32_t sk_presolve_get_info(const sk_presolved* p, sk_presolve_info* info) {
+  if (!p || !info) return SK_GPU_INVALID_ARGUMENT;
+  const sk::presolve::Result& r = p->result;
+  std::memset(info, 0, sizeof(*info));
+  info->original_rows = r.original_rows;
+  info->original_cols = r.original_cols;
+  info->original_nonzeros = r.original_nonzeros;
+  info->reduced_rows = r.reduced.num_rows;
+  info->reduced_cols = r.reduced.num_cols;
+  info->reduced_nonzeros = r.proved_infeasible ? 0 : r.reduced.nnz();
+  info->records = static_cast<int32_t>(r.records.size());
+  info->passes = r.passes;
+  info->proved_infeasible = r.proved_infeasible ? 1 : 0;
+  for (const sk::presolve::Record& rec : r.records) ++info->by_kind[static_cast<int>(rec.kind)];
+  info->seconds = p->seconds;
+  return SK_GPU_SUCCESS;
+}
+
+const char* sk_presolve_message(const sk_presolved* p) { return p ? p->result.message.c_str() : ""; }
+
+int32_t sk_presolve_reduced_lp(const sk_presolved* p, sk_lp** out) {
+  return guarded([&]() -> int32_t {
+    if (!p || !out) return SK_GPU_INVALID_ARGUMENT;
+    *out = nullptr;
+    if (p->result.proved_infeasible) {
+      sk::set_last_error("sk_presolve_reduced_lp: presolve proved the LP infeasible; there is no reduced LP");
+      return SK_GPU_INVALID_STATE;
+    }
+    auto h = std::make_unique<sk_lp>();
+    h->lp = p->result.reduced;
+    h->file.col_names = p->col_names;
+    h->file.row_names = p->row_names;
+    *out = h.release();
+    return SK_GPU_SUCCESS;
+  });
+}
+
+int32_t sk_presolve_maps(const sk_presolved* p, int32_t* col_to_original, int32_t* row_to_original) {
+  if (!p) return SK_GPU_INVALID_ARGUMENT;
+  const sk::presolve::Result& r = p->result;
+  if (col_to_original) std::copy(r.col_to_original.begin(), r.col_to_original.end(), col_to_original);
+  if (row_to_original) std::copy(r.row_to_original.begin(), r.row_to_original.end(), row_to_original);
+  return SK_GPU_SUCCESS;
+}
+
+int32_t sk_presolve_get_record(const sk_presolved* p, int32_t k, sk_presolve_record* out) {
+  if (!p || !out || k < 0 || static_cast<size_t>(k) >= p->result.records.size()) return SK_GPU_INVALID_ARGUMENT;
+  const sk::presolve::Record& rec = p->result.records[static_cast<size_t>(k)];
+  out->kind = static_cast<int32_t>(rec.kind);
+  out->index = rec.index;
+  out->value = rec.value;
+  out->coefficient = rec.coefficient;
+  out->row_lower = rec.row_lower;
+  out->row_upper = rec.row_upper;
+  out->column = rec.column;
+  out->partner_column = rec.partner_column;
+  out->substituted_rhs = rec.substituted_rhs;
+  out->partner_coefficient = rec.partner_coefficient;
+  out->eliminated_cost = rec.eliminated_cost;
+  return SK_GPU_SUCCESS;
+}
+
+int32_t sk_postsolve(const sk_presolved* p, const sk_lp* original, const double* col_value,
+                     const double* row_dual, const double* col_dual, double* out_col_value,
+                     double* out_row_dual, double* out_col_dual) {
+  return guarded([&]() -> int32_t {
+    if (!p || !original || !out_col_value || !out_row_dual || !out_col_dual) return SK_GPU_INVALID_ARGUMENT;
+    const sk::presolve::Result& r = p->result;
+    if (r.proved_infeasible) {
+      sk::set_last_error("sk_postsolve: presolve proved the LP infeasible; there is nothing to postsolve");
+      return SK_GPU_INVALID_STATE;
+    }
+    if (original->lp.num_rows != r.original_rows || original->lp.num_cols != r.original_cols ||
+        original->lp.nnz() != r.original_nonzeros) {
+      sk::set_last_error("sk_postsolve: `original` is not the LP this presolve reduced");
+      return SK_GPU_INVALID_ARGUMENT;
+    }
+    if ((r.reduced.num_cols > 0 && (!col_value || !col_dual)) || (r.reduced.num_rows > 0 && !row_dual))
+      return SK_GPU_INVALID_ARGUMENT;
+    sk::presolve::postsolve(r, original->lp, col_value, row_dual, col_dual, out_col_value,
+                            out_row_dual, out_col_dual);
+    return SK_GPU_SUCCESS;
+  });
+}
+
+void sk_presolve_free(sk_presolved* p) { delete p; }
 
 int32_t sk_batch_warm_start(sk_batch* b, int32_t index, const double* col_value,
                             const double* row_dual, double primal_weight, double step_size) {

@@ -1,0 +1,68 @@
+"""The GPU engine behind presolve (sk_pdhg_solve with presolve=true: solve.cpp:423-443's
+sequence around the engine). The one-call path is bit for bit presolve -> the engine on the
+reduced LP -> postsolve; on Netlib the FP64 engine behind presolve makes the oracle's decisions
+(iterations inside the range the oracle gives under one-ulp cost moves, on the reduced LP) and
+lands on the reference's published objective; on the 200,000 x 500,000 power-law LP, where
+presolve fires 88 reductions (51 doubleton equations with fill-in), the FP32 answer postsolved to
+the original LP meets 1e-4 there, as the oracle measures it."""
+import json
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = Path(__file__).resolve().parent / "golden"
+PUBLISHED = json.loads((GOLDEN / "pdhg_published.json").read_text())["instances"]
+ULP_MOVES = (1.0, 1.0 + 2.0 ** -52, 1.0 - 2.0 ** -53, 1.0 + 2.0 ** -51, 1.0 - 2.0 ** -52,
+             1.0 + 1e-13, 1.0 - 1e-13)
+REL = dict(pdhg_stop_at_request=True, pdhg_absolute_primal=False)
+
+
+@pytest.mark.parametrize("name", ["afiro", "sc50a", "blend"])
+def test_the_one_call_path_is_presolve_engine_postsolve(gpu, name):
+    lp = gpu.Lp.read_mps(GOLDEN / "netlib" / f"{name}.mps")
+    one = gpu.solve(lp, presolve=True, pdhg_precision="fp64")
+    p = gpu.presolve(lp)
+    r = gpu.solve(p.reduced, pdhg_precision="fp64")
+    x, y, d = p.postsolve(r.col_value, r.row_dual, r.col_dual)
+    assert one.iterations == r.iterations and one.status == r.status
+    np.testing.assert_array_equal(one.col_value, x)
+    np.testing.assert_array_equal(one.row_dual, y)
+    np.testing.assert_array_equal(one.col_dual, d)
+    assert one.info["message"].startswith(f"presolve {lp.dims[0]} x {lp.dims[1]} -> ")
+
+
+@pytest.mark.parametrize("name", ["afiro", "sc50a", "sc50b", "blend", "sc105"])
+def test_fp64_engine_behind_presolve_makes_the_oracles_decisions(gpu, oracle_mod, name):
+    lp = gpu.Lp.read_mps(GOLDEN / "netlib" / f"{name}.mps")
+    d = lp.export()
+    red = gpu.presolve(lp).reduced.export()
+    counts = [oracle_mod.solve_pdhg(dict(red, cost=red["cost"] * f), tolerance=1e-4)["iterations"]
+              for f in ULP_MOVES]
+    r = gpu.solve(lp, presolve=True, pdhg_precision="fp64")
+    assert r.status == 1, r.info["message"]
+    assert min(counts) <= r.iterations <= max(counts), (r.iterations, counts)
+    assert counts[0] == PUBLISHED[name]["iterations"]  # the oracle's own, pinned on the CPU
+    published = float(PUBLISHED[name]["objective"])
+    assert abs(r.objective - published) <= 1e-7 * abs(published), (r.objective, published)
+    # the reported quality is that of the postsolved answer on the original LP
+    q = oracle_mod.recompute_quality(d, r.col_value, r.row_dual, r.col_dual, r.dual_bound)
+    for f in ("objective", "primal_infeasibility", "dual_infeasibility", "complementarity_violation"):
+        assert abs(getattr(r, f) - q[f]) <= 1e-12 * max(1.0, abs(q[f])), f
+
+
+def test_power_law_200k_behind_presolve(gpu, oracle_mod):
+    lp = gpu.Lp.generate(2, 11)
+    d = lp.export()
+    p = gpu.presolve(lp)
+    assert p.info["by_kind"]["doubleton_equation"] == 51 and p.info["records"] == 88
+    r = gpu.solve(lp, presolve=True, pdhg_tolerance=1e-4, **REL)
+    assert r.converged, r.info["message"]
+    ev = oracle_mod.evaluate(d, r.col_value, -r.row_dual)
+    print(f"presolve {p.info['seconds']:.3f} s; {r.iterations} iterations; original-LP KKT "
+          f"primal {ev['primal']:.2e} dual {ev['dual']:.2e} gap {ev['gap']:.2e}")
+    assert ev["worst"] <= 1e-4 * (1 + 1e-6)
+    plain = gpu.solve(lp, pdhg_tolerance=1e-4, **REL)
+    assert abs(r.objective - plain.objective) <= 1e-3 * abs(plain.objective)
