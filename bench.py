@@ -337,6 +337,23 @@ def fp64_finish(sk, lp, first, tolerance: float, use_graph: int) -> dict:
                       "declines an LP of this size)"}
 
 
+def presolve_cost(sk, lp) -> dict:
+    """Host cost of the reference's presolve and postsolve (sk_presolve, sk_postsolve) on the
+    workload LP, beside the engine's solve: they run in solve() around every LP engine
+    (solve.cpp:423-443). The bench's timed solves run without them, as the reference's own
+    PDHG benchmark would be compared engine to engine."""
+    import numpy as np
+    t0 = time.perf_counter()
+    p = sk.presolve(lp)
+    t1 = time.perf_counter()
+    nr, nc = p.info["reduced_rows"], p.info["reduced_cols"]
+    p.postsolve(np.zeros(nc), np.zeros(nr), np.zeros(nc))
+    t2 = time.perf_counter()
+    return {"presolve_seconds": t1 - t0, "postsolve_seconds": t2 - t1, "passes": p.info["passes"],
+            "records": {k: v for k, v in p.info["by_kind"].items() if v},
+            "reduced": [nr, nc, p.info["reduced_nonzeros"]]}
+
+
 def polish_sample(sk, lps, results, count: int) -> dict:
     """The interior-point finish (sk_pdhg_polish) of the first `count` batch answers: the
     step from the 1e-4 first-order answer to an optimal one within 1e-6 relative objective."""
@@ -517,6 +534,8 @@ def main() -> int:
         line["precision_comparison"] = precision_comparison(sk, lps[0], 120.0, layout["use_graph"])
     if args.workload == "c5" and not args.no_compare:
         line["finish"] = fp64_finish(sk, lps[0], results[0], 1e-7, layout["use_graph"])
+    if args.workload in ("c2", "c5") and not args.no_compare:
+        line["presolve"] = presolve_cost(sk, lps[0])
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(lps, args.cpu_budget)
     if not args.no_serve and args.workload == "c3":
