@@ -602,11 +602,12 @@ void sk_batch_destroy(sk_batch* batch);
 
 /* ---- serving: a continuous-batching request loop -------------------------------------------
  * No counterpart in the reference, which solves one LP per call (src/core/solve.cpp:385-399).
- * Requests queue in arrival order; `slots` worker threads, each owning a batch engine with its
- * own stream, take every waiting request that fits the batch and device-memory limits as one
- * block-diagonal batch, so batches overlap on the device. Each request completes (retires) as
- * soon as its own instance stops, while the rest of its batch keeps running. A request's
- * answer is bitwise the same as sk_pdhg_solve's with the same options. */
+ * Requests queue in arrival order; `slots` worker threads, each owning `lanes` batch engines
+ * with their own streams, take every waiting request that fits the batch and device-memory
+ * limits as one block-diagonal batch, so batches overlap on the device. Each request completes
+ * (retires) as soon as its own instance stops, while the rest of its batch keeps running, and
+ * once a batch is in its tail its slot admits the next batch beside it. A request's answer is
+ * bitwise the same as sk_pdhg_solve's with the same options. */
 typedef struct sk_server sk_server;
 
 typedef struct {
@@ -615,7 +616,13 @@ typedef struct {
   int32_t max_batch;      /* instances per batch (default 256) */
   int64_t memory_budget;  /* device bytes all slots may hold; 0: 90% of free memory at create */
   int32_t keep_solutions; /* 1: keep the primal/dual vectors for sk_server_result (default 1) */
-  int32_t reserved;
+  int32_t lanes;          /* batches one slot may run at once, 1..8 (default 1): with more,
+                             once the slot's newest batch is in its tail the queue head is
+                             admitted into the next idle lane (its own engine and stream) while
+                             the tail finishes. It shortens the wait of a one-slot server; with
+                             two slots the device is already full and it lengthens latency */
+  double tail_fraction;   /* a batch is in its tail when at most this share of its instances
+                             is still running (default 0.25) */
 } sk_server_options;
 
 typedef struct {

@@ -98,7 +98,7 @@ class BatchStats(C.Structure):
 
 class ServerOptions(C.Structure):
     _fields_ = [("slots", i32), ("max_batch", i32), ("memory_budget", i64),
-                ("keep_solutions", i32), ("reserved", i32)]
+                ("keep_solutions", i32), ("lanes", i32), ("tail_fraction", f64)]
 
 
 class ServerRecord(C.Structure):

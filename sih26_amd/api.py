@@ -499,11 +499,15 @@ class Server:
 
     def __init__(self, options: _ffi.PdhgOptions | None = None, *, slots: int = 2,
                  max_batch: int = 256, memory_budget: int = 0, keep_solutions: bool = True,
-                 **kw):
+                 lanes: int | None = None, tail_fraction: float | None = None, **kw):
         self.options = options if options is not None else make_options(**kw)
         so = _ffi.ServerOptions()
         lib().sk_server_default_options(C.byref(so))
         so.slots, so.max_batch = slots, max_batch
+        if lanes is not None:
+            so.lanes = lanes
+        if tail_fraction is not None:
+            so.tail_fraction = tail_fraction
         so.memory_budget, so.keep_solutions = memory_budget, int(bool(keep_solutions))
         self.server_options = so
         h = C.c_void_p()

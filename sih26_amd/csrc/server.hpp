@@ -1,11 +1,14 @@
 // Continuous-batching request loop over the batched PDHG engine, on one HIP device.
 //
-// Requests (one LP each) enter a FIFO queue. Each of `slots` worker threads owns an engine
-// with its own stream; whenever a slot is free it takes, from the head of the queue, every
-// waiting request that fits its batch limits and the device-memory budget, uploads them as
-// one block-diagonal batch and starts it. Slots run concurrently, so the batches overlap on
-// the device. While a batch runs, every instance that stops is read back and completed at
-// once (it retires mid-batch); the slot takes its next batch when the last one stops.
+// Requests (one LP each) enter a FIFO queue. Each of `slots` worker threads owns `lanes`
+// engines, each with its own stream; whenever a slot has nothing running it takes, from the
+// head of the queue, every waiting request that fits its batch limits and the device-memory
+// budget, uploads them as one block-diagonal batch on an idle lane and starts it. Slots run
+// concurrently, so the batches overlap on the device. While a batch runs, every instance that
+// stops is read back and completed at once (it retires mid-batch). Once a batch is in its tail
+// (at most `tail_fraction` of its instances still running) the slot admits the queue head into
+// its next idle lane, so the capacity the retired instances freed serves new requests while
+// the tail finishes; a slot runs at most `lanes` batches at once.
 //
 // Admission by memory footprint: the server accounts the device memory every slot holds
 // (engines keep their buffers between batches, growing each one as needed) and admits a batch
@@ -36,6 +39,9 @@ struct ServerOptions {
   int32_t slots = 2;           // concurrent batches (engines, each with its own streams)
   int32_t max_batch = 256;     // instances per batch
   int64_t memory_budget = 0;   // device bytes all slots may hold; 0: 90% of free memory
+  int32_t lanes = 1;           // batches one slot may run at once (1: one batch per slot)
+  double tail_fraction = 0.25; // a batch is in its tail when at most this share of its
+                               // instances is still running
 };
 
 struct Request {
@@ -92,16 +98,31 @@ class Server {
   struct Extent {
     int64_t rows = 0, cols = 0, nnz = 0, instances = 0;
   };
-  struct Slot {
+  // One engine of a slot and the batch it runs. `held` and `extent` are shared with the other
+  // slot threads (under mu_); the batch and its counts belong to the slot's own thread.
+  struct Lane {
     std::unique_ptr<Engine> engine;
     int64_t held = 0;   // device bytes its engine holds
     Extent extent;      // per-dimension maxima of its batches since the last release
-    bool busy = false;
+    bool running = false;
+    std::vector<std::unique_ptr<Request>> batch;
+    int32_t open = 0;   // instances of the batch still running
+    uint64_t started = 0;  // order in which the slot's lanes took their batches
+  };
+  struct Slot {
+    std::vector<Lane> lanes;
   };
   void run_slot(int index);
-  bool take_batch(int index, std::vector<std::unique_ptr<Request>>* batch);
+  // Forms the next batch of lane `lane` of slot `index`; with `block`, waits until one can
+  // be formed. Returns false when none was formed (or the server stops).
+  bool take_batch(int index, int lane, std::vector<std::unique_ptr<Request>>* batch, bool block);
+  int start_lane(Lane& lane);
+  // Reads back what stopped; returns whether anything did. A failed batch completes every
+  // request still in it with the error.
+  bool poll_lane(Lane& lane, bool wait);
   void complete(std::unique_ptr<Request> r);
-  int64_t held_by_others(int index) const;
+  int64_t held_by_others(int index, int lane) const;
+  void release_lane(Lane& lane);  // under mu_
 
   ServerOptions opt_;
   Hook hook_;
@@ -114,7 +135,7 @@ class Server {
   std::vector<std::thread> threads_;
   bool stopping_ = false;
   bool release_wanted_ = false;
-  int release_requester_ = -1;
+  int release_requester_ = -1;  // slot * kMaxLanes + lane
   int64_t next_id_ = 0;
   int64_t outstanding_ = 0;
   ServerStats stats_;

@@ -159,6 +159,7 @@ def test_server_options_and_missing_device():
     so = _ffi.ServerOptions()
     sk.lib().sk_server_default_options(C.byref(so))
     assert (so.slots, so.max_batch, so.memory_budget, so.keep_solutions) == (2, 256, 0, 1)
+    assert (so.lanes, so.tail_fraction) == (1, 0.25)
     if sk.gpu_available():
         pytest.skip("a device is visible; the missing-device path is for CPU-only hosts")
     h = C.c_void_p()
@@ -168,6 +169,10 @@ def test_server_options_and_missing_device():
     sk.lib().sk_server_default_options(C.byref(bad))
     bad.slots = 0
     assert sk.lib().sk_server_create(None, C.byref(bad), C.byref(h)) == 10  # invalid argument
+    for field, value in (("lanes", 0), ("lanes", 9), ("tail_fraction", 1.5), ("tail_fraction", float("nan"))):
+        sk.lib().sk_server_default_options(C.byref(bad))
+        setattr(bad, field, value)
+        assert sk.lib().sk_server_create(None, C.byref(bad), C.byref(h)) == 10, field
     # null handles are argument errors, not crashes
     assert sk.lib().sk_server_submit(None, None, -1.0, None) == 10
 

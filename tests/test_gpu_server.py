@@ -112,3 +112,49 @@ def test_memory_budget_admission_and_rejection(gpu):
     # the refused model is intact and solves on its own
     r = gpu.solve(big, **OPTS)
     assert r.converged
+
+
+def _tail_scenario(gpu, lanes):
+    """One slot of at most 4 instances. A first request keeps the slot busy while the first
+    wave (one 40,000 x 80,000 request and three small ones) and the second wave (four small
+    ones) queue behind it, so the first wave is one batch and the second comes next."""
+    big = gpu.Lp.generate_params(**dict(gpu.Lp.preset(4, 0), num_rows=40000, num_cols=80000,
+                                        num_equality_rows=24000))
+    lps = [gpu.Lp.generate(3, 60)] + [big] + [gpu.Lp.generate(3, k) for k in (61, 62, 63)] + \
+        [gpu.Lp.generate(3, k) for k in (64, 65, 66, 67)]
+    dims = [lp.dims[:2] for lp in lps]
+    srv = gpu.Server(slots=1, max_batch=4, lanes=lanes, **OPTS)
+    ids = [srv.submit(lps[0])]
+    t_end = time.time() + 30.0
+    while srv.stats()["batches"] < 1 and time.time() < t_end:  # the first request is running
+        time.sleep(0.0002)
+    ids += [srv.submit(lp) for lp in lps[1:]]
+    got = _drain(srv, dict(zip(ids, dims)), timeout=300.0)
+    st = srv.stats()
+    srv.close()
+    return [got[i] for i in ids], st
+
+
+def test_a_batch_in_its_tail_lets_the_next_one_start(gpu):
+    """With two lanes, the second wave is admitted once the first wave's batch is down to its
+    last instance (the large one), long before that instance finishes; with one lane it waits
+    for the whole batch. Every answer is the same either way, bit for bit."""
+    two, st2 = _tail_scenario(gpu, 2)
+    one, st1 = _tail_scenario(gpu, 1)
+    for got, st in ((two, st2), (one, st1)):
+        assert st["completed"] == 9 and st["failed"] == 0
+        assert all(rec["error"] == 0 and res.converged for res, rec in got)
+        assert got[1][1]["admitted"] == got[4][1]["admitted"]  # the first wave is one batch
+    big_done_2, big_done_1 = two[1][1]["finished"], one[1][1]["finished"]
+    wave2_in_2 = min(rec["admitted"] for _, rec in two[5:])
+    wave2_in_1 = min(rec["admitted"] for _, rec in one[5:])
+    small_done_2 = max(rec["finished"] for _, rec in two[2:5])
+    print(f"two lanes: second wave admitted {1e3 * (wave2_in_2 - small_done_2):.2f} ms after the "
+          f"first wave's small ones, {1e3 * (big_done_2 - wave2_in_2):.1f} ms before the large one "
+          f"finished; one lane: {1e3 * (wave2_in_1 - big_done_1):.2f} ms after it")
+    assert wave2_in_2 < big_done_2
+    assert wave2_in_1 >= big_done_1
+    for (a, _), (b, _) in zip(two, one):
+        assert a.iterations == b.iterations
+        np.testing.assert_array_equal(a.col_value, b.col_value)
+        np.testing.assert_array_equal(a.row_dual, b.row_dual)
