@@ -114,3 +114,34 @@ def test_library_mps_reader_feeds_the_published_optimum(gpu, name):
     r = gpu.polish(lp, first)
     assert r.status_name == "optimal", r.info["message"]
     assert rel(r.objective, OPTIMA[name]["published_optimal"]) <= 1e-6
+
+
+def test_every_answer_of_the_256_lp_batch_passes_the_checker_after_the_finish(gpu, tmp_path):
+    """The headline workload end to end: the 256-LP batch solved together (FP32 split engine,
+    1e-4 relative KKT), then each answer finished by the interior point and written as the
+    reference's .sol file, and every one VERIFIED by the independent checker, with the
+    finished objective within the first-order answer's own bound of it."""
+    import time
+    lps = [gpu.Lp.generate(3, k) for k in range(256)]
+    b = gpu.Batch(**FIRST)
+    for lp in lps:
+        b.add(lp)
+    b.upload()
+    b.solve()
+    t0, worst = time.perf_counter(), 0.0
+    for k, lp in enumerate(lps):
+        first = b.result(k)
+        assert first.converged, (k, first.info["message"])
+        r = gpu.polish(lp, first)
+        assert r.status_name == "optimal", (k, r.info["message"])
+        d = lp.export()
+        sol = tmp_path / f"c3_{k}.sol"
+        gpu.write_solution(sol, lp, r, model_name=f"c3_{k}")
+        report = checker.verify(d, checker.read_sol(sol))
+        assert report.verified, (k, [line for line in report.lines if not line[0]])
+        bound = 2e-4 * (1 + abs(first.primal_objective) + abs(first.dual_objective))
+        assert abs(first.objective - r.objective) <= bound, k
+        worst = max(worst, abs(first.objective - r.objective) / max(1.0, abs(r.objective)))
+        sol.unlink()
+    print(f"256 answers finished and verified in {time.perf_counter() - t0:.1f} s; first-order "
+          f"objectives within {worst:.2e} relative of the finished ones")
