@@ -119,11 +119,20 @@ std::vector<int32_t> relabel_columns(const HostLp& lp, size_t word, int32_t mode
   const auto n = static_cast<size_t>(lp.num_cols);
   if (mode == 2 || (mode == 0 && n < kRelabelColumns)) return {};
   const size_t hot = std::min(n, kHotBytes / word);
-  std::vector<int32_t> order(n);
-  for (size_t j = 0; j < n; ++j) order[j] = static_cast<int32_t>(j);
-  auto degree = [&](int32_t j) { return lp.col_starts[static_cast<size_t>(j) + 1] - lp.col_starts[static_cast<size_t>(j)]; };
-  std::partial_sort(order.begin(), order.begin() + static_cast<std::ptrdiff_t>(hot), order.end(),
-                    [&](int32_t a, int32_t b) { return degree(a) != degree(b) ? degree(a) > degree(b) : a < b; });
+  // the head: columns by degree, largest first, ties in index order (a counting sort over the
+  // degrees, which are at most the row count); the tail: the other columns in index order
+  auto degree = [&](size_t j) {
+    return static_cast<size_t>(lp.col_starts[j + 1] - lp.col_starts[j]);
+  };
+  size_t top = 0;
+  for (size_t j = 0; j < n; ++j) top = std::max(top, degree(j));
+  std::vector<size_t> first(top + 2, 0);  // first[top - d]: where degree d starts
+  for (size_t j = 0; j < n; ++j) ++first[top - degree(j) + 1];
+  for (size_t d = 0; d <= top; ++d) first[d + 1] += first[d];
+  std::vector<int32_t> by_degree(n);
+  for (size_t j = 0; j < n; ++j) by_degree[first[top - degree(j)]++] = static_cast<int32_t>(j);
+  std::vector<int32_t> order(by_degree.begin(), by_degree.begin() + static_cast<std::ptrdiff_t>(hot));
+  order.resize(n);
   std::vector<uint8_t> taken(n, 0);
   for (size_t k = 0; k < hot; ++k) taken[static_cast<size_t>(order[k])] = 1;
   size_t next = hot;

@@ -95,6 +95,14 @@ def test_step_products_match_the_oracle(gpu, oracle_mod, case):
     np.testing.assert_array_equal(b.probe_scaling(0)["values"], scaled)
     order = b.column_order(0)
     assert sorted(order) == list(range(n))
+    if opts.get("column_order") == 1:
+        # the 2 MiB of columns with the most nonzeros first (largest first, ties in index
+        # order), then the rest in the LP's order
+        deg = np.diff(d["col_starts"])
+        hot = min(n, (2 << 20) // (8 if fp64 else 4))
+        head = sorted(range(n), key=lambda j: (-deg[j], j))[:hot]
+        rest = sorted(set(range(n)) - set(head))
+        np.testing.assert_array_equal(order, head + rest)
     rng = np.random.default_rng(11)
     x, y = rng.standard_normal(n), rng.standard_normal(m)
 
