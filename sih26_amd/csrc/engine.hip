@@ -2,11 +2,15 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <exception>
+#include <mutex>
 #include <random>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "engine.hpp"
@@ -142,6 +146,41 @@ std::vector<int32_t> relabel_columns(const HostLp& lp, size_t word, int32_t mode
   for (size_t j = 0; j < n && identity; ++j) identity = order[j] == static_cast<int32_t>(j);
   if (identity) order.clear();
   return order;
+}
+
+// f(k, scratch) for every instance k of a batch, on up to 16 host threads that take instances
+// in turn (one thread below ~100k nonzeros); `scratch` is the calling thread's vector, each
+// other thread gets its own. An exception in a worker is rethrown here after all have joined.
+template <class V, class F>
+void for_each_instance(int32_t count, size_t nnz, std::vector<V>* scratch, F&& f) {
+  const unsigned hw = std::thread::hardware_concurrency();
+  const int64_t threads = std::min<int64_t>({16, hw ? static_cast<int64_t>(hw) : 1, count,
+                                            static_cast<int64_t>(nnz / 100000) + 1});
+  if (threads <= 1) {
+    for (int32_t k = 0; k < count; ++k) f(k, *scratch);
+    return;
+  }
+  std::atomic<int32_t> next{0};
+  std::exception_ptr failed;
+  std::mutex failed_mu;
+  auto work = [&](std::vector<V>& mine) {
+    try {
+      for (int32_t k; (k = next.fetch_add(1)) < count;) f(k, mine);
+    } catch (...) {
+      std::lock_guard<std::mutex> lk(failed_mu);
+      if (!failed) failed = std::current_exception();
+      next.store(count);
+    }
+  };
+  std::vector<std::thread> pool;
+  for (int64_t t = 1; t < threads; ++t)
+    pool.emplace_back([&]() {
+      std::vector<V> mine;
+      work(mine);
+    });
+  work(*scratch);
+  for (std::thread& th : pool) th.join();
+  if (failed) std::rethrow_exception(failed);
 }
 
 void build_chunks(int32_t begin, int32_t count, int32_t inst, std::vector<Tile>* out) {
@@ -291,6 +330,13 @@ class EngineImpl final : public Engine {
   bool pack_ = true;    // packed index words in the staging order
   int32_t cus_ = 256;   // compute units of the device
 
+  // host staging of an upload (the sources of its copies), reused by the next one
+  struct HostStage {
+    std::vector<int32_t> a_ptr, a_idx, t_ptr, t_idx, next, dev_of;
+    std::vector<double> a_orig, t_orig, cmin, cl, cu, rl, ru, draw;
+    std::vector<Tile> ta, tt, tc, tr;
+  };
+  HostStage stage_host_;
   // host copies needed after upload
   std::vector<InstDesc> desc_;
   std::vector<InstState> init_state_;
@@ -400,27 +446,67 @@ int EngineImpl<T>::upload() {
 
   // Block-diagonal CSR(A) (columns ascending in each row, the order of the reference's CSC
   // scatter) and CSR(A^T) (= the CSC of A, rows ascending in each column).
-  std::vector<int32_t> a_ptr(M + 1, 0), a_idx(Z), t_ptr(N + 1, 0), t_idx(Z);
-  std::vector<double> a_orig(Z), t_orig(Z), cmin(N), cl(N), cu(N), rl(M), ru(M);
+  // host staging of the device arrays, kept by the engine between uploads so a serving slot
+  // does not fault in fresh pages for every batch (every element is written below)
+  HostStage& h = stage_host_;
+  std::vector<int32_t>& a_ptr = h.a_ptr;
+  std::vector<int32_t>& a_idx = h.a_idx;
+  std::vector<int32_t>& t_ptr = h.t_ptr;
+  std::vector<int32_t>& t_idx = h.t_idx;
+  std::vector<double>& a_orig = h.a_orig;
+  std::vector<double>& t_orig = h.t_orig;
+  std::vector<double>& cmin = h.cmin;
+  std::vector<double>& cl = h.cl;
+  std::vector<double>& cu = h.cu;
+  std::vector<double>& rl = h.rl;
+  std::vector<double>& ru = h.ru;
+  a_ptr.assign(M + 1, 0);
+  t_ptr.assign(N + 1, 0);
+  a_idx.resize(Z);
+  t_idx.resize(Z);
+  a_orig.resize(Z);
+  t_orig.resize(Z);
+  cmin.resize(N);
+  cl.resize(N);
+  cu.resize(N);
+  rl.resize(M);
+  ru.resize(M);
   desc_.assign(static_cast<size_t>(K_), InstDesc{});
   init_state_.assign(static_cast<size_t>(K_), InstState{});
   sense_.assign(static_cast<size_t>(K_), 1);
   trivial_norm_.assign(static_cast<size_t>(K_), 0);
   v0_.assign(N, 0.0);
-  std::vector<Tile> ta, tt, tc, tr;
+  std::vector<Tile>& ta = h.ta;
+  std::vector<Tile>& tt = h.tt;
+  std::vector<Tile>& tc = h.tc;
+  std::vector<Tile>& tr = h.tr;
+  ta.clear();
+  tt.clear();
+  tc.clear();
+  tr.clear();
   col_perm_.assign(static_cast<size_t>(K_), {});
-  int32_t row0 = 0, col0 = 0, nz0 = 0;
   const int64_t limit = opt_.iteration_limit < 0 ? 1000000 : opt_.iteration_limit;
+  // every instance's offsets first, then the instances prepared independently (their rows,
+  // columns and nonzeros are disjoint ranges of the batch arrays), on host threads
+  std::vector<int32_t> row_off(static_cast<size_t>(K_) + 1, 0), col_off(row_off), nz_off(row_off);
   for (int32_t k = 0; k < K_; ++k) {
     const HostLp& lp = *lps_[static_cast<size_t>(k)];
+    row_off[static_cast<size_t>(k) + 1] = row_off[static_cast<size_t>(k)] + lp.num_rows;
+    col_off[static_cast<size_t>(k) + 1] = col_off[static_cast<size_t>(k)] + lp.num_cols;
+    nz_off[static_cast<size_t>(k) + 1] = nz_off[static_cast<size_t>(k)] + static_cast<int32_t>(lp.nnz());
+  }
+  for_each_instance(K_, Z, &h.draw, [&](int32_t k, std::vector<double>& draw) {
+    const HostLp& lp = *lps_[static_cast<size_t>(k)];
     const int32_t m = lp.num_rows, n = lp.num_cols;
+    const int32_t row0 = row_off[static_cast<size_t>(k)], col0 = col_off[static_cast<size_t>(k)];
+    const int32_t nz0 = nz_off[static_cast<size_t>(k)];
     // counts per row
     for (int32_t p = 0; p < lp.nnz(); ++p) ++a_ptr[static_cast<size_t>(row0 + lp.row_indices[p]) + 1];
     std::vector<int32_t>& perm = col_perm_[static_cast<size_t>(k)];
     perm = relabel_columns(lp, sizeof(T), opt_.column_order);
     auto lp_col = [&](int32_t j) { return perm.empty() ? j : perm[static_cast<size_t>(j)]; };
     int32_t nz = nz0;
-    t_ptr[static_cast<size_t>(col0)] = nz0;
+    if (k == 0) t_ptr[0] = 0;  // every later start is the end its predecessor writes
     for (int32_t j = 0; j < n; ++j) {
       const int32_t o = lp_col(j);
       for (int32_t p = lp.col_starts[o]; p < lp.col_starts[o + 1]; ++p, ++nz) {
@@ -452,7 +538,7 @@ int EngineImpl<T>::upload() {
       // drawn in the LP's column order, placed in the device order
       std::mt19937 rng(static_cast<unsigned>(opt_.random_seed) + 1u);
       std::uniform_real_distribution<double> spread(-1.0, 1.0);
-      std::vector<double> draw(static_cast<size_t>(n));
+      draw.resize(static_cast<size_t>(n));
       for (int32_t j = 0; j < n; ++j) draw[static_cast<size_t>(j)] = spread(rng);
       for (int32_t j = 0; j < n; ++j) v0_[static_cast<size_t>(col0 + j)] = draw[static_cast<size_t>(lp_col(j))];
     } else {
@@ -485,12 +571,9 @@ int EngineImpl<T>::upload() {
       s.done = 1;
       s.status = kStIterationLimit;
       s.report_pending = 1;
-      ++initially_done_;
     }
-    row0 += m;
-    col0 += n;
-    nz0 += static_cast<int32_t>(lp.nnz());
-  }
+  });
+  if (limit <= 0) initially_done_ = static_cast<uint32_t>(K_);
   {
     // an injected fault: a NaN cost in the first column of one instance (its first step is
     // non-finite, the others run on)
@@ -505,9 +588,9 @@ int EngineImpl<T>::upload() {
     // scatter (sparse.cpp:202-212) and its row statistics (scaling.cpp:44-100) add a row's
     // terms, so row sums (scaling, A x-bar, the evaluation's activities) do not depend on the
     // device order. The staging order sorts by device column anyway, for the gathers.
-    std::vector<int32_t> next(a_ptr.begin(), a_ptr.end() - 1);
-    std::vector<int32_t> dev_of;
-    for (int32_t k = 0; k < K_; ++k) {
+    std::vector<int32_t>& next = h.next;
+    next.assign(a_ptr.begin(), a_ptr.end() - 1);
+    for_each_instance(K_, Z, &h.dev_of, [&](int32_t k, std::vector<int32_t>& dev_of) {
       const InstDesc& d = desc_[static_cast<size_t>(k)];
       const std::vector<int32_t>& perm = col_perm_[static_cast<size_t>(k)];  // device -> LP
       dev_of.assign(perm.size(), 0);
@@ -520,7 +603,7 @@ int EngineImpl<T>::upload() {
           a_orig[slot] = t_orig[static_cast<size_t>(p)];
         }
       }
-    }
+    });
   }
   // Staging order of the SpMV tiles: sorted by gather index unless told otherwise.
   stage_ = opt_.stage < 0 ? 1 : opt_.stage;
@@ -1313,6 +1396,7 @@ void EngineImpl<T>::release() {
     b->bytes = 0;
   }
   args_ = Args<T>{};
+  stage_host_ = HostStage{};
   inflight_ = 0;
   setup_done_ = false;
   K_ = 0;
