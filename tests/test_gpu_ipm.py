@@ -9,7 +9,7 @@ import pytest
 
 from oracle import checker
 from tests.mps_ref import read_mps
-from tests.lp_cases import CLOSED_FORM, closed_form
+from tests.lp_cases import CLOSED_FORM, closed_form, dense_to_lp
 
 pytestmark = pytest.mark.gpu
 NETLIB = Path(__file__).resolve().parent / "golden" / "netlib"
@@ -145,3 +145,40 @@ def test_every_answer_of_the_256_lp_batch_passes_the_checker_after_the_finish(gp
         sol.unlink()
     print(f"256 answers finished and verified in {time.perf_counter() - t0:.1f} s; first-order "
           f"objectives within {worst:.2e} relative of the finished ones")
+
+
+def _polish_lp(gpu):
+    """test_pdhg.cpp:642-684: min -3x - 5y s.t. x <= 4, 2y <= 12, 3x + 2y <= 18, x, y >= 0;
+    optimum -36 at (2, 6); PDHG is stopped at 20 iterations on purpose."""
+    d = dense_to_lp([[1.0, 0.0], [0.0, 2.0], [3.0, 2.0]], [-np.inf] * 3, [4.0, 12.0, 18.0],
+                    [-3.0, -5.0])
+    return to_lp(gpu, d), d
+
+
+def test_polish_finishes_an_iteration_limited_point_to_the_standard(gpu):
+    lp, _ = _polish_lp(gpu)
+    first = gpu.solve(lp, iteration_limit=20)
+    assert first.status_name == "iteration_limit" and first.iterations == 20
+    r = gpu.polish(lp, first)
+    assert r.status_name == "optimal", r.info["message"]
+    assert r.algorithm == "pdhg-gpu+ipm"
+    assert abs(r.objective + 36.0) <= 1e-6
+    np.testing.assert_allclose(r.col_value, [2.0, 6.0], atol=1e-5)
+    assert r.polish_iterations > 0 and r.iterations == 20 + r.polish_iterations
+    assert r.primal_infeasibility <= 1e-7 and r.dual_infeasibility_scaled <= 1e-7
+    assert "polished by the interior point" in r.info["message"]
+
+
+def test_without_the_polish_the_limited_point_is_reported_as_what_it_is(gpu):
+    lp, _ = _polish_lp(gpu)
+    rough = gpu.solve(lp, iteration_limit=20)
+    assert rough.status_name == "iteration_limit"
+    assert rough.algorithm == "pdhg-gpu" and rough.iterations == 20 and rough.polish_iterations == 0
+
+
+def test_polish_declines_when_the_factor_cap_says_so_and_the_first_order_answer_stands(gpu):
+    lp, _ = _polish_lp(gpu)
+    first = gpu.solve(lp, iteration_limit=20)
+    r = gpu.polish(lp, first, max_factor_nonzeros=0)
+    assert r.status_name == "iteration_limit" and r.algorithm == "pdhg-gpu"
+    assert r.iterations == 20 and "declined" in r.info["message"]
