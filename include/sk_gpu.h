@@ -377,7 +377,7 @@ typedef struct {
   double absolute_gap;
   double relative_gap;
   char algorithm[16]; /* "pdhg-gpu" (verify.cpp:86-89 audits on "gpu") */
-  char message[256];
+  char message[512];
   int64_t polish_iterations; /* interior-point iterations of a polish (solve.cpp:225-242) */
   double primal_weight;      /* the primal weight omega and step size eta the run ended with,
                                 to continue it (sk_batch_warm_start, sk_pdhg_refine) */

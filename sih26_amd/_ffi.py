@@ -79,7 +79,7 @@ class PdhgResult(C.Structure):
                 ("dual_infeasibility", f64), ("dual_infeasibility_scaled", f64),
                 ("complementarity_violation", f64), ("absolute_gap", f64),
                 ("relative_gap", f64), ("algorithm", C.c_char * 16),
-                ("message", C.c_char * 256), ("polish_iterations", i64),
+                ("message", C.c_char * 512), ("polish_iterations", i64),
                 ("primal_weight", f64), ("step_size", f64)]
 
 

@@ -142,12 +142,16 @@ void fill_message(const sk::InstanceResult& r, const sk_pdhg_options& o, char* o
                   static_cast<long long>(r.iterations), static_cast<long long>(r.restarts),
                   f.absolute_primal, f.absolute_dual, f.gap_as_verified);
   } else if (r.status == SK_SOLVE_FEASIBLE) {
+    // pdhg.cpp:684-696, word for word
     std::snprintf(out, size,
                   "met the requested relative tolerance %.1e after %lld iterations, but NOT the "
-                  "absolute standard (primal %.3e vs 1e-07, dual %.3e vs 1e-07, relative gap "
-                  "%.3e vs 1e-09)",
-                  o.tolerance, static_cast<long long>(r.iterations), f.absolute_primal,
-                  f.absolute_dual, f.gap_as_verified);
+                  "absolute standard this project verifies against (primal %.3e vs %.1e, dual "
+                  "%.3e vs %.1e, relative gap %.3e vs %.1e). Reported as feasible, not optimal. %s",
+                  o.tolerance, static_cast<long long>(r.iterations), f.absolute_primal, 1e-7,
+                  f.absolute_dual, 1e-7, f.gap_as_verified, 1e-9,
+                  o.stop_at_request ? "pdhg_stop_at_request is on, so this is the cheap answer "
+                                      "that was asked for; turn it off to run on to the standard"
+                                    : "Tighten --option pdhg_tolerance to close it");
   } else if (r.status == SK_SOLVE_NUMERICAL_ERROR) {
     std::snprintf(out, size, "a PDHG step produced a non-finite value after %lld iterations",
                   static_cast<long long>(r.iterations));

@@ -97,3 +97,21 @@ def test_power_law_200k_behind_presolve(gpu, oracle_mod):
         print(f"{prec}: presolve {p.info['seconds']:.3f} s; {r.iterations} iterations (without "
               f"presolve {plain.iterations}); original-LP KKT primal {ev['primal']:.2e} dual "
               f"{ev['dual']:.2e} gap {ev['gap']:.2e}")
+
+
+def test_stop_at_request_gives_the_cheap_answer_and_never_calls_it_optimal(gpu, oracle_mod):
+    """test_pdhg.cpp:340-382 on the FP64 engine behind presolve (adlittle): the switch stops
+    earlier than the run to the standard, reports `feasible`, not `optimal`, with a point that
+    is primal feasible to 1e-7 and within 1e-3 of the honest objective, and says why."""
+    lp = gpu.Lp.read_mps(GOLDEN / "netlib" / "adlittle.mps")
+    opts = dict(presolve=True, pdhg_precision="fp64", pdhg_tolerance=1e-4, iteration_limit=1000000)
+    honest = gpu.solve(lp, **opts)
+    cheap = gpu.solve(lp, pdhg_stop_at_request=True, **opts)
+    assert honest.status_name == "optimal", honest.info["message"]
+    assert cheap.status_name == "feasible", cheap.info["message"]
+    assert cheap.iterations < honest.iterations
+    assert cheap.primal_infeasibility <= 1e-7, cheap.info["message"]
+    assert abs(cheap.objective - honest.objective) <= 1e-3 * max(1.0, abs(honest.objective))
+    assert "pdhg_stop_at_request" in cheap.info["message"]
+    print(f"adlittle behind presolve: {cheap.iterations} iterations to the request, "
+          f"{honest.iterations} to the standard (the reference: 132,520 and 192,080)")

@@ -76,3 +76,15 @@ def test_presolve_statistics_of_the_fixtures():
     assert seen["israel"] == {"singleton_row": 11}
     assert sum(seen["blend"].values()) == 6 and sum(seen["sc105"].values()) == 9
     np.testing.assert_equal(sorted(seen), sorted(PUBLISHED))
+
+
+def test_the_cheap_answer_is_the_published_one_too():
+    """test_pdhg.cpp:340-382 states that adlittle, behind presolve, meets a 1e-4 request at
+    132,520 iterations and the project standard at 192,080: the oracle behind this presolve
+    stops at exactly those counts, `feasible` on the request and `optimal` on the standard."""
+    lp, p = presolved("adlittle")
+    red = p.reduced.export()
+    cheap = oracle.solve_pdhg(red, tolerance=1e-4, stop_at_request=True, iteration_limit=1000000)
+    honest = oracle.solve_pdhg(red, tolerance=1e-4, iteration_limit=1000000)
+    assert (cheap["iterations"], cheap["status"]) == (132520, 5)
+    assert (honest["iterations"], honest["status"]) == (192080, 1)
