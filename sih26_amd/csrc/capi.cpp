@@ -80,7 +80,7 @@ sk::EngineOptions to_engine(const sk_pdhg_options& o) {
   e.tile_nnz = o.tile_nnz;
   e.streams = o.streams;
   e.column_order = o.column_order == 1 || o.column_order == 2 ? o.column_order : 0;
-  e.step_fusion = o.step_fusion == 1 ? 1 : 0;
+  e.step_fusion = o.step_fusion & 3;
   return e;
 }
 

@@ -627,7 +627,7 @@ int EngineImpl<T>::upload() {
   n_tt_ = static_cast<int32_t>(tt.size());
   // A step of an instance with a few tiles is a chain of short dependent launches; one
   // workgroup then runs its tiles one after another in a single launch
-  fused_ = opt_.step_fusion == 0 && K_ > 0;
+  fused_ = (opt_.step_fusion & 1) == 0 && K_ > 0;
   for (const InstDesc& d : desc_)
     fused_ = fused_ && (d.ta1 - d.ta0) + (d.tt1 - d.tt0) <= kFusedMaxTiles && d.tc1 - d.tc0 <= kFusedMaxChunks;
   n_tc_ = static_cast<int32_t>(tc.size());
@@ -782,8 +782,12 @@ int EngineImpl<T>::upload() {
   InstDesc* pdesc;
   SK_TRY(put(b_desc_, desc_, &pdesc));
   a.desc = pdesc;
-  SK_TRY(alloc(b_state_, static_cast<size_t>(K_), &a.state));
-  SK_TRY(alloc(b_ps_c_, static_cast<size_t>(n_tc_), &a.ps_c));
+  // two buffers of instance records and of primal partials: a primal launch that decides the
+  // previous step reads one and writes the other (enqueue_block)
+  SK_TRY(alloc(b_state_, 2 * static_cast<size_t>(K_), &a.state));
+  SK_TRY(alloc(b_ps_c_, 2 * static_cast<size_t>(n_tc_), &a.ps_c));
+  a.state_next = a.state;
+  a.ps_c_prev = a.ps_c + n_tc_;
   SK_TRY(alloc(b_ps_a_, static_cast<size_t>(n_ta_), &a.ps_a));
   SK_TRY(alloc(b_ps_t_, static_cast<size_t>(n_tt_), &a.ps_t));
   SK_TRY(alloc(b_pe_c_, static_cast<size_t>(n_tc_) * 2, &a.pe_c));
@@ -823,6 +827,7 @@ int EngineImpl<T>::upload() {
   a.n_tt = n_tt_;
   a.force = 0;
   a.report = 0;
+  a.decide = 0;
 #undef SK_TRY
   // the host vectors above are sources of asynchronous copies (the drain also covers errors)
   SK_HIP(hipStreamSynchronize(stream_), kStatusSyncFailed);
@@ -984,10 +989,12 @@ int EngineImpl<T>::enqueue_eval(int report_mode) {
 
 template <class T>
 int EngineImpl<T>::enqueue_block() {
-  const Args<T>& a = args_;
   hipStream_t s = stream_;
-  for (int k = 0; k < opt_.steps_per_block; ++k) {
-    if (fused_) {
+  const int B = opt_.steps_per_block;
+  const auto eval_after = [&](int k) { return (k + 1) % opt_.eval_stride == 0 || k + 1 == B; };
+  if (fused_) {
+    const Args<T>& a = args_;
+    for (int k = 0; k < B; ++k) {
       const int st = with_layout([&](auto cap, auto pack) -> int {
         constexpr int C = decltype(cap)::value;
         constexpr bool P = decltype(pack)::value;
@@ -995,17 +1002,57 @@ int EngineImpl<T>::enqueue_block() {
         return kStatusSuccess;
       });
       if (st) return st;
-    } else {
-      SK_LAUNCH(k_step_primal<T><<<n_tc_, kBlock, 0, s>>>(a));
-      int st = launch_spmv(1, a);
-      if (st) return st;
-      if ((st = launch_spmv(2, a))) return st;
-      SK_LAUNCH(k_step_finalize<T><<<K_, kBlock, 0, s>>>(a));
+      // an instance that became due for evaluation waits at most eval_stride steps
+      if (eval_after(k)) {
+        if (const int e = enqueue_eval(0)) return e;
+      }
     }
-    // an instance that became due for evaluation waits at most eval_stride steps
-    if ((k + 1) % opt_.eval_stride == 0 || k + 1 == opt_.steps_per_block) {
-      const int st = enqueue_eval(0);
-      if (st) return st;
+    return enqueue_eval(1);
+  }
+  // alone[k]: step k is decided by its own launch (k_step_finalize), which writes the records
+  // into buffer 0; that happens before every evaluation stage and at the end of the block.
+  // Every other step is decided by the next primal launch, which reads the records from one
+  // buffer and writes them to the other. The number of such switches is made even, so the
+  // block ends, and every replay of it starts, with the records in buffer 0. Step k's primal
+  // partials are in buffer k % 2, so the launch deciding step k never overwrites them.
+  std::vector<uint8_t> alone(static_cast<size_t>(B), 1);
+  int switches = 0;
+  if ((opt_.step_fusion & 2) == 0) {
+    for (int k = 0; k < B; ++k) alone[static_cast<size_t>(k)] = eval_after(k) ? 1 : 0;
+    for (int k = 0; k + 1 < B; ++k) switches += alone[static_cast<size_t>(k)] ? 0 : 1;
+    for (int k = B - 2; k >= 0 && switches % 2; --k) {
+      if (!alone[static_cast<size_t>(k)]) {
+        alone[static_cast<size_t>(k)] = 1;
+        --switches;
+      }
+    }
+  }
+  InstState* const state[2] = {args_.state, args_.state + K_};
+  double* const ps_c[2] = {args_.ps_c, args_.ps_c + n_tc_};
+  int cur = 0;  // the buffer that holds the records
+  for (int k = 0; k < B; ++k) {
+    Args<T> a = args_;
+    a.state = state[cur];
+    a.ps_c = ps_c[k & 1];
+    if (k > 0 && !alone[static_cast<size_t>(k - 1)]) {
+      a.decide = 1;
+      a.state_next = state[1 - cur];
+      a.ps_c_prev = ps_c[(k - 1) & 1];
+      cur = 1 - cur;
+    }
+    SK_LAUNCH(k_step_primal<T><<<n_tc_, kBlock, 0, s>>>(a));
+    a.decide = 0;
+    a.state = state[cur];
+    if (const int st = launch_spmv(1, a)) return st;
+    if (const int st = launch_spmv(2, a)) return st;
+    if (alone[static_cast<size_t>(k)]) {
+      a.state_next = state[0];
+      SK_LAUNCH(k_step_finalize<T><<<K_, kBlock, 0, s>>>(a));
+      cur = 0;
+      // an instance that became due for evaluation waits at most eval_stride steps
+      if (eval_after(k)) {
+        if (const int e = enqueue_eval(0)) return e;
+      }
     }
   }
   // the final evaluation of every instance that stopped in this block, so its answer can be

@@ -33,7 +33,9 @@ struct EngineOptions {
   int32_t tile_nnz = 0;           // row-block capacity; 0 picks per instance
   int32_t streams = 0;            // concurrent sub-batches: 0 automatic, 1 one stream, n >= 2
   int32_t column_order = 0;       // 0 automatic, 1 most-referenced columns first, 2 the LP's order
-  int32_t step_fusion = 0;        // 0 automatic (one launch per step for small instances), 1 never
+  // bit 0: never run a step as one launch (k_step_fused, otherwise automatic for small
+  // instances); bit 1: never let the primal launch make the previous step's decision
+  int32_t step_fusion = 0;
 };
 
 // A warm start held by an engine until its next upload (Engine::warm_start).

@@ -4,7 +4,10 @@
 //   k_step_primal  columns:   x' = proj(x - tau (c + A^T y)), xbar = 2x' - x       (:392-400)
 //   k_step_dual    rows of A: y' = prox(y + sigma A xbar)                           (:402-408)
 //   k_step_aty     rows of A^T: A^T y', interaction (x'-x)^T (A^T y' - A^T y)      (:410-438)
-//   k_step_finalize one workgroup per instance: the adaptive step-size decision      (:440-503)
+// and the adaptive step-size decision (:440-503), which the next step's primal launch makes
+// first (every chunk of an instance makes it, the first one records it), or, before an
+// evaluation stage and at the end of a captured block, its own launch:
+//   k_step_finalize one workgroup per instance
 // The carried A^T y replaces the reference's third product A (x'-x) (pdhg.cpp:432):
 // (y'-y)^T A (x'-x) = (x'-x)^T (A^T y' - A^T y), and A^T y' is the next primal step's input.
 //
@@ -116,13 +119,18 @@ struct Args {
   const Tile* __restrict__ tt;  // row blocks of CSR(A^T)
   const InstDesc* __restrict__ desc;
   InstState* state;
+  // where a decision writes the instance records: the primal launch that decides the previous
+  // step (decide = 1) reads `state` and writes here; the decision launch (k_step_finalize)
+  // reads `state` and writes here (the same buffer or the other one of the pair)
+  InstState* state_next;
   // per-tile flags (kFlag*) of ta, tt, tc and tr
   uint32_t* fa;
   uint32_t* ft;
   uint32_t* fc;
   uint32_t* fr;
   // per-tile partials
-  double* ps_c;  // step: ||x'-x||^2 per column chunk
+  double* ps_c;  // step: ||x'-x||^2 per column chunk (this step's primal launch writes it)
+  const double* ps_c_prev;  // the previous step's, read by a deciding primal launch
   double* ps_a;  // step: ||y'-y||^2 per CSR(A) block
   double* ps_t;  // step: interaction per CSR(A^T) block
   double* pe_c;  // eval: 2 per column chunk
@@ -135,6 +143,7 @@ struct Args {
   int32_t n_tt;    // row blocks of CSR(A^T)
   int32_t force;   // timing mode: every instance steps, nothing is decided
   int32_t report;  // evaluation kernels: 0 in-loop candidates, 1 final report of best
+  int32_t decide;  // primal launch: first make the previous step's decision (see k_step_primal)
 };
 
 __device__ __forceinline__ bool stepping(const InstState& s) { return !s.done && !s.eval_pending; }
@@ -163,39 +172,24 @@ __global__ __launch_bounds__(kBlock) void k_publish_flags(Args<T> a) {
 
 // ---- step kernels ---------------------------------------------------------------------------
 
-// The primal step of column chunk `tile` (pdhg.cpp:393-400) and its ||x'-x||^2 partial.
+// Columns per lane of a column chunk: lane l of the workgroup takes columns
+// t.begin + u * kBlock + l, u in [0, kChunkCols).
+constexpr int kChunkCols = kChunk / kBlock;
+
+// The primal update of column chunk t from operands already loaded (pdhg.cpp:393-400): x' and
+// x-bar from buffer p's iterate, the running sum when the last step was accepted, and the
+// chunk's ||x'-x||^2 partial.
 template <class T>
-__device__ __forceinline__ void primal_chunk(const Args<T>& a, const int tile, double* red) {
-  const Tile t = a.tc[tile];
-  const uint32_t f = a.fc[tile];
-  if (!a.force && !(f & kFlagStepping)) return;
-  const int p = (f & kFlagParity) ? 1 : 0;
-  const bool acc = a.force || (f & kFlagAccept);
-  const InstState& s = a.state[t.inst];
-  const double eta = s.eta, omega = s.omega;
-  const T* __restrict__ x = p ? a.x1 : a.x0;
+__device__ __forceinline__ void primal_update(const Args<T>& a, const Tile& t, const int tile,
+                                              const int p, const bool acc, const T tau,
+                                              const T (&xj)[kChunkCols], const T (&c)[kChunkCols],
+                                              const T (&at)[kChunkCols], const T (&lo)[kChunkCols],
+                                              const T (&hi)[kChunkCols], const T (&sm)[kChunkCols],
+                                              double* red) {
   T* __restrict__ xn = p ? a.x0 : a.x1;
-  const T* __restrict__ aty = p ? a.aty1 : a.aty0;
-  // every operand load is issued before the step size (which waits for the instance record)
-  // is used
-  constexpr int U = kChunk / kBlock;
-  T xj[U], c[U], at[U], lo[U], hi[U], sm[U];
-#pragma unroll
-  for (int u = 0; u < U; ++u) {
-    const int j = t.begin + u * kBlock + threadIdx.x;
-    if (j < t.end) {
-      xj[u] = x[j];
-      c[u] = a.chat[j];
-      at[u] = aty[j];
-      lo[u] = a.lhat[j];
-      hi[u] = a.uhat[j];
-      if (acc) sm[u] = a.xsum[j];
-    }
-  }
-  const T tau = static_cast<T>(eta / omega);
   double d2 = 0.0;
 #pragma unroll
-  for (int u = 0; u < U; ++u) {
+  for (int u = 0; u < kChunkCols; ++u) {
     const int j = t.begin + u * kBlock + threadIdx.x;
     if (j >= t.end) continue;
     if (acc) a.xsum[j] = sm[u] + xj[u];
@@ -210,10 +204,36 @@ __device__ __forceinline__ void primal_chunk(const Args<T>& a, const int tile, d
   if (threadIdx.x == 0) a.ps_c[tile] = d2;
 }
 
+// The primal step of column chunk `tile` (pdhg.cpp:393-400) and its ||x'-x||^2 partial, with
+// the step control the last decision published (the chunk's flags and the instance record).
 template <class T>
-__global__ __launch_bounds__(kBlock) void k_step_primal(Args<T> a) {
-  __shared__ double red[kBlock / kWave];
-  primal_chunk(a, xcd_tile(blockIdx.x, gridDim.x), red);
+__device__ __forceinline__ void primal_chunk(const Args<T>& a, const int tile, double* red) {
+  const Tile t = a.tc[tile];
+  const uint32_t f = a.fc[tile];
+  if (!a.force && !(f & kFlagStepping)) return;
+  const int p = (f & kFlagParity) ? 1 : 0;
+  const bool acc = a.force || (f & kFlagAccept);
+  const InstState& s = a.state[t.inst];
+  const double eta = s.eta, omega = s.omega;
+  const T* __restrict__ x = p ? a.x1 : a.x0;
+  const T* __restrict__ aty = p ? a.aty1 : a.aty0;
+  // every operand load is issued before the step size (which waits for the instance record)
+  // is used
+  constexpr int U = kChunkCols;
+  T xj[U], c[U], at[U], lo[U], hi[U], sm[U];
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const int j = t.begin + u * kBlock + threadIdx.x;
+    if (j < t.end) {
+      xj[u] = x[j];
+      c[u] = a.chat[j];
+      at[u] = aty[j];
+      lo[u] = a.lhat[j];
+      hi[u] = a.uhat[j];
+      if (acc) sm[u] = a.xsum[j];
+    }
+  }
+  primal_update(a, t, tile, p, acc, static_cast<T>(eta / omega), xj, c, at, lo, hi, sm, red);
 }
 
 // Gather index of staging position q: a packed word holds (local index << 12 | CSR slot),
@@ -413,11 +433,13 @@ __device__ __forceinline__ double tile_sum(const double* part, int base, int cou
   return block_sum(v, red);
 }
 
+// `done_count` is null in a workgroup that decides only for its own use (a deciding primal
+// chunk other than the instance's first): the instance is counted once, by the writer.
 __device__ __forceinline__ void finish(InstState& s, int status, uint32_t* done_count) {
   s.done = 1;
   s.status = status;
   s.report_pending = 1;
-  atomicAdd(done_count, 1u);
+  if (done_count) atomicAdd(done_count, 1u);
 }
 
 // The growth and shrink factors of the step-size rule at the instance's iteration count
@@ -503,63 +525,153 @@ __global__ __launch_bounds__(kStepLanes) void k_step_aty(Args<T> a) {
   aty_tile<T, PACK>(a, xcd_tile(blockIdx.x, gridDim.x), stage, red);
 }
 
-// One workgroup per instance: reduce the step partials in tile order and decide the step.
-// A separate launch (rather than a last-arriving tile) makes the partials visible across
-// XCDs through the kernel boundary, with no per-tile device-scope fence.
-// `red` holds 3 * (kBlock / kWave) values, `rule` 2; `flags` is shared by the workgroup.
-template <class T>
-__device__ __forceinline__ void finalize_instance(const Args<T>& a, const int inst, double* red,
-                                                  double* rule, uint32_t* flags) {
-  // the instance record, its tile ranges and the tiles' partials are loaded together (the
-  // partials of a stopped instance are read and dropped), so the chain is one round trip
-  InstState& s = a.state[inst];
-  const InstDesc d = a.desc[inst];
-  const bool go = stepping(s);
-  // ||x'-x||^2, ||y'-y||^2 and the interaction: lane sums over the tiles, then one reduction
-  double v[3] = {0.0, 0.0, 0.0};
-  for (int k = threadIdx.x; k < d.tc1 - d.tc0; k += kBlock) v[0] += a.ps_c[d.tc0 + k];
-  for (int k = threadIdx.x; k < d.ta1 - d.ta0; k += kBlock) v[1] += a.ps_a[d.ta0 + k];
-  for (int k = threadIdx.x; k < d.tt1 - d.tt0; k += kBlock) v[2] += a.ps_t[d.tt0 + k];
-  if (!go) return;
-  // the rule's powers in the second wavefront, beside the reduction
-  if (threadIdx.x == kWave) step_rule_factors(s, &rule[0], &rule[1]);
-  block_reduce_n<3, 0u>(v, red);
-  if (threadIdx.x == 0) {
-    decide_step(s, v[0], v[1], v[2], d.rows > 0, rule[0], rule[1], a.done_count);
-    *flags = step_flags(s);
-  }
-  __syncthreads();
-  publish_flags(a, d, *flags);
+// Copies an instance record between global memory and a workgroup's LDS copy, in 8-byte
+// words over the lanes of the workgroup.
+__device__ __forceinline__ void copy_state(InstState* dst, const InstState* src) {
+  static_assert(sizeof(InstState) % sizeof(uint64_t) == 0, "InstState is copied in 8-byte words");
+  constexpr int W = static_cast<int>(sizeof(InstState) / sizeof(uint64_t));
+  const uint64_t* s = reinterpret_cast<const uint64_t*>(src);
+  uint64_t* d = reinterpret_cast<uint64_t*>(dst);
+  for (int k = threadIdx.x; k < W; k += kBlock) d[k] = s[k];
 }
 
+// The step decision of instance `inst` (pdhg.cpp:410-503), made on the workgroup's copy `sm`
+// of its record in[inst]: the step's partials (ps_c of the primal chunks, a.ps_a of the tiles
+// of A, a.ps_t of the tiles of A^T) are reduced in tile order and thread 0 applies decide_step.
+// With `out`, the copy is then written to out[inst] (out may be in) and, if the instance
+// stepped, its tiles' flags are published and a stop is counted in done_count. Without `out`
+// nothing is written: every workgroup that decides from the same inputs gets the same bits, so
+// the chunks of a deciding primal launch agree without exchanging anything.
+// The partials are visible because they were written by earlier launches (or, in
+// k_step_fused, by this workgroup before a barrier); no device-scope fence is needed.
+// `red` holds 3 * (kBlock / kWave) values, `rule` 2; sm is shared by the workgroup.
+template <class T>
+__device__ __forceinline__ void decide_instance(const Args<T>& a, const int inst,
+                                                const InstDesc& d, const InstState* in,
+                                                InstState* out, const double* ps_c,
+                                                InstState* sm, double* red, double* rule) {
+  // the record and the tiles' partials are loaded together (the partials of an instance that
+  // did not step are read and dropped)
+  copy_state(sm, in + inst);
+  // ||x'-x||^2, ||y'-y||^2 and the interaction: lane sums over the tiles, then one reduction
+  double v[3] = {0.0, 0.0, 0.0};
+  for (int k = threadIdx.x; k < d.tc1 - d.tc0; k += kBlock) v[0] += ps_c[d.tc0 + k];
+  for (int k = threadIdx.x; k < d.ta1 - d.ta0; k += kBlock) v[1] += a.ps_a[d.ta0 + k];
+  for (int k = threadIdx.x; k < d.tt1 - d.tt0; k += kBlock) v[2] += a.ps_t[d.tt0 + k];
+  __syncthreads();
+  const bool go = stepping(*sm);
+  if (go) {
+    // the rule's powers in the second wavefront, beside the reduction
+    if (threadIdx.x == kWave) step_rule_factors(*sm, &rule[0], &rule[1]);
+    block_reduce_n<3, 0u>(v, red);
+    if (threadIdx.x == 0)
+      decide_step(*sm, v[0], v[1], v[2], d.rows > 0, rule[0], rule[1], out ? a.done_count : nullptr);
+    __syncthreads();
+  }
+  if (out && (go || out != in)) {
+    copy_state(out + inst, sm);
+    if (go) publish_flags(a, d, step_flags(*sm));
+  }
+}
+
+// One workgroup per instance: the decision of the step that just ran, from the records in
+// a.state into a.state_next. A separate launch (rather than a last-arriving tile) makes the
+// partials visible across XCDs through the kernel boundary, with no per-tile device-scope
+// fence. The host runs it before every evaluation stage and at the end of a captured block;
+// the other steps are decided by the next primal launch (k_step_primal with decide = 1).
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_step_finalize(Args<T> a) {
   __shared__ double red[3 * (kBlock / kWave)];
   __shared__ double rule[2];
-  __shared__ uint32_t flags;
-  finalize_instance(a, blockIdx.x, red, rule, &flags);
+  __shared__ InstState sm;
+  const int inst = static_cast<int>(blockIdx.x);
+  decide_instance(a, inst, a.desc[inst], a.state, a.state_next, a.ps_c, &sm, red, rule);
+}
+
+// The primal step of column chunk `tile` after deciding the previous step itself: every chunk
+// of an instance makes the decision from the same records (a.state) and partials (ps_c_prev,
+// ps_a, ps_t), and the instance's first chunk writes the decided records to a.state_next and
+// publishes the flags, which the dual and A^T launches of this step read. This saves the
+// decision's own launch and its wait on most steps; the arithmetic, and so every answer, is
+// bit for bit that of the decision launch. The operands that do not depend on the decision,
+// and x and A^T y of the buffer an accepted step makes current (almost every step is
+// accepted), are loaded before it; a rejected step reloads them from the other buffer.
+template <class T>
+__device__ __forceinline__ void primal_chunk_decided(const Args<T>& a, const int tile,
+                                                     double* red, double* rule, InstState* sm) {
+  const Tile t = a.tc[tile];
+  const InstDesc d = a.desc[t.inst];
+  const int p_old = a.state[t.inst].parity;
+  const T* __restrict__ xs = p_old ? a.x0 : a.x1;
+  const T* __restrict__ atys = p_old ? a.aty0 : a.aty1;
+  constexpr int U = kChunkCols;
+  T xj[U], c[U], at[U], lo[U], hi[U], sum[U];
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const int j = t.begin + u * kBlock + threadIdx.x;
+    if (j < t.end) {
+      xj[u] = xs[j];
+      c[u] = a.chat[j];
+      at[u] = atys[j];
+      lo[u] = a.lhat[j];
+      hi[u] = a.uhat[j];
+      sum[u] = a.xsum[j];
+    }
+  }
+  decide_instance(a, t.inst, d, a.state, tile == d.tc0 ? a.state_next : nullptr, a.ps_c_prev,
+                  sm, red, rule);
+  const InstState& s = *sm;
+  if (!stepping(s)) return;
+  const int p = s.parity;
+  if (p == p_old) {
+    const T* __restrict__ x = p ? a.x1 : a.x0;
+    const T* __restrict__ aty = p ? a.aty1 : a.aty0;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int j = t.begin + u * kBlock + threadIdx.x;
+      if (j < t.end) {
+        xj[u] = x[j];
+        at[u] = aty[j];
+      }
+    }
+  }
+  primal_update(a, t, tile, p, s.pending_accept != 0, static_cast<T>(s.eta / s.omega), xj, c, at,
+                lo, hi, sum, red);
+}
+
+// The primal step launch: decide = 0 steps with the published decision (primal_chunk),
+// decide = 1 first makes the previous step's decision (primal_chunk_decided).
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_step_primal(Args<T> a) {
+  __shared__ double red[3 * (kBlock / kWave)];
+  __shared__ double rule[2];
+  __shared__ InstState sm;
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  if (a.decide) primal_chunk_decided(a, tile, red, rule, &sm);
+  else primal_chunk(a, tile, red);
 }
 
 // One workgroup per instance runs a whole step: the primal step of its column chunks, then
 // A x-bar and the dual step on its tiles of A, then A^T y' on its tiles of A^T, then the
-// decision, each with the per-tile code of the four step kernels above and tile after tile in
+// decision, each with the per-tile code of the step kernels above and tile after tile in
 // index order, with a barrier between the phases. The partials and so the answers are bit for
-// bit those of the four launches; for instances with only a few tiles the step is one launch
-// instead of four.
+// bit those of the separate launches; for instances with only a few tiles the step is one
+// launch instead of three or four.
 template <class T, int CAP, bool PACK>
 __global__ __launch_bounds__(kBlock) void k_step_fused(Args<T> a) {
   __shared__ T stage[CAP];
   __shared__ double red[3 * (kBlock / kWave)];
   __shared__ double rule[2];
-  __shared__ uint32_t flags;
-  const InstDesc d = a.desc[blockIdx.x];
+  __shared__ InstState sm;
+  const int inst = static_cast<int>(blockIdx.x);
+  const InstDesc d = a.desc[inst];
   for (int k = d.tc0; k < d.tc1; ++k) primal_chunk(a, k, red);
   __syncthreads();
   for (int k = d.ta0; k < d.ta1; ++k) dual_tile<T, PACK>(a, k, stage, red);
   __syncthreads();
   for (int k = d.tt0; k < d.tt1; ++k) aty_tile<T, PACK>(a, k, stage, red);
   __syncthreads();
-  finalize_instance(a, blockIdx.x, red, rule, &flags);
+  decide_instance(a, inst, d, a.state, a.state, a.ps_c, &sm, red, rule);
 }
 
 // ---- evaluation kernels (FP64) ---------------------------------------------------------------

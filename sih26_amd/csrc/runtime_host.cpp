@@ -47,7 +47,8 @@ int64_t fault_take_nan_instance() { return g_nan_instance.exchange(-1); }
 //   column:  pointer, cost, bounds, scaling, 3 scaled vectors + FP64 bounds, iterates (x, x',
 //            A^T y twice, xbar, restart point), sums, candidates, best, reduced costs, power
 //            iteration vectors
-//   instance: state record, descriptor and minimum tile entries
+//   instance: two state records (a decision reads one and writes the other), descriptor and
+//            minimum tile entries
 // plus a fixed allowance for the allocation granularity of some 60 buffers.
 int64_t estimate_device_bytes(int64_t rows, int64_t cols, int64_t nnz, int64_t instances,
                               int32_t precision) {
@@ -57,7 +58,7 @@ int64_t estimate_device_bytes(int64_t rows, int64_t cols, int64_t nnz, int64_t i
   const double per_row = 84.0 + 6.0 * w + 0.5;
   const double per_col = 116.0 + 10.0 * w + 0.5;
   const double bytes = per_nnz * static_cast<double>(nnz) + per_row * static_cast<double>(rows) +
-                       per_col * static_cast<double>(cols) + 1024.0 * static_cast<double>(instances) +
+                       per_col * static_cast<double>(cols) + 1664.0 * static_cast<double>(instances) +
                        262144.0;
   return static_cast<int64_t>(std::ceil(bytes));
 }

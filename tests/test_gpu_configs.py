@@ -144,7 +144,7 @@ def test_admission_after_a_row_heavy_batch_does_not_stall(gpu):
                                               num_equality_rows=600, degree_kind=0, degree_lo=20))
 
     def est(m, n, z):  # estimate_device_bytes of engine.hip, FP32 iterates, one instance
-        return int(np.ceil(77.1 * z + 108.5 * m + 156.5 * n + 1024 + 262144))
+        return int(np.ceil(77.1 * z + 108.5 * m + 156.5 * n + 1664 + 262144))
 
     a, b = row_heavy.dims, nnz_heavy.dims
     alone = max(est(*a), est(*b))

@@ -319,8 +319,10 @@ def test_autotuned_batch_runs_with_the_capacity_it_picked(gpu, oracle_mod):
 @pytest.mark.parametrize("precision", ["fp32", "fp64"])
 def test_fused_step_equals_four_launches(gpu, precision):
     """Instances with a few tiles run a whole step in one launch (k_step_fused) with the
-    per-tile code of the four step kernels: the answers are bit for bit those of the four
-    launches, alone and as a batch of small instances."""
+    per-tile code of the separate step kernels: the answers are bit for bit those of the
+    separate launches (with the decision folded into the next primal launch, step_fusion = 1,
+    or in a launch of its own every step, step_fusion = 3), alone and as a batch of small
+    instances."""
     kw = dict(pdhg_tolerance=1e-4, pdhg_stop_at_request=True, pdhg_absolute_primal=False,
               pdhg_precision=precision)
     groups = [[gpu.Lp.generate(1, 1)],
@@ -328,17 +330,53 @@ def test_fused_step_equals_four_launches(gpu, precision):
               [to_lp(gpu, closed_form(name)[0]) for name in ("sum", "range", "maximise")]]
     for lps in groups:
         runs = {}
-        for fusion in (0, 1):
+        for fusion in (0, 1, 3):
             b = gpu.Batch(step_fusion=fusion, **kw)
             for lp in lps:
                 b.add(lp)
             b.upload()
             b.solve()
             runs[fusion] = (b.stats()["fused"], [b.result(k) for k in range(len(lps))])
-        assert runs[0][0] == 1 and runs[1][0] == 0
-        for fused, plain in zip(runs[0][1], runs[1][1]):
-            assert fused.converged and fused.iterations == plain.iterations
-            assert fused.status == plain.status
-            np.testing.assert_array_equal(fused.col_value, plain.col_value)
-            np.testing.assert_array_equal(fused.row_dual, plain.row_dual)
-            np.testing.assert_array_equal(fused.col_dual, plain.col_dual)
+        assert runs[0][0] == 1 and runs[1][0] == 0 and runs[3][0] == 0
+        for other in (1, 3):
+            for fused, plain in zip(runs[0][1], runs[other][1]):
+                assert fused.converged and fused.iterations == plain.iterations
+                assert fused.status == plain.status and fused.attempts == plain.attempts
+                np.testing.assert_array_equal(fused.col_value, plain.col_value)
+                np.testing.assert_array_equal(fused.row_dual, plain.row_dual)
+                np.testing.assert_array_equal(fused.col_dual, plain.col_dual)
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+@pytest.mark.parametrize("block,stride", [(40, 20), (8, 3)])
+def test_decisions_in_the_primal_launch_equal_their_own_launch(gpu, precision, block, stride):
+    """Most steps are decided by the next step's primal launch: every column chunk of an
+    instance decides from the same records and partials, and the first chunk records the
+    decision in the other buffer of the record pair. Iterations, attempts (rejected steps
+    included, which reload the iterate the speculative loads skipped), restarts, statuses
+    and answers are bit for bit those of deciding every step in a launch of its own
+    (step_fusion = 2), for instances of many column chunks. The 8-step block with an
+    evaluation every 3 steps has an odd number of such decisions, which the schedule evens
+    out by deciding one more step in its own launch."""
+    kw = dict(pdhg_tolerance=1e-4, pdhg_stop_at_request=True, pdhg_absolute_primal=False,
+              pdhg_precision=precision, steps_per_block=block, eval_stride=stride)
+    lps = [gpu.Lp.generate(3, k) for k in range(6)] + [gpu.Lp.generate(1, 1)]
+    runs = {}
+    for fusion in (0, 2):
+        b = gpu.Batch(step_fusion=fusion, **kw)
+        for lp in lps:
+            b.add(lp)
+        b.upload()
+        b.solve()
+        assert b.stats()["fused"] == 0
+        runs[fusion] = [b.result(k) for k in range(len(lps))]
+    rejected = 0
+    for folded, alone in zip(runs[0], runs[2]):
+        assert folded.converged and folded.iterations == alone.iterations
+        assert folded.attempts == alone.attempts and folded.restarts == alone.restarts
+        assert folded.status == alone.status and folded.evaluations == alone.evaluations
+        np.testing.assert_array_equal(folded.col_value, alone.col_value)
+        np.testing.assert_array_equal(folded.row_dual, alone.row_dual)
+        np.testing.assert_array_equal(folded.col_dual, alone.col_dual)
+        rejected += folded.attempts - folded.iterations
+    assert rejected > 0, "no rejected step: the reload path was not exercised"
