@@ -103,10 +103,12 @@ void build_row_tiles(const std::vector<int32_t>& ptr, int32_t row0, int32_t rows
   if (out->size() == before) emit(row0, row0, kTileShort);
 }
 
-// Step fusion (k_step_fused) applies when every instance has at most this many SpMV tiles
-// (A and A^T together) and column chunks.
+// The resident solve (k_resident) applies when every instance has at most this many SpMV
+// tiles (A and A^T together) and column chunks; a launch runs up to kResidentRounds
+// steps_per_block rounds of each instance (steps or evaluation stages).
 constexpr int32_t kFusedMaxTiles = 4;
 constexpr int32_t kFusedMaxChunks = 2;
+constexpr int32_t kResidentRounds = 4;
 
 // Column order on the device. For an instance of at least kRelabelColumns columns (mode 0) or
 // any instance (mode 1), its kHotBytes / w columns with the most nonzeros (the x-bar entries
@@ -326,7 +328,7 @@ class EngineImpl final : public Engine {
   int32_t K_ = 0;
   int stage_ = 1;
   int32_t cap_ = kDefaultTileNnz;  // row-block tile capacity (nonzeros)
-  bool fused_ = false;  // one launch per step (k_step_fused): every instance has few tiles
+  bool fused_ = false;  // the resident solve (k_resident): every instance has few tiles
   bool pack_ = true;    // packed index words in the staging order
   int32_t cus_ = 256;   // compute units of the device
 
@@ -993,21 +995,14 @@ int EngineImpl<T>::enqueue_block() {
   const int B = opt_.steps_per_block;
   const auto eval_after = [&](int k) { return (k + 1) % opt_.eval_stride == 0 || k + 1 == B; };
   if (fused_) {
-    const Args<T>& a = args_;
-    for (int k = 0; k < B; ++k) {
-      const int st = with_layout([&](auto cap, auto pack) -> int {
-        constexpr int C = decltype(cap)::value;
-        constexpr bool P = decltype(pack)::value;
-        SK_LAUNCH(k_step_fused<T, C, P><<<K_, kBlock, 0, s>>>(a));
-        return kStatusSuccess;
-      });
-      if (st) return st;
-      // an instance that became due for evaluation waits at most eval_stride steps
-      if (eval_after(k)) {
-        if (const int e = enqueue_eval(0)) return e;
-      }
-    }
-    return enqueue_eval(1);
+    // one launch: each instance's steps, evaluations and final evaluation (k_resident)
+    const int32_t rounds = kResidentRounds * B;
+    return with_layout([&](auto cap, auto pack) -> int {
+      constexpr int C = decltype(cap)::value;
+      constexpr bool P = decltype(pack)::value;
+      SK_LAUNCH(k_resident<T, C, P><<<K_, kBlock, 0, s>>>(args_, rounds));
+      return kStatusSuccess;
+    });
   }
   // alone[k]: step k is decided by its own launch (k_step_finalize), which writes the records
   // into buffer 0; that happens before every evaluation stage and at the end of the block.

@@ -33,8 +33,8 @@ struct EngineOptions {
   int32_t tile_nnz = 0;           // row-block capacity; 0 picks per instance
   int32_t streams = 0;            // concurrent sub-batches: 0 automatic, 1 one stream, n >= 2
   int32_t column_order = 0;       // 0 automatic, 1 most-referenced columns first, 2 the LP's order
-  // bit 0: never run a step as one launch (k_step_fused, otherwise automatic for small
-  // instances); bit 1: never let the primal launch make the previous step's decision
+  // bit 0: never run the resident solve (k_resident, otherwise automatic when every instance
+  // is small); bit 1: never let the primal launch make the previous step's decision
   int32_t step_fusion = 0;
 };
 
@@ -109,7 +109,8 @@ class Engine {
   virtual int stage_choice() const = 0;
   // Sub-batches the batch runs as (each on its own stream).
   virtual int32_t streams() const = 0;
-  // whether the uploaded batch runs a PDHG step as one launch (k_step_fused)
+  // whether the uploaded batch runs as the resident solve (k_resident): one launch for many
+  // steps and evaluations of every instance
   virtual bool fused() const = 0;
 
   // Probes for tests, valid once a solve (or start()) has run the device set-up. They return

@@ -275,7 +275,11 @@ __device__ __forceinline__ void load4(const uint16_t* __restrict__ p, int v[4]) 
 // tile's base (the first gather index of its instance, which exists because the tile has a
 // nonzero) and its product is dropped, so no gather waits in a branch of its own for the one
 // before it.
-template <int NT, class A, bool PACK, class V, class F>
+// IN_FLIGHT: groups each lane loads before it gathers (0: the measured best for a launch of
+// many workgroups, one for a 256-lane workgroup and four for a wavefront); a workgroup that
+// runs alone (k_resident) issues a whole tile's groups at once. Only the order of the loads
+// changes, not what is staged.
+template <int NT, class A, bool PACK, class V, class F, int IN_FLIGHT = 0>
 struct GroupStager {
   const int32_t* __restrict__ sidx;
   const uint16_t* __restrict__ pos;
@@ -285,7 +289,7 @@ struct GroupStager {
     const int p0 = t.p0, p1 = t.p1, qa = p0 & ~3;
     if (p1 <= p0) return;
     const int groups = (p1 - qa + 3) >> 2;
-    constexpr int U = NT >= kBlock ? 1 : 4;  // groups per lane in flight (measured best)
+    constexpr int U = IN_FLIGHT > 0 ? IN_FLIGHT : (NT >= kBlock ? 1 : 4);
     for (int g0 = 0; g0 < groups; g0 += U * NT) {
       int w[U][4], sl[U][4];
       V v[U][4];
@@ -330,11 +334,11 @@ struct GroupStager {
   }
 };
 // stages val[q] * x[c] (products in A) with NT lanes
-template <int NT, class A, bool PACK, class V, class X>
+template <int NT, class A, bool PACK, class V, class X, int IN_FLIGHT = 0>
 __device__ __forceinline__ auto gather_stager(const int32_t* sidx, const uint16_t* pos,
                                               const V* val, const X* x) {
   auto f = [x](int c, int, V v) { return static_cast<A>(v) * static_cast<A>(x[c]); };
-  return GroupStager<NT, A, PACK, V, decltype(f)>{sidx, pos, val, f};
+  return GroupStager<NT, A, PACK, V, decltype(f), IN_FLIGHT>{sidx, pos, val, f};
 }
 
 // CAP: the tile capacity the tables were built for, which sizes the LDS staging buffer (and
@@ -349,7 +353,7 @@ constexpr int kStepRows = kMaxTileRows / kStepLanes;
 // and each row reduced in its fixed order (rowblock_product; pre and epi as there).
 // k_step_dual runs it on CSR(A) with in = x-bar, k_step_aty on CSR(A^T) with in = y', and
 // k_probe_product on either table with a caller's vector.
-template <class T, bool PACK, class Pre, class Epi>
+template <class T, bool PACK, int IN_FLIGHT = 0, class Pre, class Epi>
 __device__ __forceinline__ void step_product(const Tile& t, const int32_t* __restrict__ ptr,
                                              const int32_t* __restrict__ sidx,
                                              const uint16_t* __restrict__ pos,
@@ -357,12 +361,12 @@ __device__ __forceinline__ void step_product(const Tile& t, const int32_t* __res
                                              Pre pre, Epi epi, T* stage) {
   rowblock_product<kStepLanes, T>(
       t, ptr, [&](int q) { return val[q] * in[gather_index<PACK>(t, sidx, q)]; },
-      gather_stager<kStepLanes, T, PACK>(sidx, pos, val, in), pre, epi, stage);
+      gather_stager<kStepLanes, T, PACK, T, T, IN_FLIGHT>(sidx, pos, val, in), pre, epi, stage);
 }
 
 // A x-bar and the dual step on the rows of tile `tile` of A (pdhg.cpp:402-408), with its
 // ||y'-y||^2 partial.
-template <class T, bool PACK>
+template <class T, bool PACK, int IN_FLIGHT = 0>
 __device__ __forceinline__ void dual_tile(const Args<T>& a, const int tile, T* stage, double* red) {
   const Tile t = a.ta[tile];
   const uint32_t f = a.fa[tile];
@@ -377,7 +381,7 @@ __device__ __forceinline__ void dual_tile(const Args<T>& a, const int tile, T* s
   T* __restrict__ yn = p ? a.y0 : a.y1;
   double d2 = 0.0;
   T yi[kStepRows], lo[kStepRows], hi[kStepRows], sum[kStepRows];
-  step_product<T, PACK>(
+  step_product<T, PACK, IN_FLIGHT>(
       t, a.a_ptr, a.a_sidx, a.a_pos, a.a_val, a.xbar,
       [&](int i, int k) {
         yi[k] = y[i];
@@ -489,7 +493,7 @@ __device__ void decide_step(InstState& s, double dx2, double dy2, double inter_s
 }
 
 // A^T y' on the rows of tile `tile` of A^T and its share of the interaction (pdhg.cpp:410-438).
-template <class T, bool PACK>
+template <class T, bool PACK, int IN_FLIGHT = 0>
 __device__ __forceinline__ void aty_tile(const Args<T>& a, const int tile, T* stage, double* red) {
   const Tile t = a.tt[tile];
   const uint32_t f = a.ft[tile];
@@ -502,7 +506,7 @@ __device__ __forceinline__ void aty_tile(const Args<T>& a, const int tile, T* st
   T* __restrict__ atyn = p ? a.aty0 : a.aty1;
   double inter = 0.0;
   T xj[kStepRows], xnj[kStepRows], atyj[kStepRows];
-  step_product<T, PACK>(
+  step_product<T, PACK, IN_FLIGHT>(
       t, a.t_ptr, a.t_sidx, a.t_pos, a.t_val, yn,
       [&](int j, int k) {
         xj[k] = x[j];
@@ -543,7 +547,7 @@ __device__ __forceinline__ void copy_state(InstState* dst, const InstState* src)
 // nothing is written: every workgroup that decides from the same inputs gets the same bits, so
 // the chunks of a deciding primal launch agree without exchanging anything.
 // The partials are visible because they were written by earlier launches (or, in
-// k_step_fused, by this workgroup before a barrier); no device-scope fence is needed.
+// k_resident, by this workgroup before a barrier); no device-scope fence is needed.
 // `red` holds 3 * (kBlock / kWave) values, `rule` 2; sm is shared by the workgroup.
 template <class T>
 __device__ __forceinline__ void decide_instance(const Args<T>& a, const int inst,
@@ -651,40 +655,15 @@ __global__ __launch_bounds__(kBlock) void k_step_primal(Args<T> a) {
   else primal_chunk(a, tile, red);
 }
 
-// One workgroup per instance runs a whole step: the primal step of its column chunks, then
-// A x-bar and the dual step on its tiles of A, then A^T y' on its tiles of A^T, then the
-// decision, each with the per-tile code of the step kernels above and tile after tile in
-// index order, with a barrier between the phases. The partials and so the answers are bit for
-// bit those of the separate launches; for instances with only a few tiles the step is one
-// launch instead of three or four.
-template <class T, int CAP, bool PACK>
-__global__ __launch_bounds__(kBlock) void k_step_fused(Args<T> a) {
-  __shared__ T stage[CAP];
-  __shared__ double red[3 * (kBlock / kWave)];
-  __shared__ double rule[2];
-  __shared__ InstState sm;
-  const int inst = static_cast<int>(blockIdx.x);
-  const InstDesc d = a.desc[inst];
-  for (int k = d.tc0; k < d.tc1; ++k) primal_chunk(a, k, red);
-  __syncthreads();
-  for (int k = d.ta0; k < d.ta1; ++k) dual_tile<T, PACK>(a, k, stage, red);
-  __syncthreads();
-  for (int k = d.tt0; k < d.tt1; ++k) aty_tile<T, PACK>(a, k, stage, red);
-  __syncthreads();
-  decide_instance(a, inst, d, a.state, a.state, a.ps_c, &sm, red, rule);
-}
-
 // ---- evaluation kernels (FP64) ---------------------------------------------------------------
 
 // k_eval_prepare: for instances with a pending evaluation, fold the last accepted iterate into
 // the running sums and write the unscaled candidates (pdhg.cpp:357-366, 505-531), plus the
 // squared distances to the restart point used by the primal-weight update (pdhg.cpp:613-624).
+// eval_prepare_chunk does it for column chunk `tile` (is_col) or row chunk `tile`.
 template <class T>
-__global__ __launch_bounds__(kBlock) void k_eval_prepare(Args<T> a) {
-  __shared__ double red[kBlock / kWave];
-  const int b = xcd_tile(blockIdx.x, gridDim.x);
-  const bool is_col = b < a.n_tc;
-  const int tile = is_col ? b : b - a.n_tc;
+__device__ __forceinline__ void eval_prepare_chunk(const Args<T>& a, const bool is_col,
+                                                   const int tile, double* red) {
   if (!((is_col ? a.fc[tile] : a.fr[tile]) & kFlagEval)) return;
   const Tile t = is_col ? a.tc[tile] : a.tr[tile];
   const InstState& s = a.state[t.inst];
@@ -723,6 +702,14 @@ __global__ __launch_bounds__(kBlock) void k_eval_prepare(Args<T> a) {
   }
   d2 = block_sum(d2, red);
   if (threadIdx.x == 0) (is_col ? a.pe_c : a.pe_r)[tile * 2] = d2;
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_eval_prepare(Args<T> a) {
+  __shared__ double red[kBlock / kWave];
+  const int b = xcd_tile(blockIdx.x, gridDim.x);
+  const bool is_col = b < a.n_tc;
+  eval_prepare_chunk(a, is_col, is_col ? b : b - a.n_tc, red);
 }
 
 // eval_rows_tile: activity = A x for both candidates on the original A, and every row term of
@@ -952,14 +939,15 @@ __global__ __launch_bounds__(kBlock) void k_eval_products(Args<T> a, int rows) {
   else eval_cols_tile<T, PACK>(a, xcd_tile(b - rows, static_cast<int>(gridDim.x) - rows), stage, red);
 }
 
-// One workgroup per evaluated instance: reduce every evaluation partial in tile order, form
-// both Residuals and decide (or store the report).
+// For an evaluated instance (one workgroup): reduce every evaluation partial in tile order,
+// form both Residuals and decide (or store the report). `red` holds 18 * (kBlock / kWave)
+// values; `flags` is shared by the workgroup.
 template <class T>
-__global__ __launch_bounds__(kBlock) void k_eval_finalize(Args<T> a) {
-  __shared__ double red[18 * (kBlock / kWave)];
-  InstState& s = a.state[blockIdx.x];
+__device__ __forceinline__ void eval_finalize_instance(const Args<T>& a, const int inst,
+                                                       double* red, uint32_t* flags) {
+  InstState& s = a.state[inst];
   if (!(a.report ? s.report_pending : s.eval_pending)) return;
-  const InstDesc d = a.desc[blockIdx.x];
+  const InstDesc d = a.desc[inst];
   // per candidate c, v[8c + ...]: 0 viol^2, 1 support, 2 dual violation (rows), 3 compl (rows),
   // 4 dual violation (cols), 5 bound, 6 compl (cols), 7 primal objective; 16, 17 the squared
   // distances to the restart point. Lane sums (maxima) over the tiles, then one reduction.
@@ -991,7 +979,6 @@ __global__ __launch_bounds__(kBlock) void k_eval_finalize(Args<T> a) {
     for (int k = threadIdx.x; k < d.tr1 - d.tr0; k += kBlock) v[17] += a.pe_r[(size_t)(d.tr0 + k) * 2];
   }
   block_reduce_n<18, (1u << 3) | (1u << 6) | (1u << 11) | (1u << 14)>(v, red);
-  __shared__ uint32_t flags;
   if (threadIdx.x == 0) {
     Residuals r[2];
     for (int c = 0; c < 2; ++c) {
@@ -1008,19 +995,25 @@ __global__ __launch_bounds__(kBlock) void k_eval_finalize(Args<T> a) {
     } else {
       decide_evaluation(s, r[0], r[1], v[16], v[17], a.done_count);
     }
-    flags = step_flags(s);
+    *flags = step_flags(s);
   }
   __syncthreads();
-  publish_flags(a, d, flags);
+  publish_flags(a, d, *flags);
 }
 
-// k_eval_commit: apply the decisions of k_eval_finalize (pdhg.cpp:541-545, 592-594, 632-638).
-// Idempotent, so an instance that stopped may see it again without effect.
 template <class T>
-__global__ __launch_bounds__(kBlock) void k_eval_commit(Args<T> a) {
-  const int b = xcd_tile(blockIdx.x, gridDim.x);
-  const bool is_col = b < a.n_tc;
-  const int tile = is_col ? b : b - a.n_tc;
+__global__ __launch_bounds__(kBlock) void k_eval_finalize(Args<T> a) {
+  __shared__ double red[18 * (kBlock / kWave)];
+  __shared__ uint32_t flags;
+  eval_finalize_instance(a, static_cast<int>(blockIdx.x), red, &flags);
+}
+
+// eval_commit_chunk: apply the decisions of the evaluation finalize (pdhg.cpp:541-545, 592-594,
+// 632-638) to column chunk `tile` (is_col) or row chunk `tile`. Idempotent, so an instance
+// that stopped may see it again without effect.
+template <class T>
+__device__ __forceinline__ void eval_commit_chunk(const Args<T>& a, const bool is_col,
+                                                  const int tile) {
   if (!((is_col ? a.fc[tile] : a.fr[tile]) & kFlagCommit)) return;
   const Tile t = is_col ? a.tc[tile] : a.tr[tile];
   const InstState& s = a.state[t.inst];
@@ -1040,6 +1033,77 @@ __global__ __launch_bounds__(kBlock) void k_eval_commit(Args<T> a) {
       if constexpr (kSplitSum) tot[k] = 0.0;
       rst[k] = v[k];
     }
+  }
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_eval_commit(Args<T> a) {
+  const int b = xcd_tile(blockIdx.x, gridDim.x);
+  const bool is_col = b < a.n_tc;
+  eval_commit_chunk(a, is_col, is_col ? b : b - a.n_tc);
+}
+
+// ---- resident solve of small instances ------------------------------------------------------
+
+// One workgroup per instance runs up to `steps` rounds of its solve in one launch, each round a
+// PDHG step or, when one is pending, an evaluation stage; an instance that stops runs its final
+// (report) evaluation before the launch ends. Every phase is the per-tile code of the separate
+// launches above (primal chunks, tiles of A, tiles of A^T, the decision; evaluation: chunk
+// preparation, row and column tiles, finalize, commit), tile after tile in index order with a
+// barrier between phases, so the partials and every answer are bit for bit those of the
+// separate launches. It is used when every instance of a batch has only a few tiles: such a
+// step is a short chain of dependent memory round trips, and running all of them in one
+// launch keeps the instance's data in the caches instead of crossing a kernel boundary (and
+// its launch gap) four times per step. The step product stages its values in the first
+// CAP * sizeof(T) bytes of the evaluation's staging buffer.
+template <class T, int CAP, bool PACK>
+__global__ __launch_bounds__(kBlock) void k_resident(Args<T> a, int32_t steps) {
+  __shared__ D2 stage[CAP];
+  __shared__ double red[18 * (kBlock / kWave)];
+  __shared__ double rule[2];
+  __shared__ InstState sm;
+  __shared__ uint32_t flags;
+  T* const stage_t = reinterpret_cast<T*>(stage);
+  // a tile's groups of four staged nonzeros (CAP / 4, one more when the tile does not start
+  // on a multiple of four) all in flight at once
+  constexpr int kInFlight = (CAP / 4 + 1 + kBlock - 1) / kBlock;
+  const int inst = static_cast<int>(blockIdx.x);
+  const InstDesc d = a.desc[inst];
+  const InstState* rec = a.state + inst;
+  for (int round = 0; round < steps; ++round) {
+    // the record was last written by this workgroup's previous phase
+    __syncthreads();
+    const int32_t done = rec->done, evaluate = rec->eval_pending;
+    if (done) break;
+    if (evaluate) {
+      for (int k = d.tc0; k < d.tc1; ++k) eval_prepare_chunk(a, true, k, red);
+      for (int k = d.tr0; k < d.tr1; ++k) eval_prepare_chunk(a, false, k, red);
+      __syncthreads();
+      for (int k = d.ta0; k < d.ta1; ++k) eval_rows_tile<T, PACK>(a, k, stage, red);
+      for (int k = d.tt0; k < d.tt1; ++k) eval_cols_tile<T, PACK>(a, k, stage, red);
+      __syncthreads();
+      eval_finalize_instance(a, inst, red, &flags);
+      __syncthreads();
+      for (int k = d.tc0; k < d.tc1; ++k) eval_commit_chunk(a, true, k);
+      for (int k = d.tr0; k < d.tr1; ++k) eval_commit_chunk(a, false, k);
+      continue;
+    }
+    for (int k = d.tc0; k < d.tc1; ++k) primal_chunk(a, k, red);
+    __syncthreads();
+    for (int k = d.ta0; k < d.ta1; ++k) dual_tile<T, PACK, kInFlight>(a, k, stage_t, red);
+    __syncthreads();
+    for (int k = d.tt0; k < d.tt1; ++k) aty_tile<T, PACK, kInFlight>(a, k, stage_t, red);
+    __syncthreads();
+    decide_instance(a, inst, d, a.state, a.state, a.ps_c, &sm, red, rule);
+  }
+  __syncthreads();
+  if (rec->report_pending) {
+    Args<T> r = a;
+    r.report = 1;
+    for (int k = d.ta0; k < d.ta1; ++k) eval_rows_tile<T, PACK>(r, k, stage, red);
+    for (int k = d.tt0; k < d.tt1; ++k) eval_cols_tile<T, PACK>(r, k, stage, red);
+    __syncthreads();
+    eval_finalize_instance(r, inst, red, &flags);
   }
 }
 
