@@ -109,6 +109,9 @@ struct InstState {
   double best_worst;
   // a warm start's step size (0: the reference's 1 / ||A-hat||, pdhg.cpp:368)
   double eta_start;
+  // the step-size rule's shrink and grow factors at `iteration` (pdhg.cpp:472-474)
+  double rule_shrink;
+  double rule_grow;
   // termination options and problem norms (pdhg.cpp:285-315)
   double tolerance;
   double cost_norm;

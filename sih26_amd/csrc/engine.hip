@@ -30,6 +30,51 @@ namespace {
     }                                                                                   \
   } while (0)
 
+// The step-size rule's factors (pdhg.cpp:472-474) for iterations [0, kRuleIterations), which
+// covers the default limit of 10^6 iterations: at iteration k, with e = max(k + 1, 2),
+// shrink = 1 - e^-0.3 and grow = 1 + e^-0.6, computed with the host's libm pow as the
+// reference computes them. One table per process, copied once to each device and kept until
+// the process ends; a decision then loads the factors instead of evaluating two FP64 powers.
+constexpr int64_t kRuleIterations = (int64_t{1} << 20) + 1;
+
+const std::vector<double>& rule_table_host() {
+  static std::once_flag once;
+  static std::vector<double> table;
+  std::call_once(once, [] {
+    table.resize(2 * static_cast<size_t>(kRuleIterations));
+    for (int64_t k = 0; k < kRuleIterations; ++k) {
+      const double exponent = static_cast<double>(k + 1 > 2 ? k + 1 : 2);
+      table[2 * static_cast<size_t>(k)] = 1.0 - std::pow(exponent, -0.3);
+      table[2 * static_cast<size_t>(k) + 1] = 1.0 + std::pow(exponent, -0.6);
+    }
+  });
+  return table;
+}
+
+// The table on the current device (allocated and filled on first use).
+int rule_table_device(const double** out) {
+  static std::mutex mu;
+  static std::vector<double*> per_device;
+  int dev = 0;
+  SK_HIP(hipGetDevice(&dev), kStatusInitFailed);
+  const std::vector<double>& host = rule_table_host();
+  std::lock_guard<std::mutex> lk(mu);
+  if (per_device.size() <= static_cast<size_t>(dev)) per_device.resize(static_cast<size_t>(dev) + 1, nullptr);
+  double*& d = per_device[static_cast<size_t>(dev)];
+  if (!d) {
+    double* p = nullptr;
+    SK_HIP(hipMalloc(&p, host.size() * sizeof(double)), kStatusOutOfMemory);
+    if (hipMemcpy(p, host.data(), host.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
+      set_last_error("hipMemcpy of the step-size rule table failed");
+      (void)hipFree(p);
+      return kStatusCopyFailed;
+    }
+    d = p;
+  }
+  *out = d;
+  return kStatusSuccess;
+}
+
 #define SK_LAUNCH(...)                                                                  \
   do {                                                                                  \
     if (fault_launch_fails()) {                                                         \
@@ -337,6 +382,7 @@ class EngineImpl final : public Engine {
     std::vector<int32_t> a_ptr, a_idx, t_ptr, t_idx, next, dev_of;
     std::vector<double> a_orig, t_orig, cmin, cl, cu, rl, ru, draw;
     std::vector<Tile> ta, tt, tc, tr;
+    std::vector<InstDesc> cdesc;
   };
   HostStage stage_host_;
   // host copies needed after upload
@@ -362,7 +408,7 @@ class EngineImpl final : public Engine {
   DevBuf b_chat_, b_lhat_, b_uhat_, b_rlhat_, b_ruhat_, b_lhat64_, b_uhat64_;
   DevBuf b_x0_, b_x1_, b_y0_, b_y1_, b_aty0_, b_aty1_, b_xbar_, b_xrst_, b_yrst_;
   DevBuf b_xsum_, b_ysum_, b_xtot_, b_ytot_, b_xu0_, b_xu1_, b_yu0_, b_yu1_, b_bestx_, b_besty_, b_act_, b_red_;
-  DevBuf b_tc_, b_tr_, b_ta_, b_tt_, b_desc_, b_state_;
+  DevBuf b_tc_, b_tr_, b_ta_, b_tt_, b_desc_, b_cdesc_, b_state_;
   DevBuf b_ps_c_, b_ps_a_, b_ps_t_, b_pe_c_, b_pe_r_, b_pe_a_, b_pe_t_;
   DevBuf b_done_, b_v_, b_v0_, b_av_, b_len_, b_zero_, b_fa_, b_ft_, b_fc_, b_fr_;
   DevBuf b_a_orig_st_, b_t_orig_st_;
@@ -374,7 +420,7 @@ class EngineImpl final : public Engine {
             &b_lhat64_, &b_uhat64_, &b_x0_, &b_x1_, &b_y0_, &b_y1_, &b_aty0_, &b_aty1_,
             &b_xbar_, &b_xrst_, &b_yrst_, &b_xsum_, &b_ysum_, &b_xtot_, &b_ytot_, &b_xu0_, &b_xu1_, &b_yu0_,
             &b_yu1_, &b_bestx_, &b_besty_, &b_act_, &b_red_, &b_tc_, &b_tr_, &b_ta_, &b_tt_,
-            &b_desc_, &b_state_, &b_ps_c_, &b_ps_a_, &b_ps_t_, &b_pe_c_, &b_pe_r_, &b_pe_a_,
+            &b_desc_, &b_cdesc_, &b_state_, &b_ps_c_, &b_ps_a_, &b_ps_t_, &b_pe_c_, &b_pe_r_, &b_pe_a_,
             &b_pe_t_, &b_done_, &b_v_, &b_v0_, &b_av_, &b_len_, &b_zero_, &b_fa_, &b_ft_,
             &b_fc_, &b_fr_, &b_a_orig_st_, &b_t_orig_st_, &b_wx_, &b_wy_};
   }
@@ -557,6 +603,8 @@ int EngineImpl<T>::upload() {
     s.cost_norm = std::sqrt(cost_square);
     s.bound_norm = std::sqrt(bound_square);
     s.omega = 1.0;
+    s.rule_shrink = rule_table_host()[0];
+    s.rule_grow = rule_table_host()[1];
     if (k < static_cast<int32_t>(warm_.size()) && warm_[static_cast<size_t>(k)].set) {
       const WarmStart& w = warm_[static_cast<size_t>(k)];
       if (w.omega > 0.0) s.omega = std::min(1e6, std::max(1e-6, w.omega));  // pdhg.cpp:627-629
@@ -784,6 +832,17 @@ int EngineImpl<T>::upload() {
   InstDesc* pdesc;
   SK_TRY(put(b_desc_, desc_, &pdesc));
   a.desc = pdesc;
+  {
+    std::vector<InstDesc>& cd = stage_host_.cdesc;
+    cd.resize(static_cast<size_t>(n_tc_));
+    for (int32_t k = 0; k < K_; ++k) {
+      const InstDesc& d = desc_[static_cast<size_t>(k)];
+      for (int32_t c = d.tc0; c < d.tc1; ++c) cd[static_cast<size_t>(c)] = d;
+    }
+    InstDesc* pcd;
+    SK_TRY(put(b_cdesc_, cd, &pcd));
+    a.cdesc = pcd;
+  }
   // two buffers of instance records and of primal partials: a primal launch that decides the
   // previous step reads one and writes the other (enqueue_block)
   SK_TRY(alloc(b_state_, 2 * static_cast<size_t>(K_), &a.state));
@@ -824,6 +883,13 @@ int EngineImpl<T>::upload() {
   SK_TRY(alloc(b_ft_, static_cast<size_t>(n_tt_), &a.ft));
   SK_TRY(alloc(b_fc_, static_cast<size_t>(n_tc_), &a.fc));
   SK_TRY(alloc(b_fr_, static_cast<size_t>(n_tr_), &a.fr));
+  {
+    const double* rule = nullptr;
+    const int st_rule = rule_table_device(&rule);
+    if (st_rule) return st_rule;
+    a.rule = rule;
+    a.rule_len = static_cast<int32_t>(kRuleIterations);
+  }
   a.n_tc = n_tc_;
   a.n_ta = n_ta_;
   a.n_tt = n_tt_;

@@ -118,6 +118,9 @@ struct Args {
   const Tile* __restrict__ ta;  // row blocks of CSR(A)
   const Tile* __restrict__ tt;  // row blocks of CSR(A^T)
   const InstDesc* __restrict__ desc;
+  // per column chunk, its instance's descriptor: a deciding primal launch loads it beside the
+  // chunk's tile record instead of after it
+  const InstDesc* __restrict__ cdesc;
   InstState* state;
   // where a decision writes the instance records: the primal launch that decides the previous
   // step (decide = 1) reads `state` and writes here; the decision launch (k_step_finalize)
@@ -138,6 +141,10 @@ struct Args {
   double* pe_a;  // eval: 8 per CSR(A) block
   double* pe_t;  // eval: 8 per CSR(A^T) block
   uint32_t* done_count;
+  // the step-size rule's factors for iterations [0, rule_len), two per iteration (shrink,
+  // grow), computed once on the host with the reference's formula
+  const double* __restrict__ rule;
+  int32_t rule_len;
   int32_t n_tc;    // number of column-chunk tiles (eval kernels cover tc then tr)
   int32_t n_ta;    // row blocks of CSR(A)
   int32_t n_tt;    // row blocks of CSR(A^T)
@@ -446,16 +453,23 @@ __device__ __forceinline__ void finish(InstState& s, int status, uint32_t* done_
   if (done_count) atomicAdd(done_count, 1u);
 }
 
-// The growth and shrink factors of the step-size rule at the instance's iteration count
-// (pdhg.cpp:472-474).
-__device__ __forceinline__ void step_rule_factors(const InstState& s, double* shrink, double* grow) {
-  const double exponent = static_cast<double>(s.iteration + 1 > 2 ? s.iteration + 1 : 2);
-  *shrink = 1.0 - pow(exponent, -0.3);
-  *grow = 1.0 + pow(exponent, -0.6);
+// The shrink and grow factors of the step-size rule at iteration k (pdhg.cpp:472-474): from
+// the table for k below rule_len (which covers the default limit of 10^6 iterations), else
+// computed here with the device's pow.
+template <class T>
+__device__ __forceinline__ void rule_factors(const Args<T>& a, const int64_t k, double* f) {
+  if (k < a.rule_len) {
+    f[0] = a.rule[2 * k];
+    f[1] = a.rule[2 * k + 1];
+  } else {
+    const double exponent = static_cast<double>(k + 1 > 2 ? k + 1 : 2);
+    f[0] = 1.0 - pow(exponent, -0.3);
+    f[1] = 1.0 + pow(exponent, -0.6);
+  }
 }
 
-// The adaptive step-size rule and acceptance (pdhg.cpp:410-503); shrink and grow from
-// step_rule_factors.
+// The adaptive step-size rule and acceptance (pdhg.cpp:410-503); shrink and grow are the
+// rule's factors at the current iteration (rule_factors).
 __device__ void decide_step(InstState& s, double dx2, double dy2, double inter_sum, int has_rows,
                             double shrink, double grow, uint32_t* done_count) {
   const double omega = s.omega;
@@ -548,28 +562,38 @@ __device__ __forceinline__ void copy_state(InstState* dst, const InstState* src)
 // the chunks of a deciding primal launch agree without exchanging anything.
 // The partials are visible because they were written by earlier launches (or, in
 // k_resident, by this workgroup before a barrier); no device-scope fence is needed.
-// `red` holds 3 * (kBlock / kWave) values, `rule` 2; sm is shared by the workgroup.
+// `red` holds 3 * (kBlock / kWave) values; sm is shared by the workgroup.
 template <class T>
 __device__ __forceinline__ void decide_instance(const Args<T>& a, const int inst,
                                                 const InstDesc& d, const InstState* in,
                                                 InstState* out, const double* ps_c,
-                                                InstState* sm, double* red, double* rule) {
-  // the record and the tiles' partials are loaded together (the partials of an instance that
-  // did not step are read and dropped)
-  copy_state(sm, in + inst);
-  // ||x'-x||^2, ||y'-y||^2 and the interaction: lane sums over the tiles, then one reduction
+                                                InstState* sm, double* red) {
+  // the tiles' partials and the record are loaded together (the partials of an instance that
+  // did not step are read and dropped): ||x'-x||^2, ||y'-y||^2 and the interaction as lane
+  // sums over the tiles, then one reduction
   double v[3] = {0.0, 0.0, 0.0};
   for (int k = threadIdx.x; k < d.tc1 - d.tc0; k += kBlock) v[0] += ps_c[d.tc0 + k];
   for (int k = threadIdx.x; k < d.ta1 - d.ta0; k += kBlock) v[1] += a.ps_a[d.ta0 + k];
   for (int k = threadIdx.x; k < d.tt1 - d.tt0; k += kBlock) v[2] += a.ps_t[d.tt0 + k];
+  copy_state(sm, in + inst);
   __syncthreads();
   const bool go = stepping(*sm);
   if (go) {
-    // the rule's powers in the second wavefront, beside the reduction
-    if (threadIdx.x == kWave) step_rule_factors(*sm, &rule[0], &rule[1]);
+    // the rule's factors at this iteration come with the record; the workgroup that writes
+    // the record loads those of the next iteration (after an accepted step) beside the
+    // reduction
+    double next_rule[2] = {0.0, 0.0};
+    if (out && threadIdx.x == 0) rule_factors(a, sm->iteration + 1, next_rule);
     block_reduce_n<3, 0u>(v, red);
-    if (threadIdx.x == 0)
-      decide_step(*sm, v[0], v[1], v[2], d.rows > 0, rule[0], rule[1], out ? a.done_count : nullptr);
+    if (threadIdx.x == 0) {
+      const int64_t before = sm->iteration;
+      decide_step(*sm, v[0], v[1], v[2], d.rows > 0, sm->rule_shrink, sm->rule_grow,
+                  out ? a.done_count : nullptr);
+      if (out && sm->iteration != before) {
+        sm->rule_shrink = next_rule[0];
+        sm->rule_grow = next_rule[1];
+      }
+    }
     __syncthreads();
   }
   if (out && (go || out != in)) {
@@ -586,10 +610,9 @@ __device__ __forceinline__ void decide_instance(const Args<T>& a, const int inst
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_step_finalize(Args<T> a) {
   __shared__ double red[3 * (kBlock / kWave)];
-  __shared__ double rule[2];
   __shared__ InstState sm;
   const int inst = static_cast<int>(blockIdx.x);
-  decide_instance(a, inst, a.desc[inst], a.state, a.state_next, a.ps_c, &sm, red, rule);
+  decide_instance(a, inst, a.desc[inst], a.state, a.state_next, a.ps_c, &sm, red);
 }
 
 // The primal step of column chunk `tile` after deciding the previous step itself: every chunk
@@ -602,9 +625,9 @@ __global__ __launch_bounds__(kBlock) void k_step_finalize(Args<T> a) {
 // accepted), are loaded before it; a rejected step reloads them from the other buffer.
 template <class T>
 __device__ __forceinline__ void primal_chunk_decided(const Args<T>& a, const int tile,
-                                                     double* red, double* rule, InstState* sm) {
+                                                     double* red, InstState* sm) {
   const Tile t = a.tc[tile];
-  const InstDesc d = a.desc[t.inst];
+  const InstDesc d = a.cdesc[tile];
   const int p_old = a.state[t.inst].parity;
   const T* __restrict__ xs = p_old ? a.x0 : a.x1;
   const T* __restrict__ atys = p_old ? a.aty0 : a.aty1;
@@ -623,7 +646,7 @@ __device__ __forceinline__ void primal_chunk_decided(const Args<T>& a, const int
     }
   }
   decide_instance(a, t.inst, d, a.state, tile == d.tc0 ? a.state_next : nullptr, a.ps_c_prev,
-                  sm, red, rule);
+                  sm, red);
   const InstState& s = *sm;
   if (!stepping(s)) return;
   const int p = s.parity;
@@ -648,10 +671,9 @@ __device__ __forceinline__ void primal_chunk_decided(const Args<T>& a, const int
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_step_primal(Args<T> a) {
   __shared__ double red[3 * (kBlock / kWave)];
-  __shared__ double rule[2];
   __shared__ InstState sm;
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
-  if (a.decide) primal_chunk_decided(a, tile, red, rule, &sm);
+  if (a.decide) primal_chunk_decided(a, tile, red, &sm);
   else primal_chunk(a, tile, red);
 }
 
@@ -1060,7 +1082,6 @@ template <class T, int CAP, bool PACK>
 __global__ __launch_bounds__(kBlock) void k_resident(Args<T> a, int32_t steps) {
   __shared__ D2 stage[CAP];
   __shared__ double red[18 * (kBlock / kWave)];
-  __shared__ double rule[2];
   __shared__ InstState sm;
   __shared__ uint32_t flags;
   T* const stage_t = reinterpret_cast<T*>(stage);
@@ -1094,7 +1115,7 @@ __global__ __launch_bounds__(kBlock) void k_resident(Args<T> a, int32_t steps) {
     __syncthreads();
     for (int k = d.tt0; k < d.tt1; ++k) aty_tile<T, PACK, kInFlight>(a, k, stage_t, red);
     __syncthreads();
-    decide_instance(a, inst, d, a.state, a.state, a.ps_c, &sm, red, rule);
+    decide_instance(a, inst, d, a.state, a.state, a.ps_c, &sm, red);
   }
   __syncthreads();
   if (rec->report_pending) {
