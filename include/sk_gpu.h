@@ -255,10 +255,12 @@ typedef struct {
                                 nonzeros first, so the most-gathered part of x stays in L2),
                                 1 every instance so, 2 the LP's own order. Answers are returned
                                 in the LP's order either way. */
-  int32_t step_fusion;       /* 0 automatic (default): when every instance has at most 4 SpMV
-                                tiles and 2 column chunks, one workgroup per instance runs a
-                                whole PDHG step in one launch, with the same per-tile code and
-                                bit for bit the same answers as the four launches; 1 never */
+  int32_t step_fusion;       /* bit mask, 0 (default) both on, each bit for bit the same
+                                answers. Bit 0 (1) off: the resident solve, where every instance
+                                has at most 4 SpMV tiles and 2 column chunks, one workgroup per
+                                instance runs many steps and evaluations in one launch. Bit 1
+                                (2) off: each step's decision made by the next primal launch
+                                instead of a launch of its own. */
   int32_t backend;           /* backend (options.cpp:228-236): SK_BACKEND_AUTO (default), _CPU,
                                 _GPU or _HYBRID. Read by sk_select_backend; a solve, batch or
                                 server of this library refuses _CPU (the CPU engine is the
