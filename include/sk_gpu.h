@@ -548,7 +548,8 @@ typedef struct {
   int32_t streams;  /* sub-batches the batch ran as (sk_pdhg_options.streams) */
   int32_t tile_nnz; /* row-block capacity the batch runs with (picked by the autotuner when
                        sk_pdhg_options.tile_nnz is -1) */
-  int32_t fused;    /* 1 when a PDHG step runs as one launch (sk_pdhg_options.step_fusion) */
+  int32_t fused;    /* 1 when the batch runs as the resident solve: many steps and
+                       evaluations per launch (sk_pdhg_options.step_fusion) */
   int32_t reserved;
 } sk_batch_stats;
 

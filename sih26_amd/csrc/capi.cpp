@@ -937,7 +937,7 @@ int32_t sk_batch_stats_get(const sk_batch* b, sk_batch_stats* out) {
   out->spmv_stage = b->engine->stage_choice();
   out->streams = b->engine->streams();
   out->tile_nnz = b->tile_nnz;
-  out->fused = b->engine->fused() ? 1 : 0;
+  out->fused = b->engine->fused();
   out->reserved = 0;
   return SK_GPU_SUCCESS;
 }

@@ -282,7 +282,7 @@ class EngineImpl final : public Engine {
   int64_t total_cols() const override { return n_; }
   int stage_choice() const override { return stage_; }
   int32_t streams() const override { return 1; }
-  bool fused() const override { return fused_; }
+  int32_t fused() const override { return fused_ ? 1 : 0; }
 
   int upload() override;
   int solve() override;

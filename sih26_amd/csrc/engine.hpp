@@ -109,9 +109,9 @@ class Engine {
   virtual int stage_choice() const = 0;
   // Sub-batches the batch runs as (each on its own stream).
   virtual int32_t streams() const = 0;
-  // whether the uploaded batch runs as the resident solve (k_resident): one launch for many
-  // steps and evaluations of every instance
-  virtual bool fused() const = 0;
+  // how the uploaded batch runs: 0 separate step launches; 1 the resident solve (k_resident),
+  // one launch for many steps and evaluations of every instance
+  virtual int32_t fused() const = 0;
 
   // Probes for tests, valid once a solve (or start()) has run the device set-up. They return
   // what the device holds or computes, in the LP's own row and column order.

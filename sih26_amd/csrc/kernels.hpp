@@ -1078,16 +1078,56 @@ __global__ __launch_bounds__(kBlock) void k_eval_commit(Args<T> a) {
 // launch keeps the instance's data in the caches instead of crossing a kernel boundary (and
 // its launch gap) four times per step. The step product stages its values in the first
 // CAP * sizeof(T) bytes of the evaluation's staging buffer.
+// The phases of one round of the resident solve for instance `inst` (whole workgroup).
+template <class T, int CAP, bool PACK>
+__device__ __forceinline__ void resident_eval(const Args<T>& a, const int inst, const InstDesc& d,
+                                              D2* stage, double* red, uint32_t* flags) {
+  for (int k = d.tc0; k < d.tc1; ++k) eval_prepare_chunk(a, true, k, red);
+  for (int k = d.tr0; k < d.tr1; ++k) eval_prepare_chunk(a, false, k, red);
+  __syncthreads();
+  for (int k = d.ta0; k < d.ta1; ++k) eval_rows_tile<T, PACK>(a, k, stage, red);
+  for (int k = d.tt0; k < d.tt1; ++k) eval_cols_tile<T, PACK>(a, k, stage, red);
+  __syncthreads();
+  eval_finalize_instance(a, inst, red, flags);
+  __syncthreads();
+  for (int k = d.tc0; k < d.tc1; ++k) eval_commit_chunk(a, true, k);
+  for (int k = d.tr0; k < d.tr1; ++k) eval_commit_chunk(a, false, k);
+}
+
+template <class T, int CAP, bool PACK>
+__device__ __forceinline__ void resident_step(const Args<T>& a, const int inst, const InstDesc& d,
+                                              T* stage, double* red, InstState* sm) {
+  // a tile's groups of four staged nonzeros (CAP / 4, one more when the tile does not start
+  // on a multiple of four) all in flight at once
+  constexpr int kInFlight = (CAP / 4 + 1 + kBlock - 1) / kBlock;
+  for (int k = d.tc0; k < d.tc1; ++k) primal_chunk(a, k, red);
+  __syncthreads();
+  for (int k = d.ta0; k < d.ta1; ++k) dual_tile<T, PACK, kInFlight>(a, k, stage, red);
+  __syncthreads();
+  for (int k = d.tt0; k < d.tt1; ++k) aty_tile<T, PACK, kInFlight>(a, k, stage, red);
+  __syncthreads();
+  decide_instance(a, inst, d, a.state, a.state, a.ps_c, sm, red);
+}
+
+// The final (report) evaluation of a stopped instance.
+template <class T, int CAP, bool PACK>
+__device__ __forceinline__ void resident_report(const Args<T>& a, const int inst,
+                                                const InstDesc& d, D2* stage, double* red,
+                                                uint32_t* flags) {
+  Args<T> r = a;
+  r.report = 1;
+  for (int k = d.ta0; k < d.ta1; ++k) eval_rows_tile<T, PACK>(r, k, stage, red);
+  for (int k = d.tt0; k < d.tt1; ++k) eval_cols_tile<T, PACK>(r, k, stage, red);
+  __syncthreads();
+  eval_finalize_instance(r, inst, red, flags);
+}
+
 template <class T, int CAP, bool PACK>
 __global__ __launch_bounds__(kBlock) void k_resident(Args<T> a, int32_t steps) {
   __shared__ D2 stage[CAP];
   __shared__ double red[18 * (kBlock / kWave)];
   __shared__ InstState sm;
   __shared__ uint32_t flags;
-  T* const stage_t = reinterpret_cast<T*>(stage);
-  // a tile's groups of four staged nonzeros (CAP / 4, one more when the tile does not start
-  // on a multiple of four) all in flight at once
-  constexpr int kInFlight = (CAP / 4 + 1 + kBlock - 1) / kBlock;
   const int inst = static_cast<int>(blockIdx.x);
   const InstDesc d = a.desc[inst];
   const InstState* rec = a.state + inst;
@@ -1096,36 +1136,11 @@ __global__ __launch_bounds__(kBlock) void k_resident(Args<T> a, int32_t steps) {
     __syncthreads();
     const int32_t done = rec->done, evaluate = rec->eval_pending;
     if (done) break;
-    if (evaluate) {
-      for (int k = d.tc0; k < d.tc1; ++k) eval_prepare_chunk(a, true, k, red);
-      for (int k = d.tr0; k < d.tr1; ++k) eval_prepare_chunk(a, false, k, red);
-      __syncthreads();
-      for (int k = d.ta0; k < d.ta1; ++k) eval_rows_tile<T, PACK>(a, k, stage, red);
-      for (int k = d.tt0; k < d.tt1; ++k) eval_cols_tile<T, PACK>(a, k, stage, red);
-      __syncthreads();
-      eval_finalize_instance(a, inst, red, &flags);
-      __syncthreads();
-      for (int k = d.tc0; k < d.tc1; ++k) eval_commit_chunk(a, true, k);
-      for (int k = d.tr0; k < d.tr1; ++k) eval_commit_chunk(a, false, k);
-      continue;
-    }
-    for (int k = d.tc0; k < d.tc1; ++k) primal_chunk(a, k, red);
-    __syncthreads();
-    for (int k = d.ta0; k < d.ta1; ++k) dual_tile<T, PACK, kInFlight>(a, k, stage_t, red);
-    __syncthreads();
-    for (int k = d.tt0; k < d.tt1; ++k) aty_tile<T, PACK, kInFlight>(a, k, stage_t, red);
-    __syncthreads();
-    decide_instance(a, inst, d, a.state, a.state, a.ps_c, &sm, red);
+    if (evaluate) resident_eval<T, CAP, PACK>(a, inst, d, stage, red, &flags);
+    else resident_step<T, CAP, PACK>(a, inst, d, reinterpret_cast<T*>(stage), red, &sm);
   }
   __syncthreads();
-  if (rec->report_pending) {
-    Args<T> r = a;
-    r.report = 1;
-    for (int k = d.ta0; k < d.ta1; ++k) eval_rows_tile<T, PACK>(r, k, stage, red);
-    for (int k = d.tt0; k < d.tt1; ++k) eval_cols_tile<T, PACK>(r, k, stage, red);
-    __syncthreads();
-    eval_finalize_instance(r, inst, red, &flags);
-  }
+  if (rec->report_pending) resident_report<T, CAP, PACK>(a, inst, d, stage, red, &flags);
 }
 
 // ---- set-up kernels (FP64, no FMA contraction: bit-for-bit the reference's arithmetic) -------

@@ -255,10 +255,11 @@ class SplitEngine final : public Engine {
   }
   int stage_choice() const override { return parts_.empty() ? 1 : parts_[0]->stage_choice(); }
   int32_t streams() const override { return static_cast<int32_t>(parts_.size()); }
-  bool fused() const override {
-    bool all = !parts_.empty();
-    for (const auto& e : parts_) all = all && e->fused();
-    return all;
+  int32_t fused() const override {
+    // the least of the parts: a split batch is resident only if every part is
+    int32_t least = parts_.empty() ? 0 : 1;
+    for (const auto& e : parts_) least = std::min(least, e->fused());
+    return least;
   }
 
  private:

@@ -317,11 +317,11 @@ def test_autotuned_batch_runs_with_the_capacity_it_picked(gpu, oracle_mod):
 
 
 @pytest.mark.parametrize("precision", ["fp32", "fp64"])
-def test_fused_step_equals_four_launches(gpu, precision):
-    """Instances with a few tiles run a whole step in one launch (k_step_fused) with the
-    per-tile code of the separate step kernels: the answers are bit for bit those of the
-    separate launches (with the decision folded into the next primal launch, step_fusion = 1,
-    or in a launch of its own every step, step_fusion = 3), alone and as a batch of small
+def test_resident_solve_equals_separate_launches(gpu, precision):
+    """Instances with a few tiles run as the resident solve (many steps and evaluations per
+    launch, with the per-tile code of the separate step kernels): the answers are bit for bit
+    those of the separate launches (with the decision folded into the next primal launch,
+    step_fusion = 1, or in a launch of its own every step, 3), alone and as a batch of small
     instances."""
     kw = dict(pdhg_tolerance=1e-4, pdhg_stop_at_request=True, pdhg_absolute_primal=False,
               pdhg_precision=precision)
