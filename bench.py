@@ -229,7 +229,7 @@ def fp64_single(sk, lps, budget_s: float) -> dict:
 
 
 def serve_stream(sk, requests: int, rate: float, slots: int, max_batch: int,
-                 warm: int = 8, lookahead: int = 256, lanes: int = 1) -> dict:
+                 warm: int = 16, lookahead: int = 256, lanes: int = 1) -> dict:
     """Open-loop load: request i arrives at the i-th point of a Poisson process (numpy
     default_rng(42) exponential gaps, mean 1/rate) and is submitted then; the LP (preset 4,
     instance i) is generated ahead by worker threads. Latency = completion - arrival."""
@@ -241,9 +241,14 @@ def serve_stream(sk, requests: int, rate: float, slots: int, max_batch: int,
     srv = sk.Server(solve_options(sk), slots=slots, max_batch=max_batch, keep_solutions=False,
                     lanes=lanes)
     pool = ThreadPoolExecutor(max_workers=3)
-    # warm the runtime (module load, first graph captures) on requests outside the stream
-    for i in range(warm):
-        srv.submit(sk.Lp.generate(4, requests + i))
+    # warm the runtime (module load, every slot's first graph captures and its buffers grown
+    # to the large end of the size range) on requests outside the stream: the largest of 256
+    # further ones, submitted a few ms apart so that every slot takes some
+    side = range(requests, requests + 256)
+    big = sorted(side, key=lambda i: -sk.Lp.preset(4, i)["num_rows"])[:warm]
+    for i in big:
+        srv.submit(sk.Lp.generate(4, i))
+        time.sleep(0.003)
     srv.wait()
     for rid in srv.completed():
         srv.result(rid)
