@@ -134,10 +134,10 @@ constexpr int kTriRows = 64;
 // floored positive), by forward substitution of the identity, one column per lane and one
 // workgroup per block; block k's inverse is column-major at inv + k * kTriRows^2.
 __global__ __launch_bounds__(kLanes) void k_tri_inverse(const double* __restrict__ L, int32_t m,
-                                                        double* __restrict__ inv) {
+                                                        int32_t first, double* __restrict__ inv) {
   __shared__ double D[kTriRows][kTriRows + 1];
   __shared__ double X[kTriRows][kTriRows + 1];  // X[i][j]: row i, column j of the inverse
-  const int k = static_cast<int>(blockIdx.x), r0 = k * kTriRows;
+  const int k = first + static_cast<int>(blockIdx.x), r0 = k * kTriRows;
   const int rows = min(kTriRows, m - r0), tid = threadIdx.x;
   for (int idx = tid; idx < kTriRows * kTriRows; idx += kLanes) {
     const int i = idx % kTriRows, c = idx / kTriRows;
@@ -299,6 +299,7 @@ struct NormalEquations::Impl {
   uint32_t* flags = nullptr;
   int32_t* failed = nullptr;
   double* inv = nullptr;  // inverses of the factor's diagonal blocks
+  double* panel = nullptr;  // m x kPanel: a panel solve's product before it replaces the panel
   uint32_t epoch = 0;
   int32_t* failed_host = nullptr;  // pinned
 
@@ -307,7 +308,8 @@ struct NormalEquations::Impl {
                     static_cast<void*>(ci), static_cast<void*>(rv), static_cast<void*>(cv),
                     static_cast<void*>(theta), static_cast<void*>(shift), static_cast<void*>(M),
                     static_cast<void*>(x), static_cast<void*>(floored), static_cast<void*>(flags),
-                    static_cast<void*>(failed), static_cast<void*>(inv)})
+                    static_cast<void*>(failed), static_cast<void*>(inv),
+                    static_cast<void*>(panel)})
       if (p) (void)hipFree(p);
     if (failed_host) (void)hipHostFree(failed_host);
     if (blas) (void)rocblas_destroy_handle(blas);
@@ -359,6 +361,9 @@ int NormalEquations::init(int32_t m, int32_t n, const std::vector<int32_t>& row_
   NE_HIP(hipMalloc(reinterpret_cast<void**>(&d.inv),
                    std::max<size_t>(1, nb) * kTriRows * kTriRows * sizeof(double)),
          kStatusOutOfMemory);
+  NE_HIP(hipMalloc(reinterpret_cast<void**>(&d.panel),
+                   std::max<size_t>(1, m) * kPanel * sizeof(double)),
+         kStatusOutOfMemory);
   NE_HIP(hipHostMalloc(reinterpret_cast<void**>(&d.failed_host), sizeof(int32_t)), kStatusOutOfMemory);
   // epochs start at 1, so no flag holds one before its block publishes it
   NE_HIP(hipMemsetAsync(d.flags, 0, std::max<size_t>(1, nb) * sizeof(uint32_t), d.stream), kStatusCopyFailed);
@@ -387,25 +392,32 @@ int NormalEquations::factor(const double* theta, const double* shift, double flo
         m, d.rp, d.ri, d.rv, d.cp, d.ci, d.cv, d.theta, d.shift, w0, w1, d.M);
     NE_HIP(hipGetLastError(), kStatusLaunchFailed);
   }
-  const double one = 1.0, minus_one = -1.0;
+  static_assert(kPanel == kTriRows, "a panel is one block of the triangular solves");
+  const double one = 1.0, zero = 0.0, minus_one = -1.0;
   for (int32_t kb = 0; kb < m; kb += kPanel) {
     const int32_t b = std::min(kPanel, m - kb);
     k_chol_block<<<1, kPanel, 0, s>>>(d.M, m, kb, b, floor, d.floored);
     NE_HIP(hipGetLastError(), kStatusLaunchFailed);
+    // the inverse of the diagonal block: the panel solve below and the triangular solves use it
+    k_tri_inverse<<<1, kLanes, 0, s>>>(d.M, m, kb / kPanel, d.inv);
+    NE_HIP(hipGetLastError(), kStatusLaunchFailed);
     const int32_t rest = m - kb - b;
     if (rest == 0) continue;
-    double* l11 = d.M + static_cast<size_t>(kb) * m + kb;
-    double* l21 = l11 + b;
+    double* l21 = d.M + static_cast<size_t>(kb) * m + kb + b;
     double* a22 = d.M + static_cast<size_t>(kb + b) * m + kb + b;
-    NE_BLAS(rocblas_dtrsm(d.blas, rocblas_side_right, rocblas_fill_lower,
-                          rocblas_operation_transpose, rocblas_diagonal_non_unit, rest, b, &one,
-                          l11, m, l21, m));
+    const double* inv11 = d.inv + static_cast<size_t>(kb / kPanel) * kPanel * kPanel;
+    // L21 = A21 L11^-T (the panel solve as one product with the block's inverse), then the
+    // product replaces the panel
+    NE_BLAS(rocblas_dgemm(d.blas, rocblas_operation_none, rocblas_operation_transpose, rest, b, b,
+                          &one, l21, m, inv11, kPanel, &zero, d.panel, rest));
+    NE_HIP(hipMemcpy2DAsync(l21, static_cast<size_t>(m) * sizeof(double), d.panel,
+                            static_cast<size_t>(rest) * sizeof(double),
+                            static_cast<size_t>(rest) * sizeof(double), static_cast<size_t>(b),
+                            hipMemcpyDeviceToDevice, s),
+           kStatusCopyFailed);
     NE_BLAS(rocblas_dsyrk(d.blas, rocblas_fill_lower, rocblas_operation_none, rest, b,
                           &minus_one, l21, m, &one, a22, m));
   }
-  // the diagonal blocks' inverses for the blocked triangular solves
-  k_tri_inverse<<<static_cast<unsigned>((m + kTriRows - 1) / kTriRows), kLanes, 0, s>>>(d.M, m, d.inv);
-  NE_HIP(hipGetLastError(), kStatusLaunchFailed);
   NE_HIP(hipMemcpyAsync(floored, d.floored, sizeof(int64_t), hipMemcpyDeviceToHost, s),
          kStatusCopyFailed);
   NE_HIP(hipStreamSynchronize(s), kStatusSyncFailed);
