@@ -77,9 +77,14 @@ const char* sk_gpu_last_error(void);
  * 2: the n-th kernel launch an engine makes from now on fails with SK_GPU_LAUNCH_FAILED (the
  * launch is not made, so no kernel runs in a bad state); kind 3: instance n of the next batch
  * upload gets a NaN cost in its first column, so its first step is non-finite and it ends with
- * SK_SOLVE_NUMERICAL_ERROR while the other instances run on; kind 0 clears every injection.
- * Process-wide; each injection fires once. */
+ * SK_SOLVE_NUMERICAL_ERROR while the other instances run on; kind 4: the next n triangular
+ * solves of the interior point give up at their first wait (a bound of zero polls), so each is
+ * redone by the rocBLAS fallback; kind 0 clears every injection. Process-wide; kinds 1-3 fire
+ * once. */
 int32_t sk_fault_inject(int32_t kind, int64_t n);
+/* How many times injections of `kind` (1-4) have taken effect in this process (no counterpart
+ * in the reference; for the same tests). */
+int64_t sk_fault_fired(int32_t kind);
 /* gpu_runtime.hpp:90-105 allocate/free/copy/memset */
 int32_t sk_gpu_malloc(void** dev_ptr, size_t bytes);
 int32_t sk_gpu_free(void* dev_ptr);

@@ -137,6 +137,7 @@ SIGNATURES = {
     "sk_gpu_synchronize": (i32, []),
     "sk_gpu_last_error": (cstr, []),
     "sk_fault_inject": (i32, [i32, i64]),
+    "sk_fault_fired": (i64, [i32]),
     "sk_gpu_malloc": (i32, [P(vp), C.c_size_t]),
     "sk_gpu_free": (i32, [vp]),
     "sk_gpu_copy_h2d": (i32, [vp, vp, C.c_size_t]),

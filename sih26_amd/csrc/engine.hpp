@@ -158,6 +158,9 @@ int query_device_memory(int64_t* free_bytes, int64_t* total_bytes);
 bool fault_allocation_fails();
 bool fault_launch_fails();
 int64_t fault_take_nan_instance();
+// True for each of the next n triangular solves of the interior point (kind 4): its blocks give
+// up at their first wait, so the solve takes the rocBLAS path.
+bool fault_solve_gives_up();
 
 // Thread-local description of the last failure inside the library.
 void set_last_error(const std::string& message);

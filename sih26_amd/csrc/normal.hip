@@ -126,8 +126,9 @@ __global__ __launch_bounds__(kPanel) void k_chol_block(double* __restrict__ M, i
 // for it is loaded before the wait. It then multiplies by the inverse of its diagonal block
 // (k_tri_inverse, once per factorization) and publishes. A block only waits for blocks
 // dispatched before it, which never wait for it, and each wait is bounded by spin_limit
-// polls; a block that runs out sets *failed and stops, and the host then redoes the solve
-// with rocBLAS. Every launch uses a new epoch, so the flags are never cleared between solves.
+// polls (a bound of zero: at its first wait, published or not); a block that runs out sets
+// *failed and stops, and the host then redoes the solve with rocBLAS. Every launch uses a new
+// epoch, so the flags are never cleared between solves.
 constexpr int kTriRows = 64;
 
 // Inverse of each kTriRows x kTriRows diagonal block of L (lower triangular, pivots already
@@ -206,8 +207,10 @@ __global__ __launch_bounds__(kLanes) void k_trsv_blocks(const double* __restrict
     }
     if (tid == 0) {
       uint32_t spins = 0;
-      while (__hip_atomic_load(flags + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != epoch) {
-        if (++spins > spin_limit || __hip_atomic_load(failed, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+      while (spin_limit == 0 ||
+           __hip_atomic_load(flags + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != epoch) {
+        if (spin_limit == 0 || ++spins > spin_limit ||
+          __hip_atomic_load(failed, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
           __hip_atomic_store(failed, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
           stop = 1;
           break;
@@ -435,9 +438,11 @@ int NormalEquations::solve(double* rhs) {
   // forward then backward substitution with the blocked look-back kernel (k_trsv_blocks)
   const unsigned nb = static_cast<unsigned>((m + kTriRows - 1) / kTriRows);
   constexpr uint32_t kSpinLimit = 1u << 20;  // polls of one flag (~1 s), far above any wait
+  const bool give_up = fault_solve_gives_up();  // injected (sk_fault_inject kind 4)
   for (int32_t trans = 0; trans < 2; ++trans) {
     if (++d.epoch == 0) ++d.epoch;
-    k_trsv_blocks<<<nb, kLanes, 0, s>>>(d.M, d.inv, m, trans, d.x, d.flags, d.epoch, kSpinLimit, d.failed);
+    k_trsv_blocks<<<nb, kLanes, 0, s>>>(d.M, d.inv, m, trans, d.x, d.flags, d.epoch,
+                                        give_up ? 0u : kSpinLimit, d.failed);
     NE_HIP(hipGetLastError(), kStatusLaunchFailed);
   }
   NE_HIP(hipMemcpyAsync(d.failed_host, d.failed, sizeof(int32_t), hipMemcpyDeviceToHost, s), kStatusCopyFailed);

@@ -25,6 +25,8 @@ namespace {
 std::atomic<int64_t> g_alloc_countdown{0};
 std::atomic<int64_t> g_launch_countdown{0};
 std::atomic<int64_t> g_nan_instance{-1};
+std::atomic<int64_t> g_solve_giveups{0};
+std::atomic<int64_t> g_fired[5];  // per kind: how many times an injection took effect
 
 // Counts one event down; true exactly when the countdown reaches zero from one.
 bool count_down(std::atomic<int64_t>& c) {
@@ -35,9 +37,28 @@ bool count_down(std::atomic<int64_t>& c) {
 }
 }  // namespace
 
-bool fault_allocation_fails() { return count_down(g_alloc_countdown); }
-bool fault_launch_fails() { return count_down(g_launch_countdown); }
-int64_t fault_take_nan_instance() { return g_nan_instance.exchange(-1); }
+bool fault_allocation_fails() {
+  const bool f = count_down(g_alloc_countdown);
+  if (f) ++g_fired[1];
+  return f;
+}
+bool fault_launch_fails() {
+  const bool f = count_down(g_launch_countdown);
+  if (f) ++g_fired[2];
+  return f;
+}
+int64_t fault_take_nan_instance() {
+  const int64_t k = g_nan_instance.exchange(-1);
+  if (k >= 0) ++g_fired[3];
+  return k;
+}
+bool fault_solve_gives_up() {
+  int64_t v = g_solve_giveups.load();
+  while (v > 0 && !g_solve_giveups.compare_exchange_weak(v, v - 1)) {
+  }
+  if (v > 0) ++g_fired[4];
+  return v > 0;
+}
 
 // Per-element byte counts of the buffers upload() allocates (w = iterate bytes):
 //   nonzero: in both layouts, the CSR-order index, the FP64 original and FP64 scaled value,
@@ -96,6 +117,7 @@ int32_t sk_fault_inject(int32_t kind, int64_t n) {
       g_alloc_countdown = 0;
       g_launch_countdown = 0;
       g_nan_instance = -1;
+      g_solve_giveups = 0;
       return kStatusSuccess;
     case 1:
     case 2:
@@ -106,12 +128,18 @@ int32_t sk_fault_inject(int32_t kind, int64_t n) {
       if (n < 0) break;
       g_nan_instance = n;
       return kStatusSuccess;
+    case 4:
+      if (n < 1) break;
+      g_solve_giveups = n;
+      return kStatusSuccess;
     default:
       break;
   }
-  set_last_error("sk_fault_inject: kind must be 0..3, with n >= 1 (kinds 1, 2) or n >= 0 (kind 3)");
+  set_last_error("sk_fault_inject: kind must be 0..4, with n >= 1 (kinds 1, 2, 4) or n >= 0 (kind 3)");
   return kStatusInvalidArgument;
 }
+
+int64_t sk_fault_fired(int32_t kind) { return kind >= 1 && kind <= 4 ? g_fired[kind].load() : 0; }
 
 /* gpu_ops_cpu.cpp:85-124 compare_vectors (a host function in the reference too) */
 int32_t sk_gpu_compare_vectors(int32_t n, const double* t, const double* r, double tol,

@@ -200,9 +200,13 @@ def test_ipm_options_and_argument_checks():
 def test_fault_injection_arguments_are_checked():
     lib = sk.lib()
     assert lib.sk_fault_inject(1, 3) == 0 and lib.sk_fault_inject(2, 1) == 0
-    assert lib.sk_fault_inject(3, 0) == 0 and lib.sk_fault_inject(0, 0) == 0
-    for kind, n in ((1, 0), (2, -1), (3, -2), (4, 1), (-1, 0)):
+    assert lib.sk_fault_inject(3, 0) == 0 and lib.sk_fault_inject(4, 2) == 0
+    assert lib.sk_fault_inject(0, 0) == 0
+    for kind, n in ((1, 0), (2, -1), (3, -2), (4, 0), (5, 1), (-1, 0)):
         assert lib.sk_fault_inject(kind, n) == 10
+    # nothing fired: no device work ran
+    assert [lib.sk_fault_fired(k) for k in (1, 2, 3, 4)] == [0, 0, 0, 0]
+    assert lib.sk_fault_fired(0) == 0 and lib.sk_fault_fired(9) == 0
 
 
 def test_validated_primal_step_without_a_device_reports_unavailable():

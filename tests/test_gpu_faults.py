@@ -130,3 +130,28 @@ def test_the_server_retires_a_failed_batch_and_serves_on(gpu):
     srv.close()
     assert rec["error"] == 0 and res.converged
     assert st["completed"] == 3 and st["failed"] == 1
+
+
+def test_a_triangular_solve_that_gives_up_is_redone_by_rocblas(gpu):
+    """The interior point's look-back triangular solves wait for earlier blocks with a bounded
+    number of polls. Forced to give up at the first wait (sk_fault_inject kind 4: a bound of
+    zero polls, every kernel still ends normally), each solve is redone by the rocBLAS
+    fallback, and the polish ends at the same optimum as with the look-back solves."""
+    lp = gpu.Lp.generate(3, 0)
+    assert lp.dims[0] > 64  # more than one block, so blocks wait for each other
+    first = gpu.solve(lp, **REL)
+    assert first.converged
+    lib = gpu.lib()
+    plain = gpu.polish(lp, first)
+    before = lib.sk_fault_fired(4)
+    assert lib.sk_fault_inject(4, 10**6) == 0
+    forced = gpu.polish(lp, first)
+    lib.sk_fault_inject(0, 0)
+    fired = lib.sk_fault_fired(4) - before
+    assert fired >= 6 * forced.polish_iterations  # every solve of every iteration
+    assert plain.status_name == forced.status_name == "optimal"
+    assert abs(forced.objective - plain.objective) <= 1e-8 * (1.0 + abs(plain.objective))
+    # and the look-back path works again afterwards
+    again = gpu.polish(lp, first)
+    assert again.status_name == "optimal"
+    np.testing.assert_array_equal(again.col_value, plain.col_value)
