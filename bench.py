@@ -411,7 +411,7 @@ def main() -> int:
     ap.add_argument("--eval-stride", type=int, default=20,
                     help="evaluation stage after every this many steps of a captured block")
     ap.add_argument("--requests", type=int, default=0,
-                    help="serving stream length (0: 10000 for --workload c4, 1000 otherwise)")
+                    help="serving stream length (0: 10000 for --workload c4, 3000 otherwise)")
     ap.add_argument("--rate", type=float, default=200.0, help="serving arrivals per second")
     ap.add_argument("--slots", type=int, default=2, help="serving: concurrent batches")
     ap.add_argument("--max-batch", type=int, default=256, help="serving: instances per batch")
@@ -553,7 +553,7 @@ def main() -> int:
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(lps, args.cpu_budget)
     if not args.no_serve and args.workload == "c3":
-        srv = serve_stream(sk, args.requests or 1000, args.rate, args.slots, args.max_batch,
+        srv = serve_stream(sk, args.requests or 3000, args.rate, args.slots, args.max_batch,
                            lanes=args.lanes)
         line["serving"] = srv
         line["latency_ms"] = {"p50": srv["latency_ms"]["p50"], "p99": srv["latency_ms"]["p99"],
