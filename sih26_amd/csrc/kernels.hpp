@@ -435,12 +435,33 @@ __global__ __launch_bounds__(kStepLanes) void k_probe_product(const Tile* __rest
       stage);
 }
 
+// A lane's share of a table of per-tile partials: acc + part[k * stride] for k = threadIdx.x,
+// threadIdx.x + kBlock, ... below count, added in that order. The loads are issued kSumBatch
+// at a time before their adds (a plain loop waits for each load before issuing the next), so a
+// table of count partials costs about count / (kBlock * kSumBatch) dependent memory round trips
+// instead of count / kBlock; the sum is bit for bit the plain loop's. A position past the table
+// reloads its last entry and is not added.
+constexpr int kSumBatch = 8;
+__device__ __forceinline__ double lane_sum(const double* __restrict__ part, int count,
+                                           int stride, double acc) {
+  for (int k0 = static_cast<int>(threadIdx.x); k0 < count; k0 += kSumBatch * kBlock) {
+    double w[kSumBatch];
+#pragma unroll
+    for (int u = 0; u < kSumBatch; ++u) {
+      const int k = k0 + u * kBlock;
+      w[u] = part[static_cast<size_t>(k < count ? k : count - 1) * stride];
+    }
+#pragma unroll
+    for (int u = 0; u < kSumBatch; ++u) acc = k0 + u * kBlock < count ? acc + w[u] : acc;
+  }
+  return acc;
+}
+
 // Reduce `count` per-tile partials starting at tile `base` with the whole workgroup in a
 // fixed order (result valid in every thread).
 __device__ __forceinline__ double tile_sum(const double* part, int base, int count, int stride,
                                            int slot, double* red) {
-  double v = 0.0;
-  for (int k = threadIdx.x; k < count; k += kBlock) v += part[(size_t)(base + k) * stride + slot];
+  const double v = lane_sum(part + static_cast<size_t>(base) * stride + slot, count, stride, 0.0);
   return block_sum(v, red);
 }
 
@@ -571,10 +592,10 @@ __device__ __forceinline__ void decide_instance(const Args<T>& a, const int inst
   // the tiles' partials and the record are loaded together (the partials of an instance that
   // did not step are read and dropped): ||x'-x||^2, ||y'-y||^2 and the interaction as lane
   // sums over the tiles, then one reduction
-  double v[3] = {0.0, 0.0, 0.0};
-  for (int k = threadIdx.x; k < d.tc1 - d.tc0; k += kBlock) v[0] += ps_c[d.tc0 + k];
-  for (int k = threadIdx.x; k < d.ta1 - d.ta0; k += kBlock) v[1] += a.ps_a[d.ta0 + k];
-  for (int k = threadIdx.x; k < d.tt1 - d.tt0; k += kBlock) v[2] += a.ps_t[d.tt0 + k];
+  double v[3];
+  v[0] = lane_sum(ps_c + d.tc0, d.tc1 - d.tc0, 1, 0.0);
+  v[1] = lane_sum(a.ps_a + d.ta0, d.ta1 - d.ta0, 1, 0.0);
+  v[2] = lane_sum(a.ps_t + d.tt0, d.tt1 - d.tt0, 1, 0.0);
   copy_state(sm, in + inst);
   __syncthreads();
   const bool go = stepping(*sm);
@@ -976,8 +997,30 @@ __device__ __forceinline__ void eval_finalize_instance(const Args<T>& a, const i
   double v[18];
 #pragma unroll
   for (int i = 0; i < 18; ++i) v[i] = 0.0;
-  for (int k = threadIdx.x; k < d.ta1 - d.ta0; k += kBlock) {
-    const double* q = a.pe_a + (size_t)(d.ta0 + k) * 8;
+  // each lane takes its tiles in order, kEvalBatch tiles' records loaded before they are added
+  // (as lane_sum; a position past the table reloads its last record and is not added)
+  constexpr int kEvalBatch = 4;
+  const auto fold = [&](const double* recs, const int count, auto add) {
+    for (int k0 = static_cast<int>(threadIdx.x); k0 < count; k0 += kEvalBatch * kBlock) {
+      double2 w[kEvalBatch][4];
+#pragma unroll
+      for (int u = 0; u < kEvalBatch; ++u) {
+        const int k = k0 + u * kBlock;
+        const double2* q = reinterpret_cast<const double2*>(recs + (size_t)(k < count ? k : count - 1) * 8);
+#pragma unroll
+        for (int h = 0; h < 4; ++h) w[u][h] = q[h];
+      }
+#pragma unroll
+      for (int u = 0; u < kEvalBatch; ++u) {
+        if (k0 + u * kBlock < count) {
+          const double q[8] = {w[u][0].x, w[u][0].y, w[u][1].x, w[u][1].y,
+                               w[u][2].x, w[u][2].y, w[u][3].x, w[u][3].y};
+          add(q);
+        }
+      }
+    }
+  };
+  fold(a.pe_a + (size_t)d.ta0 * 8, d.ta1 - d.ta0, [&](const double* q) {
 #pragma unroll
     for (int c = 0; c < 2; ++c) {
       v[8 * c + 0] += q[4 * c + 0];
@@ -985,9 +1028,8 @@ __device__ __forceinline__ void eval_finalize_instance(const Args<T>& a, const i
       v[8 * c + 2] += q[4 * c + 2];
       v[8 * c + 3] = dev_max(v[8 * c + 3], q[4 * c + 3]);
     }
-  }
-  for (int k = threadIdx.x; k < d.tt1 - d.tt0; k += kBlock) {
-    const double* q = a.pe_t + (size_t)(d.tt0 + k) * 8;
+  });
+  fold(a.pe_t + (size_t)d.tt0 * 8, d.tt1 - d.tt0, [&](const double* q) {
 #pragma unroll
     for (int c = 0; c < 2; ++c) {
       v[8 * c + 4] += q[4 * c + 0];
@@ -995,10 +1037,10 @@ __device__ __forceinline__ void eval_finalize_instance(const Args<T>& a, const i
       v[8 * c + 6] = dev_max(v[8 * c + 6], q[4 * c + 2]);
       v[8 * c + 7] += q[4 * c + 3];
     }
-  }
+  });
   if (!a.report) {
-    for (int k = threadIdx.x; k < d.tc1 - d.tc0; k += kBlock) v[16] += a.pe_c[(size_t)(d.tc0 + k) * 2];
-    for (int k = threadIdx.x; k < d.tr1 - d.tr0; k += kBlock) v[17] += a.pe_r[(size_t)(d.tr0 + k) * 2];
+    v[16] = lane_sum(a.pe_c + (size_t)d.tc0 * 2, d.tc1 - d.tc0, 2, v[16]);
+    v[17] = lane_sum(a.pe_r + (size_t)d.tr0 * 2, d.tr1 - d.tr0, 2, v[17]);
   }
   block_reduce_n<18, (1u << 3) | (1u << 6) | (1u << 11) | (1u << 14)>(v, red);
   if (threadIdx.x == 0) {
