@@ -155,6 +155,16 @@ constexpr int32_t kFusedMaxTiles = 4;
 constexpr int32_t kFusedMaxChunks = 2;
 constexpr int32_t kResidentRounds = 4;
 
+// A step decided inside the next primal launch is decided by every column chunk of the
+// instance, each reading all of the instance's per-tile partials. That costs chunks x partials
+// reads per launch, which grows with the square of an instance's size: a batch whose sum of it
+// over the instances exceeds this many reads decides every step in a launch of its own
+// (k_step_finalize, one workgroup per instance) instead. Measured on the 2M x 5M LP (4,883
+// chunks, ~24,400 partials, 119M reads): a deciding primal launch 99 us against 28 us for the
+// plain one plus 28 us for the decision's launch; on the 200k x 500k LP (1.5M reads) folding
+// still saves ~2 us per step.
+constexpr int64_t kFoldMaxReads = int64_t{1} << 24;
+
 // Column order on the device. For an instance of at least kRelabelColumns columns (mode 0) or
 // any instance (mode 1), its kHotBytes / w columns with the most nonzeros (the x-bar entries
 // that the gathers of A x-bar touch most often) come first, largest first with ties in index
@@ -374,6 +384,7 @@ class EngineImpl final : public Engine {
   int stage_ = 1;
   int32_t cap_ = kDefaultTileNnz;  // row-block tile capacity (nonzeros)
   bool fused_ = false;  // the resident solve (k_resident): every instance has few tiles
+  bool fold_ = true;    // steps decided in the next primal launch (kFoldMaxReads)
   bool pack_ = true;    // packed index words in the staging order
   int32_t cus_ = 256;   // compute units of the device
 
@@ -680,6 +691,12 @@ int EngineImpl<T>::upload() {
   fused_ = (opt_.step_fusion & 1) == 0 && K_ > 0;
   for (const InstDesc& d : desc_)
     fused_ = fused_ && (d.ta1 - d.ta0) + (d.tt1 - d.tt0) <= kFusedMaxTiles && d.tc1 - d.tc0 <= kFusedMaxChunks;
+  int64_t fold_reads = 0;
+  for (const InstDesc& d : desc_) {
+    const int64_t chunks = d.tc1 - d.tc0;
+    fold_reads += chunks * (chunks + (d.ta1 - d.ta0) + (d.tt1 - d.tt0));
+  }
+  fold_ = (opt_.step_fusion & 2) == 0 && fold_reads <= kFoldMaxReads;
   n_tc_ = static_cast<int32_t>(tc.size());
   n_tr_ = static_cast<int32_t>(tr.size());
 
@@ -1078,7 +1095,7 @@ int EngineImpl<T>::enqueue_block() {
   // partials are in buffer k % 2, so the launch deciding step k never overwrites them.
   std::vector<uint8_t> alone(static_cast<size_t>(B), 1);
   int switches = 0;
-  if ((opt_.step_fusion & 2) == 0) {
+  if (fold_) {
     for (int k = 0; k < B; ++k) alone[static_cast<size_t>(k)] = eval_after(k) ? 1 : 0;
     for (int k = 0; k + 1 < B; ++k) switches += alone[static_cast<size_t>(k)] ? 0 : 1;
     for (int k = B - 2; k >= 0 && switches % 2; --k) {
