@@ -649,21 +649,46 @@ int EngineImpl<T>::upload() {
     // scatter (sparse.cpp:202-212) and its row statistics (scaling.cpp:44-100) add a row's
     // terms, so row sums (scaling, A x-bar, the evaluation's activities) do not depend on the
     // device order. The staging order sorts by device column anyway, for the gathers.
+    // The scatter reads the LP's own column-major arrays in order (its columns ascending, each
+    // column's rows as stored) and writes each entry to the next free slot of its row. A large
+    // instance alone in the batch is scattered by several host threads, each owning a range of
+    // rows and skipping the entries of the others: every row still receives its entries in the
+    // same order, so the arrays are the same as one thread's.
     std::vector<int32_t>& next = h.next;
     next.assign(a_ptr.begin(), a_ptr.end() - 1);
+    const unsigned hw = std::thread::hardware_concurrency();
+    const int32_t row_threads = K_ == 1 ? static_cast<int32_t>(std::min<int64_t>(
+                                              {8, hw ? static_cast<int64_t>(hw) : 1,
+                                               static_cast<int64_t>(Z >> 17) + 1}))
+                                        : 1;
     for_each_instance(K_, Z, &h.dev_of, [&](int32_t k, std::vector<int32_t>& dev_of) {
+      const HostLp& lp = *lps_[static_cast<size_t>(k)];
       const InstDesc& d = desc_[static_cast<size_t>(k)];
       const std::vector<int32_t>& perm = col_perm_[static_cast<size_t>(k)];  // device -> LP
       dev_of.assign(perm.size(), 0);
       for (size_t j = 0; j < perm.size(); ++j) dev_of[static_cast<size_t>(perm[j])] = static_cast<int32_t>(j);
-      for (int32_t o = 0; o < d.cols; ++o) {
-        const auto j = static_cast<size_t>(d.col0 + (perm.empty() ? o : dev_of[static_cast<size_t>(o)]));
-        for (int32_t p = t_ptr[j]; p < t_ptr[j + 1]; ++p) {
-          const auto slot = static_cast<size_t>(next[static_cast<size_t>(t_idx[static_cast<size_t>(p)])]++);
-          a_idx[slot] = static_cast<int32_t>(j);
-          a_orig[slot] = t_orig[static_cast<size_t>(p)];
+      auto scatter = [&](int32_t r_lo, int32_t r_hi) {
+        for (int32_t o = 0; o < d.cols; ++o) {
+          const int32_t j = d.col0 + (perm.empty() ? o : dev_of[static_cast<size_t>(o)]);
+          for (int32_t p = lp.col_starts[o]; p < lp.col_starts[o + 1]; ++p) {
+            const int32_t r = lp.row_indices[p];
+            if (r < r_lo || r >= r_hi) continue;
+            const auto slot = static_cast<size_t>(next[static_cast<size_t>(d.row0 + r)]++);
+            a_idx[slot] = j;
+            a_orig[slot] = lp.values[p];
+          }
         }
+      };
+      if (row_threads <= 1) {
+        scatter(0, d.rows);
+        return;
       }
+      std::vector<std::thread> pool;
+      for (int32_t t = 1; t < row_threads; ++t)
+        pool.emplace_back(scatter, static_cast<int32_t>(static_cast<int64_t>(d.rows) * t / row_threads),
+                          static_cast<int32_t>(static_cast<int64_t>(d.rows) * (t + 1) / row_threads));
+      scatter(0, static_cast<int32_t>(d.rows / row_threads));
+      for (std::thread& th : pool) th.join();
     });
   }
   // Staging order of the SpMV tiles: sorted by gather index unless told otherwise.
