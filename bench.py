@@ -517,6 +517,7 @@ def main() -> int:
                    "rows": stats["total_rows"], "cols": stats["total_cols"],
                    "nnz": stats["total_nnz"], "sub_batches": stats["streams"],
                    "tile_nnz": stats["tile_nnz"], "fused_step": bool(stats["fused"]),
+                   "column_slices": stats["column_slices"],
                    "tolerance": 1e-4,
                    "stop": "relative KKT (primal, dual, gap) <= 1e-4, FP64 on the original model"},
         "pdhg_iters_per_s": iterations * args.steps / total,

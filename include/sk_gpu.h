@@ -276,6 +276,12 @@ typedef struct {
                                 (solve.cpp:423-443); 1 makes sk_pdhg_solve do the same:
                                 sk_presolve, the engine on the reduced LP, sk_postsolve, and
                                 the answer measured on the original LP */
+  int32_t column_slices;     /* no counterpart in the reference: A x-bar of a solve or batch of
+                                one LP as 8 partial products over column slices, one per XCD,
+                                so that the x-bar entries each gathers touch stay in that XCD's
+                                L2; 0 automatic (default: an LP of at least 2^22 columns), 1 on,
+                                2 off. A sliced LP keeps its own column order on the device, and
+                                its row sums add the slices' partials in slice order */
 } sk_pdhg_options;
 
 /* The defaults listed above (options.cpp:83-106, 220-236, 284-290, 328-334, 395-418). */
@@ -555,7 +561,8 @@ typedef struct {
                        sk_pdhg_options.tile_nnz is -1) */
   int32_t fused;    /* 1 when the batch runs as the resident solve: many steps and
                        evaluations per launch (sk_pdhg_options.step_fusion) */
-  int32_t reserved;
+  int32_t column_slices; /* column slices A x-bar runs over (sk_pdhg_options.column_slices),
+                            0 when it runs unsliced */
 } sk_batch_stats;
 
 int32_t sk_batch_create(const sk_pdhg_options* options, sk_batch** out);

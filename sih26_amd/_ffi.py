@@ -45,7 +45,7 @@ class PdhgOptions(C.Structure):
                 ("use_graph", i32), ("spmv_stage", i32), ("tile_nnz", i32),
                 ("compute_quality", i32), ("eval_stride", i32), ("streams", i32),
                 ("column_order", i32), ("step_fusion", i32), ("backend", i32), ("gpu", i32),
-                ("presolve", i32)]
+                ("presolve", i32), ("column_slices", i32)]
 
 
 class ProblemCharacteristics(C.Structure):
@@ -93,7 +93,7 @@ class BatchStats(C.Structure):
                 ("graph_launches", i64), ("total_rows", i64), ("total_cols", i64),
                 ("total_nnz", i64), ("instances", i32), ("spmv_stage", i32),
                 ("streams", i32), ("tile_nnz", i32),
-                ("fused", i32), ("reserved", i32)]
+                ("fused", i32), ("column_slices", i32)]
 
 
 class ServerOptions(C.Structure):

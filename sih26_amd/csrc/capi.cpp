@@ -81,6 +81,7 @@ sk::EngineOptions to_engine(const sk_pdhg_options& o) {
   e.streams = o.streams;
   e.column_order = o.column_order == 1 || o.column_order == 2 ? o.column_order : 0;
   e.step_fusion = o.step_fusion & 3;
+  e.column_slices = o.column_slices == 1 || o.column_slices == 2 ? o.column_slices : 0;
   return e;
 }
 
@@ -511,6 +512,7 @@ void sk_pdhg_default_options(sk_pdhg_options* o) {
   o->eval_stride = 20;
   o->streams = 0;
   o->column_order = 0;
+  o->column_slices = 0;
 }
 
 int32_t sk_pdhg_set_option(sk_pdhg_options* o, const char* key_in, const char* value_in) {
@@ -938,7 +940,7 @@ int32_t sk_batch_stats_get(const sk_batch* b, sk_batch_stats* out) {
   out->streams = b->engine->streams();
   out->tile_nnz = b->tile_nnz;
   out->fused = b->engine->fused();
-  out->reserved = 0;
+  out->column_slices = b->engine->column_slices();
   return SK_GPU_SUCCESS;
 }
 

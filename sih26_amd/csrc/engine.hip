@@ -165,6 +165,15 @@ constexpr int32_t kResidentRounds = 4;
 // still saves ~2 us per step.
 constexpr int64_t kFoldMaxReads = int64_t{1} << 24;
 
+// Column slices (EngineOptions::column_slices): one per XCD, used automatically for a batch of
+// one instance of at least kSliceMinCols columns. Slice s holds the columns
+// [s * n / kColumnSlices, (s + 1) * n / kColumnSlices) of the LP's column order. Measured on the
+// 2M x 5M LP as a batch of its 8 column slices (each slice's tiles on one XCD): A x-bar 228 us
+// against 262 us in the most-referenced-first order and 359 us in the LP's order, with the dual
+// step's per-row work done eight times over.
+constexpr int32_t kColumnSlices = 8;
+constexpr int32_t kSliceMinCols = 1 << 22;
+
 // Column order on the device. For an instance of at least kRelabelColumns columns (mode 0) or
 // any instance (mode 1), its kHotBytes / w columns with the most nonzeros (the x-bar entries
 // that the gathers of A x-bar touch most often) come first, largest first with ties in index
@@ -293,6 +302,7 @@ class EngineImpl final : public Engine {
   int stage_choice() const override { return stage_; }
   int32_t streams() const override { return 1; }
   int32_t fused() const override { return fused_ ? 1 : 0; }
+  int32_t column_slices() const override { return slices_; }
 
   int upload() override;
   int solve() override;
@@ -340,6 +350,59 @@ class EngineImpl final : public Engine {
     return kStatusSuccess;
   }
 
+  // S, the stacked column slices of the batch's one instance: S row k * m + i holds the entries
+  // of row i of A whose column lies in slice k, in CSR(A) order (columns ascending, since a
+  // sliced instance keeps the LP's column order), with s_src the CSR(A) position of each. Its
+  // row-block tiles go slice by slice, each slice's part of the table padded with empty tiles
+  // to the longest part, so that xcd_tile gives XCD k exactly the tiles of slice k.
+  void build_slices(int32_t cap) {
+    auto& h = stage_host_;
+    const InstDesc& d = desc_[0];
+    const int32_t m = d.rows, n = d.cols, S = slices_;
+    const auto M = static_cast<size_t>(m);
+    std::vector<int32_t> bound(static_cast<size_t>(S) + 1);
+    for (int32_t k = 0; k <= S; ++k)
+      bound[static_cast<size_t>(k)] = d.col0 + static_cast<int32_t>(static_cast<int64_t>(n) * k / S);
+    std::vector<int32_t>& sp = h.s_ptr;
+    sp.assign(static_cast<size_t>(S) * M + 1, 0);
+    // a row's entries of slice k are one run of it (columns ascending); rows walk the bounds
+    auto walk = [&](int32_t i, auto&& f) {
+      int32_t k = 0;
+      for (int32_t p = h.a_ptr[static_cast<size_t>(d.row0 + i)]; p < h.a_ptr[static_cast<size_t>(d.row0 + i) + 1]; ++p) {
+        while (h.a_idx[static_cast<size_t>(p)] >= bound[static_cast<size_t>(k) + 1]) ++k;
+        f(k, p);
+      }
+    };
+    for (int32_t i = 0; i < m; ++i)
+      walk(i, [&](int32_t k, int32_t) { ++sp[static_cast<size_t>(k) * M + static_cast<size_t>(i) + 1]; });
+    for (size_t r = 0; r + 1 < sp.size(); ++r) sp[r + 1] += sp[r];
+    const auto Z = static_cast<size_t>(sp.back());
+    h.s_idx.resize(Z);
+    h.s_src.resize(Z);
+    std::vector<int32_t>& next = h.next;
+    next.assign(sp.begin(), sp.end() - 1);
+    for (int32_t i = 0; i < m; ++i)
+      walk(i, [&](int32_t k, int32_t p) {
+        const auto q = static_cast<size_t>(next[static_cast<size_t>(k) * M + static_cast<size_t>(i)]++);
+        h.s_idx[q] = h.a_idx[static_cast<size_t>(p)];
+        h.s_src[q] = p;
+      });
+    std::vector<std::vector<Tile>> part(static_cast<size_t>(S));
+    size_t longest = 0;
+    for (int32_t k = 0; k < S; ++k) {
+      build_row_tiles(sp, k * m, m, 0, cap, d.col0, &part[static_cast<size_t>(k)]);
+      longest = std::max(longest, part[static_cast<size_t>(k)].size());
+    }
+    h.ts.clear();
+    for (int32_t k = 0; k < S; ++k) {
+      const std::vector<Tile>& tk = part[static_cast<size_t>(k)];
+      h.ts.insert(h.ts.end(), tk.begin(), tk.end());
+      const int32_t e = (k + 1) * m;
+      const int32_t pe = sp[static_cast<size_t>(e)];
+      for (size_t u = tk.size(); u < longest; ++u) h.ts.push_back(Tile{e, e, 0, cap, pe, pe, d.col0, kTileShort});
+    }
+  }
+
   int setup_device();
   int enqueue_setup();
   int capture(int (EngineImpl<T>::*body)(), hipGraphExec_t* exec);
@@ -350,14 +413,21 @@ class EngineImpl final : public Engine {
   int launch_block();
   // One step SpMV launch (1: A xbar + dual step, 2: A^T y' + interaction), instantiated for
   // the staging capacity the tile tables were built with.
+  // With column slices, A x-bar is the slices' product followed by their sum and the dual step.
   int launch_spmv(int kernel, const Args<T>& a) {
     hipStream_t s = stream_;
     const unsigned grid = static_cast<unsigned>(kernel == 1 ? n_ta_ : n_tt_);
     return with_layout([&](auto cap, auto pack) -> int {
       constexpr int C = decltype(cap)::value;
       constexpr bool P = decltype(pack)::value;
-      if (kernel == 1) SK_LAUNCH(k_step_dual<T, C, P><<<grid, kStepLanes, 0, s>>>(a));
-      else SK_LAUNCH(k_step_aty<T, C, P><<<grid, kStepLanes, 0, s>>>(a));
+      if (kernel == 1 && slices_) {
+        SK_LAUNCH(k_slice_product<T, C, P><<<static_cast<unsigned>(n_ts_), kStepLanes, 0, s>>>(a));
+        SK_LAUNCH(k_slice_dual<T><<<static_cast<unsigned>(n_tr_), kBlock, 0, s>>>(a));
+      } else if (kernel == 1) {
+        SK_LAUNCH(k_step_dual<T, C, P><<<grid, kStepLanes, 0, s>>>(a));
+      } else {
+        SK_LAUNCH(k_step_aty<T, C, P><<<grid, kStepLanes, 0, s>>>(a));
+      }
       return kStatusSuccess;
     });
   }
@@ -386,13 +456,14 @@ class EngineImpl final : public Engine {
   bool fused_ = false;  // the resident solve (k_resident): every instance has few tiles
   bool fold_ = true;    // steps decided in the next primal launch (kFoldMaxReads)
   bool pack_ = true;    // packed index words in the staging order
+  int32_t slices_ = 0;  // column slices of A x-bar (kColumnSlices), or 0
   int32_t cus_ = 256;   // compute units of the device
 
   // host staging of an upload (the sources of its copies), reused by the next one
   struct HostStage {
-    std::vector<int32_t> a_ptr, a_idx, t_ptr, t_idx, next, dev_of;
+    std::vector<int32_t> a_ptr, a_idx, t_ptr, t_idx, next, dev_of, s_ptr, s_idx, s_src;
     std::vector<double> a_orig, t_orig, cmin, cl, cu, rl, ru, draw;
-    std::vector<Tile> ta, tt, tc, tr;
+    std::vector<Tile> ta, tt, tc, tr, ts;
     std::vector<InstDesc> cdesc;
   };
   HostStage stage_host_;
@@ -403,7 +474,7 @@ class EngineImpl final : public Engine {
   std::vector<double> v0_;  // power-iteration start, all instances
   std::vector<double> wx_host_, wy_host_;  // warm start in the device order (copy sources)
   std::vector<uint8_t> trivial_norm_;
-  int32_t n_ta_ = 0, n_tt_ = 0, n_tc_ = 0, n_tr_ = 0;
+  int32_t n_ta_ = 0, n_tt_ = 0, n_tc_ = 0, n_tr_ = 0, n_ts_ = 0;
   uint32_t initially_done_ = 0;
   bool setup_done_ = false;  // the device set-up of the uploaded batch has been enqueued
 
@@ -424,6 +495,7 @@ class EngineImpl final : public Engine {
   DevBuf b_done_, b_v_, b_v0_, b_av_, b_len_, b_zero_, b_fa_, b_ft_, b_fc_, b_fr_;
   DevBuf b_a_orig_st_, b_t_orig_st_;
   DevBuf b_wx_, b_wy_;  // warm start, device order, NaN where cold
+  DevBuf b_s_ptr_, b_s_idx_, b_s_src_, b_s_sidx_, b_s_pos_, b_s_val_, b_ts_, b_xpart_, b_ps_r_;
   std::vector<DevBuf*> buffers() {
     return {&b_a_ptr_, &b_a_idx_, &b_a_orig_, &b_t_ptr_, &b_t_idx_, &b_t_orig_, &b_sa_, &b_st_,
             &b_a_val_, &b_t_val_, &b_a_sidx_, &b_t_sidx_, &b_a_pos_, &b_t_pos_, &b_cmin_, &b_cl_, &b_cu_, &b_rl_, &b_ru_, &b_dc_, &b_dr_,
@@ -433,7 +505,8 @@ class EngineImpl final : public Engine {
             &b_yu1_, &b_bestx_, &b_besty_, &b_act_, &b_red_, &b_tc_, &b_tr_, &b_ta_, &b_tt_,
             &b_desc_, &b_cdesc_, &b_state_, &b_ps_c_, &b_ps_a_, &b_ps_t_, &b_pe_c_, &b_pe_r_, &b_pe_a_,
             &b_pe_t_, &b_done_, &b_v_, &b_v0_, &b_av_, &b_len_, &b_zero_, &b_fa_, &b_ft_,
-            &b_fc_, &b_fr_, &b_a_orig_st_, &b_t_orig_st_, &b_wx_, &b_wy_};
+            &b_fc_, &b_fr_, &b_a_orig_st_, &b_t_orig_st_, &b_wx_, &b_wy_, &b_s_ptr_, &b_s_idx_,
+            &b_s_src_, &b_s_sidx_, &b_s_pos_, &b_s_val_, &b_ts_, &b_xpart_, &b_ps_r_};
   }
   std::vector<const DevBuf*> buffers() const {
     auto all = const_cast<EngineImpl<T>*>(this)->buffers();
@@ -443,6 +516,7 @@ class EngineImpl final : public Engine {
   Args<T> args_{};
   double* d_sa_ = nullptr;
   double* d_st_ = nullptr;
+  const int32_t* s_src_dev_ = nullptr;  // S's CSR(A) positions (build_slices)
   double* d_rpass_ = nullptr;
   double* d_cpass_ = nullptr;
   double* d_lhat64_ = nullptr;
@@ -501,6 +575,11 @@ int EngineImpl<T>::upload() {
     set_last_error("batch exceeds 32-bit row, column or nonzero offsets");
     return kStatusInvalidArgument;
   }
+  slices_ = K_ == 1 && opt_.column_slices != 2 && lps_[0]->num_cols >= kColumnSlices &&
+                    (opt_.column_slices == 1 || lps_[0]->num_cols >= kSliceMinCols)
+                ? kColumnSlices
+                : 0;
+  if (static_cast<int64_t>(slices_) * m_ >= INT32_MAX) slices_ = 0;
   const auto M = static_cast<size_t>(m_), N = static_cast<size_t>(n_), Z = static_cast<size_t>(nnz_);
 
   // Block-diagonal CSR(A) (columns ascending in each row, the order of the reference's CSC
@@ -562,7 +641,7 @@ int EngineImpl<T>::upload() {
     // counts per row
     for (int32_t p = 0; p < lp.nnz(); ++p) ++a_ptr[static_cast<size_t>(row0 + lp.row_indices[p]) + 1];
     std::vector<int32_t>& perm = col_perm_[static_cast<size_t>(k)];
-    perm = relabel_columns(lp, sizeof(T), opt_.column_order);
+    perm = relabel_columns(lp, sizeof(T), slices_ ? 2 : opt_.column_order);
     auto lp_col = [&](int32_t j) { return perm.empty() ? j : perm[static_cast<size_t>(j)]; };
     int32_t nz = nz0;
     if (k == 0) t_ptr[0] = 0;  // every later start is the end its predecessor writes
@@ -711,9 +790,13 @@ int EngineImpl<T>::upload() {
   }
   n_ta_ = static_cast<int32_t>(ta.size());
   n_tt_ = static_cast<int32_t>(tt.size());
+  std::vector<Tile>& ts = h.ts;
+  ts.clear();
+  if (slices_) build_slices(cap_);
+  n_ts_ = static_cast<int32_t>(ts.size());
   // A step of an instance with a few tiles is a chain of short dependent launches; one
   // workgroup then runs its tiles one after another in a single launch
-  fused_ = (opt_.step_fusion & 1) == 0 && K_ > 0;
+  fused_ = (opt_.step_fusion & 1) == 0 && K_ > 0 && !slices_;
   for (const InstDesc& d : desc_)
     fused_ = fused_ && (d.ta1 - d.ta0) + (d.tt1 - d.tt0) <= kFusedMaxTiles && d.tc1 - d.tc0 <= kFusedMaxChunks;
   int64_t fold_reads = 0;
@@ -870,6 +953,33 @@ int EngineImpl<T>::upload() {
     SK_LAUNCH(k_stage_values<double><<<n_tt_, kBlock, 0, stream_>>>(ptt, tp, tsx, tpos, pack, a.t_orig, tos));
     a.a_orig_st = aos;
     a.t_orig_st = tos;
+    if (slices_) {
+      // the stacked column slices: structure, staging order, and the partial row sums
+      const size_t zs = h.s_idx.size();
+      int32_t *sp, *si, *ss, *ssx;
+      uint16_t* spos;
+      Tile* pts;
+      T *sv, *xp;
+      SK_TRY(put(b_s_ptr_, h.s_ptr, &sp));
+      SK_TRY(put(b_s_idx_, h.s_idx, &si));
+      SK_TRY(put(b_s_src_, h.s_src, &ss));
+      SK_TRY(put(b_ts_, h.ts, &pts));
+      SK_TRY(alloc(b_s_sidx_, zs + kStagePad, &ssx));
+      SK_TRY(alloc(b_s_pos_, pack_ ? 0 : zs + kStagePad, &spos));
+      SK_TRY(alloc(b_s_val_, zs + kStagePad, &sv));
+      SK_TRY(alloc(b_xpart_, static_cast<size_t>(slices_) * M, &xp));
+      SK_LAUNCH(k_sort_tiles<<<n_ts_, kBlock, 0, stream_>>>(pts, sp, si, sorted, pack, ssx, spos));
+      a.ts = pts;
+      a.s_ptr = sp;
+      a.s_sidx = ssx;
+      a.s_pos = spos;
+      a.s_val = sv;
+      a.xpart = xp;
+      s_src_dev_ = ss;
+    }
+    a.n_ts = n_ts_;
+    a.slices = slices_;
+    a.slice_rows = slices_ ? static_cast<int32_t>(M) : 0;
   }
   InstDesc* pdesc;
   SK_TRY(put(b_desc_, desc_, &pdesc));
@@ -893,6 +1003,7 @@ int EngineImpl<T>::upload() {
   a.ps_c_prev = a.ps_c + n_tc_;
   SK_TRY(alloc(b_ps_a_, static_cast<size_t>(n_ta_), &a.ps_a));
   SK_TRY(alloc(b_ps_t_, static_cast<size_t>(n_tt_), &a.ps_t));
+  SK_TRY(alloc(b_ps_r_, slices_ ? static_cast<size_t>(n_tr_) : 0, &a.ps_r));
   SK_TRY(alloc(b_pe_c_, static_cast<size_t>(n_tc_) * 2, &a.pe_c));
   SK_TRY(alloc(b_pe_r_, static_cast<size_t>(n_tr_) * 2, &a.pe_r));
   SK_TRY(alloc(b_pe_a_, static_cast<size_t>(n_ta_) * 8, &a.pe_a));
@@ -995,6 +1106,9 @@ int EngineImpl<T>::enqueue_setup() {
                                                        const_cast<T*>(a.a_val)));
   SK_LAUNCH(k_stage_values<T><<<n_tt_, kBlock, 0, s>>>(a.tt, a.t_ptr, a.t_sidx, a.t_pos, pack, d_st_,
                                                        const_cast<T*>(a.t_val)));
+  if (slices_)
+    SK_LAUNCH(k_stage_slice_values<T><<<n_ts_, kBlock, 0, s>>>(a.ts, a.s_ptr, a.s_sidx, a.s_pos, pack,
+                                                               s_src_dev_, d_sa_, const_cast<T*>(a.s_val)));
   // power iteration on A^T A, 30 steps (scaling.cpp:161-194)
   SK_LAUNCH(k_copy<<<blocks_for(N), kBlock, 0, s>>>(d_v0_, d_v_, N));
   SK_LAUNCH(k_fill_i32<<<blocks_for(K_), kBlock, 0, s>>>(d_zero_seen_, 0, K_));
@@ -1598,7 +1712,12 @@ int EngineImpl<T>::probe_spmv(int32_t index, int which, const double* in, double
   st = with_layout([&](auto cap, auto pack) -> int {
     constexpr int C = decltype(cap)::value;
     constexpr bool P = decltype(pack)::value;
-    if (which == 1)
+    if (which == 1 && slices_) {
+      // the step kernels' sliced product: each slice's row sums, then their sum per row
+      SK_LAUNCH(k_probe_product<T, C, P><<<static_cast<unsigned>(n_ts_), kStepLanes, 0, s>>>(
+          a.ts, a.s_ptr, a.s_sidx, a.s_pos, a.s_val, din, a.xpart));
+      SK_LAUNCH(k_slice_sum<T><<<blocks_for(m_), kBlock, 0, s>>>(a.xpart, dout, a.slice_rows, a.slices));
+    } else if (which == 1)
       SK_LAUNCH(k_probe_product<T, C, P><<<static_cast<unsigned>(n_ta_), kStepLanes, 0, s>>>(
           a.ta, a.a_ptr, a.a_sidx, a.a_pos, a.a_val, din, dout));
     else

@@ -36,6 +36,10 @@ struct EngineOptions {
   // bit 0: never run the resident solve (k_resident, otherwise automatic when every instance
   // is small); bit 1: never let the primal launch make the previous step's decision
   int32_t step_fusion = 0;
+  // A x-bar over column slices, one per XCD (each slice's x-bar entries stay in that XCD's
+  // L2), for a batch of one instance: 0 automatic (instances of at least 2^22 columns, whose
+  // x-bar is several times an L2), 1 on, 2 off. A sliced instance keeps the LP's column order.
+  int32_t column_slices = 0;
 };
 
 // A warm start held by an engine until its next upload (Engine::warm_start).
@@ -112,6 +116,8 @@ class Engine {
   // how the uploaded batch runs: 0 separate step launches; 1 the resident solve (k_resident),
   // one launch for many steps and evaluations of every instance
   virtual int32_t fused() const = 0;
+  // column slices A x-bar runs over (EngineOptions::column_slices), 0 when unsliced
+  virtual int32_t column_slices() const = 0;
 
   // Probes for tests, valid once a solve (or start()) has run the device set-up. They return
   // what the device holds or computes, in the LP's own row and column order.

@@ -151,6 +151,21 @@ struct Args {
   int32_t force;   // timing mode: every instance steps, nothing is decided
   int32_t report;  // evaluation kernels: 0 in-loop candidates, 1 final report of best
   int32_t decide;  // primal launch: first make the previous step's decision (see k_step_primal)
+  // Column slices of A (a batch of one instance): A x-bar as `slices` partial products, one
+  // per slice of the columns, then their sum and the dual step per row chunk (k_slice_product,
+  // k_slice_dual). S row s * slice_rows + i holds row i's entries in slice s; ts is the row-block
+  // table of S, laid out so that XCD s runs slice s; xpart holds S's row sums; ps_r the row
+  // chunks' ||y'-y||^2 partials, which replace ps_a for such a batch.
+  const Tile* __restrict__ ts;
+  const int32_t* __restrict__ s_ptr;
+  const int32_t* __restrict__ s_sidx;
+  const T* __restrict__ s_val;
+  const uint16_t* __restrict__ s_pos;
+  T* xpart;
+  double* ps_r;
+  int32_t n_ts;
+  int32_t slices;      // 0: A x-bar by k_step_dual
+  int32_t slice_rows;
 };
 
 __device__ __forceinline__ bool stepping(const InstState& s) { return !s.done && !s.eval_pending; }
@@ -371,6 +386,20 @@ __device__ __forceinline__ void step_product(const Tile& t, const int32_t* __res
       gather_stager<kStepLanes, T, PACK, T, T, IN_FLIGHT>(sidx, pos, val, in), pre, epi, stage);
 }
 
+// The dual step of row i from r = (A-hat x-bar)_i (pdhg.cpp:402-408): y' into yn, the running
+// sum when the last step was accepted, and (y'_i - y_i)^2 added to d2.
+template <class T>
+__device__ __forceinline__ void dual_row(const Args<T>& a, const int i, const T yi, const T lo,
+                                         const T hi, const T sum, const bool acc, const T sigma,
+                                         const T r, T* __restrict__ yn, double& d2) {
+  if (acc) a.ysum[i] = sum + yi;
+  const T v = yi + sigma * r;
+  const T ynew = v - sigma * dev_project<T>(v / sigma, lo, hi);
+  yn[i] = ynew;
+  const double d = static_cast<double>(ynew - yi);
+  d2 += d * d;
+}
+
 // A x-bar and the dual step on the rows of tile `tile` of A (pdhg.cpp:402-408), with its
 // ||y'-y||^2 partial.
 template <class T, bool PACK, int IN_FLIGHT = 0>
@@ -394,16 +423,11 @@ __device__ __forceinline__ void dual_tile(const Args<T>& a, const int tile, T* s
         yi[k] = y[i];
         lo[k] = a.rlhat[i];
         hi[k] = a.ruhat[i];
-        if (acc) sum[k] = a.ysum[i];
+        sum[k] = acc ? a.ysum[i] : T(0);
       },
       [&](int i, int k, T r) {
         const T sigma = static_cast<T>(eta * omega);
-        if (acc) a.ysum[i] = sum[k] + yi[k];
-        const T v = yi[k] + sigma * r;
-        const T ynew = v - sigma * dev_project<T>(v / sigma, lo[k], hi[k]);
-        yn[i] = ynew;
-        const double d = static_cast<double>(ynew - yi[k]);
-        d2 += d * d;
+        dual_row(a, i, yi[k], lo[k], hi[k], sum[k], acc, sigma, r, yn, d2);
       },
       stage);
   d2 = block_sum(d2, red);
@@ -455,6 +479,76 @@ __device__ __forceinline__ double lane_sum(const double* __restrict__ part, int 
     for (int u = 0; u < kSumBatch; ++u) acc = k0 + u * kBlock < count ? acc + w[u] : acc;
   }
   return acc;
+}
+
+// A x-bar over column slices: the row sums of one tile of S (the stacked slices) into xpart.
+// XCD s runs the tiles of slice s (xcd_tile over a table of equal slice parts), so the entries
+// of x-bar its gathers touch, one slice's columns, stay in that XCD's L2.
+template <class T, int CAP, bool PACK>
+__global__ __launch_bounds__(kStepLanes) void k_slice_product(Args<T> a) {
+  __shared__ T stage[CAP];
+  // the batch holds one instance, and every tile of an instance carries the same flags
+  if (!a.force && !(a.fa[0] & kFlagStepping)) return;
+  const Tile t = a.ts[xcd_tile(blockIdx.x, gridDim.x)];
+  step_product<T, PACK>(
+      t, a.s_ptr, a.s_sidx, a.s_pos, a.s_val, a.xbar, [](int, int) {},
+      [&](int r, int, T acc) { a.xpart[r] = acc; }, stage);
+}
+
+// Row i of A x-bar from the slices' partials, added in slice order.
+template <class T>
+__device__ __forceinline__ T slice_sum(const T* __restrict__ part, const int i, const int rows,
+                                       const int slices) {
+  T r = part[i];
+  for (int k = 1; k < slices; ++k) r = r + part[static_cast<size_t>(k) * rows + i];
+  return r;
+}
+
+// The dual step of row chunk `tile` from the slices' partials (pdhg.cpp:402-408), with the
+// chunk's ||y'-y||^2 partial in ps_r.
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_slice_dual(Args<T> a) {
+  __shared__ double red[kBlock / kWave];
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const uint32_t f = a.fr[tile];
+  if (!a.force && !(f & kFlagStepping)) return;
+  const Tile t = a.tr[tile];
+  const int p = (f & kFlagParity) ? 1 : 0;
+  const bool acc = a.force || (f & kFlagAccept);
+  const T* __restrict__ y = p ? a.y1 : a.y0;
+  T* __restrict__ yn = p ? a.y0 : a.y1;
+  constexpr int U = kChunk / kBlock;
+  T yi[U], lo[U], hi[U], sum[U], r[U];
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const int i = t.begin + u * kBlock + static_cast<int>(threadIdx.x);
+    if (i < t.end) {
+      yi[u] = y[i];
+      lo[u] = a.rlhat[i];
+      hi[u] = a.ruhat[i];
+      sum[u] = acc ? a.ysum[i] : T(0);
+      r[u] = slice_sum(a.xpart, i, a.slice_rows, a.slices);
+    }
+  }
+  const InstState& s = a.state[t.inst];
+  const T sigma = static_cast<T>(s.eta * s.omega);
+  double d2 = 0.0;
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const int i = t.begin + u * kBlock + static_cast<int>(threadIdx.x);
+    if (i < t.end) dual_row(a, i, yi[u], lo[u], hi[u], sum[u], acc, sigma, r[u], yn, d2);
+  }
+  d2 = block_sum(d2, red);
+  if (threadIdx.x == 0) a.ps_r[tile] = d2;
+}
+
+// Probe of the sliced product (tests): out[i] = row i's sum of the slices' partials.
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_slice_sum(const T* __restrict__ part,
+                                                      T* __restrict__ out, int32_t rows,
+                                                      int32_t slices) {
+  const int i = static_cast<int>(blockIdx.x) * kBlock + static_cast<int>(threadIdx.x);
+  if (i < rows) out[i] = slice_sum(part, i, rows, slices);
 }
 
 // Reduce `count` per-tile partials starting at tile `base` with the whole workgroup in a
@@ -594,7 +688,8 @@ __device__ __forceinline__ void decide_instance(const Args<T>& a, const int inst
   // sums over the tiles, then one reduction
   double v[3];
   v[0] = lane_sum(ps_c + d.tc0, d.tc1 - d.tc0, 1, 0.0);
-  v[1] = lane_sum(a.ps_a + d.ta0, d.ta1 - d.ta0, 1, 0.0);
+  v[1] = a.slices ? lane_sum(a.ps_r + d.tr0, d.tr1 - d.tr0, 1, 0.0)
+                  : lane_sum(a.ps_a + d.ta0, d.ta1 - d.ta0, 1, 0.0);
   v[2] = lane_sum(a.ps_t + d.tt0, d.tt1 - d.tt0, 1, 0.0);
   copy_state(sm, in + inst);
   __syncthreads();
@@ -1378,6 +1473,27 @@ __global__ __launch_bounds__(kBlock) void k_stage_values(const Tile* __restrict_
                      : pack ? static_cast<int>(static_cast<uint32_t>(sidx[p0 + k]) & 0xFFFu)
                             : static_cast<int>(pos[p0 + k]);
     dst[p0 + k] = static_cast<T>(src[p0 + slot]);
+  }
+}
+
+// The same for the stacked column slices: S's CSR position q holds A's CSR entry map[q].
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_stage_slice_values(const Tile* __restrict__ tiles,
+                                                                const int32_t* __restrict__ ptr,
+                                                                const int32_t* __restrict__ sidx,
+                                                                const uint16_t* __restrict__ pos,
+                                                                int32_t pack,
+                                                                const int32_t* __restrict__ map,
+                                                                const double* __restrict__ src,
+                                                                T* __restrict__ dst) {
+  const Tile t = tiles[blockIdx.x];
+  const int p0 = ptr[t.begin], nnz = ptr[t.end] - p0;
+  const bool vector = t.kind == kTileVector;
+  for (int k = threadIdx.x; k < nnz; k += kBlock) {
+    const int slot = vector ? k
+                     : pack ? static_cast<int>(static_cast<uint32_t>(sidx[p0 + k]) & 0xFFFu)
+                            : static_cast<int>(pos[p0 + k]);
+    dst[p0 + k] = static_cast<T>(src[map[p0 + slot]]);
   }
 }
 

@@ -261,6 +261,12 @@ class SplitEngine final : public Engine {
     for (const auto& e : parts_) least = std::min(least, e->fused());
     return least;
   }
+  int32_t column_slices() const override {
+    // a part slices only when it holds a single instance
+    int32_t most = 0;
+    for (const auto& e : parts_) most = std::max(most, e->column_slices());
+    return most;
+  }
 
  private:
   struct Place {

@@ -14,7 +14,7 @@ order and in the most-referenced-first order. Three checks per product:
 import numpy as np
 import pytest
 
-from tests.spmv_order import error_bound, step_product
+from tests.spmv_order import error_bound, sliced_step_product, step_product
 
 pytestmark = pytest.mark.gpu
 
@@ -163,3 +163,81 @@ def test_batch_member_products_are_its_own(gpu, oracle_mod):
         scaled = oracle_mod.build_scaling(d)["values"]
         want = step_product(d, scaled, x, 1, np.float32).astype(np.float64)
         np.testing.assert_array_equal(b.probe_spmv(k, 1, x), want)
+
+
+SLICED_CASES = {
+    # A x-bar over 8 column slices (sk_pdhg_options.column_slices = 1): short-row tiles of the
+    # stacked slices (power law), long-row tiles (rows of ~125 nonzeros per slice) and vector
+    # tiles (the same rows above a capacity of 100)
+    "power_law": (power_law, {}),
+    "power_law_cap512": (power_law, {"tile_nnz": 512}),
+    "long_rows": (long_rows, {}),
+    "long_rows_cap100": (long_rows, {"tile_nnz": 100}),
+    "power_law_fp64": (power_law, {"pdhg_precision": "fp64"}),
+}
+
+
+@pytest.mark.parametrize("case", list(SLICED_CASES))
+def test_sliced_products_match_the_oracle(gpu, oracle_mod, case):
+    """The sliced A x-bar: bit for bit the restatement of its order (each slice's row sums in
+    the fixed per-row order, then the slices added in order), within the rounding bound of the
+    FP64 oracle product; A^T y' and the device scaling are those of the unsliced engine."""
+    make, opts = SLICED_CASES[case]
+    lp = make(gpu)
+    d = lp.export()
+    m, n = d["num_rows"], d["num_cols"]
+    b = solved_batch(gpu, lp, column_slices=1, **opts)
+    assert b.stats()["column_slices"] == 8
+    np.testing.assert_array_equal(b.column_order(0), np.arange(n))  # a sliced LP keeps its order
+    fp64 = opts.get("pdhg_precision") == "fp64"
+    dtype, unit = (np.float64, 2.0 ** -53) if fp64 else (np.float32, 2.0 ** -24)
+    scaled = oracle_mod.build_scaling(d)["values"]
+    np.testing.assert_array_equal(b.probe_scaling(0)["values"], scaled)
+    rng = np.random.default_rng(13)
+    x, y = rng.standard_normal(n), rng.standard_normal(m)
+
+    def mult(vals, vec, transpose):
+        return oracle_mod.multiply(dict(d, values=vals), vec, transpose=transpose)
+
+    got = b.probe_spmv(0, 1, x)
+    np.testing.assert_array_equal(got, sliced_step_product(d, scaled, x, 8, dtype).astype(np.float64))
+    exact = mult(scaled, x, False)
+    bound = error_bound(d, scaled, x, 1, unit, mult) * (2 if fp64 else 1)
+    assert np.all(np.abs(got - exact) <= bound), np.max(np.abs(got - exact) / bound)
+    np.testing.assert_array_equal(b.probe_spmv(0, 2, y),
+                                  step_product(d, scaled, y, 2, dtype).astype(np.float64))
+    if case.startswith("long_rows"):
+        cs, ri = np.asarray(d["col_starts"]), np.asarray(d["row_indices"])
+        lens = [np.bincount(ri[cs[k * n // 8]:cs[(k + 1) * n // 8]], minlength=m) for k in range(8)]
+        assert min(v.min() for v in lens) > 64  # every row of every slice is a long row
+        if "cap100" in case:
+            assert max(v.max() for v in lens) > 100  # and some are vector tiles at 100
+
+
+def test_slices_apply_to_a_batch_of_one_instance(gpu):
+    """Slicing is per batch of one LP: a batch of two runs unsliced even when asked."""
+    lps = [power_law(gpu, 5), power_law(gpu, 6)]
+    b = gpu.Batch(iteration_limit=3, column_slices=1, streams=1)
+    for lp in lps:
+        b.add(lp)
+    b.upload()
+    b.solve()
+    assert b.stats()["column_slices"] == 0
+    one = solved_batch(gpu, lps[0], column_slices=2)
+    assert one.stats()["column_slices"] == 0
+
+
+def test_sliced_solve_meets_the_request_and_repeats(gpu, oracle_mod):
+    """A solve with A x-bar over column slices: it meets the request as the oracle measures
+    the answer, and a second solve gives the same bits (every sum has a fixed order)."""
+    lp = power_law(gpu)
+    d = lp.export()
+    opts = dict(pdhg_tolerance=1e-4, pdhg_stop_at_request=True, pdhg_absolute_primal=False,
+                column_slices=1)
+    a, b = gpu.solve(lp, **opts), gpu.solve(lp, **opts)
+    assert a.converged, a.info["message"]
+    ev = oracle_mod.evaluate(d, a.col_value, -d["sense"] * a.row_dual)
+    assert ev["worst"] <= 1e-4 * (1 + 1e-9), ev
+    assert a.iterations == b.iterations
+    np.testing.assert_array_equal(a.col_value, b.col_value)
+    np.testing.assert_array_equal(a.row_dual, b.row_dual)

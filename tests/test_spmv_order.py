@@ -2,7 +2,8 @@
 order that the GPU probe tests hold the step kernels to."""
 import numpy as np
 
-from tests.spmv_order import csr_rows, error_bound, fixed_order_sums, step_product
+from tests.spmv_order import (csr_rows, error_bound, fixed_order_sums, sliced_step_product,
+                              step_product)
 
 
 def random_csc(rng, m, n, per_col):
@@ -94,3 +95,26 @@ def test_error_bound_holds_for_float32_in_any_order(oracle_mod):
     assert np.all(np.abs(got - exact) <= bound)
     # and the bound is not vacuous: float32 rounding is visible against it
     assert np.max(np.abs(got - exact) / bound) > 1e-3
+
+
+def test_one_slice_is_the_unsliced_order_and_eight_are_slice_sums():
+    # one slice is every row whole: the unsliced order bit for bit; eight slices are, per row,
+    # the fixed-order sums of its entries in each column range, added in range order
+    rng = np.random.default_rng(17)
+    lp = random_csc(rng, 40, 400, lambda: rng.integers(1, 12))
+    x = rng.standard_normal(400)
+    np.testing.assert_array_equal(sliced_step_product(lp, lp["values"], x, 1, np.float32),
+                                  step_product(lp, lp["values"], x, 1, np.float32))
+    a = dense(lp).astype(np.float32)
+    xf = x.astype(np.float32)
+    got = sliced_step_product(lp, lp["values"], x, 8, np.float32)
+    bounds = [k * 400 // 8 for k in range(9)]
+    for i in range(40):
+        total = None
+        for k in range(8):
+            part = np.float32(0)
+            for j in range(bounds[k], bounds[k + 1]):  # each slice holds < 64 entries of the row
+                if a[i, j] != 0:
+                    part = np.float32(part + np.float32(a[i, j] * xf[j]))
+            total = part if total is None else np.float32(total + part)
+        assert got[i] == total, i
