@@ -173,6 +173,11 @@ constexpr int64_t kFoldMaxReads = int64_t{1} << 24;
 // step's per-row work done eight times over.
 constexpr int32_t kColumnSlices = 8;
 constexpr int32_t kSliceMinCols = 1 << 22;
+// The same batches run A^T y' over kRowSlices slices of A's rows (y' entries), two XCDs per
+// slice. Measured as a batch of the 2M x 5M LP's 4 row slices: A^T y' 247 us against 289 us
+// unsliced (LP column order), with the interaction's per-column work done four times over; 8
+// slices (1 MB of y' each, but 40M partial sums): 322 us.
+constexpr int32_t kRowSlices = 4;
 
 // Column order on the device. For an instance of at least kRelabelColumns columns (mode 0) or
 // any instance (mode 1), its kHotBytes / w columns with the most nonzeros (the x-bar entries
@@ -350,56 +355,59 @@ class EngineImpl final : public Engine {
     return kStatusSuccess;
   }
 
-  // S, the stacked column slices of the batch's one instance: S row k * m + i holds the entries
-  // of row i of A whose column lies in slice k, in CSR(A) order (columns ascending, since a
-  // sliced instance keeps the LP's column order), with s_src the CSR(A) position of each. Its
-  // row-block tiles go slice by slice, each slice's part of the table padded with empty tiles
-  // to the longest part, so that xcd_tile gives XCD k exactly the tiles of slice k.
-  void build_slices(int32_t cap) {
-    auto& h = stage_host_;
-    const InstDesc& d = desc_[0];
-    const int32_t m = d.rows, n = d.cols, S = slices_;
-    const auto M = static_cast<size_t>(m);
+  // Stacked slices of one CSR table of the batch's one instance: row k * rows + r of the
+  // result holds the entries of the table's row row0 + r whose gather index lies in slice k of
+  // [g0, g0 + count) (slice k: [g0 + k count / S, g0 + (k + 1) count / S)), in the table's order,
+  // with src the table position of each. A row's entries of one slice are one run of it, since
+  // the gather indices ascend in every row (CSR(A) of a sliced instance keeps the LP's column
+  // order; CSR(A^T) holds rows ascending). The row-block tiles go slice by slice, each slice's
+  // part padded with empty tiles to one length that its 8 / S XCDs share evenly, so that
+  // xcd_tile gives slice k's tiles to XCDs k 8 / S, ..., (k + 1) 8 / S - 1.
+  void build_stacked(const std::vector<int32_t>& ptr, const std::vector<int32_t>& idx,
+                     int32_t row0, int32_t rows, int32_t g0, int32_t count, int32_t S,
+                     int32_t cap, std::vector<int32_t>& sp, std::vector<int32_t>& sidx,
+                     std::vector<int32_t>& src, std::vector<Tile>& tiles) {
+    const auto R = static_cast<size_t>(rows);
     std::vector<int32_t> bound(static_cast<size_t>(S) + 1);
     for (int32_t k = 0; k <= S; ++k)
-      bound[static_cast<size_t>(k)] = d.col0 + static_cast<int32_t>(static_cast<int64_t>(n) * k / S);
-    std::vector<int32_t>& sp = h.s_ptr;
-    sp.assign(static_cast<size_t>(S) * M + 1, 0);
-    // a row's entries of slice k are one run of it (columns ascending); rows walk the bounds
-    auto walk = [&](int32_t i, auto&& f) {
+      bound[static_cast<size_t>(k)] = g0 + static_cast<int32_t>(static_cast<int64_t>(count) * k / S);
+    sp.assign(static_cast<size_t>(S) * R + 1, 0);
+    auto walk = [&](int32_t r, auto&& f) {
       int32_t k = 0;
-      for (int32_t p = h.a_ptr[static_cast<size_t>(d.row0 + i)]; p < h.a_ptr[static_cast<size_t>(d.row0 + i) + 1]; ++p) {
-        while (h.a_idx[static_cast<size_t>(p)] >= bound[static_cast<size_t>(k) + 1]) ++k;
+      for (int32_t p = ptr[static_cast<size_t>(row0 + r)]; p < ptr[static_cast<size_t>(row0 + r) + 1]; ++p) {
+        while (idx[static_cast<size_t>(p)] >= bound[static_cast<size_t>(k) + 1]) ++k;
         f(k, p);
       }
     };
-    for (int32_t i = 0; i < m; ++i)
-      walk(i, [&](int32_t k, int32_t) { ++sp[static_cast<size_t>(k) * M + static_cast<size_t>(i) + 1]; });
-    for (size_t r = 0; r + 1 < sp.size(); ++r) sp[r + 1] += sp[r];
+    for (int32_t r = 0; r < rows; ++r)
+      walk(r, [&](int32_t k, int32_t) { ++sp[static_cast<size_t>(k) * R + static_cast<size_t>(r) + 1]; });
+    for (size_t q = 0; q + 1 < sp.size(); ++q) sp[q + 1] += sp[q];
     const auto Z = static_cast<size_t>(sp.back());
-    h.s_idx.resize(Z);
-    h.s_src.resize(Z);
-    std::vector<int32_t>& next = h.next;
+    sidx.resize(Z);
+    src.resize(Z);
+    std::vector<int32_t>& next = stage_host_.next;
     next.assign(sp.begin(), sp.end() - 1);
-    for (int32_t i = 0; i < m; ++i)
-      walk(i, [&](int32_t k, int32_t p) {
-        const auto q = static_cast<size_t>(next[static_cast<size_t>(k) * M + static_cast<size_t>(i)]++);
-        h.s_idx[q] = h.a_idx[static_cast<size_t>(p)];
-        h.s_src[q] = p;
+    for (int32_t r = 0; r < rows; ++r)
+      walk(r, [&](int32_t k, int32_t p) {
+        const auto q = static_cast<size_t>(next[static_cast<size_t>(k) * R + static_cast<size_t>(r)]++);
+        sidx[q] = idx[static_cast<size_t>(p)];
+        src[q] = p;
       });
     std::vector<std::vector<Tile>> part(static_cast<size_t>(S));
+    const size_t share = static_cast<size_t>(kXcds / S);  // XCDs per slice
     size_t longest = 0;
     for (int32_t k = 0; k < S; ++k) {
-      build_row_tiles(sp, k * m, m, 0, cap, d.col0, &part[static_cast<size_t>(k)]);
+      build_row_tiles(sp, k * rows, rows, 0, cap, g0, &part[static_cast<size_t>(k)]);
       longest = std::max(longest, part[static_cast<size_t>(k)].size());
     }
-    h.ts.clear();
+    longest = (longest + share - 1) / share * share;
+    tiles.clear();
     for (int32_t k = 0; k < S; ++k) {
       const std::vector<Tile>& tk = part[static_cast<size_t>(k)];
-      h.ts.insert(h.ts.end(), tk.begin(), tk.end());
-      const int32_t e = (k + 1) * m;
+      tiles.insert(tiles.end(), tk.begin(), tk.end());
+      const int32_t e = (k + 1) * rows;
       const int32_t pe = sp[static_cast<size_t>(e)];
-      for (size_t u = tk.size(); u < longest; ++u) h.ts.push_back(Tile{e, e, 0, cap, pe, pe, d.col0, kTileShort});
+      for (size_t u = tk.size(); u < longest; ++u) tiles.push_back(Tile{e, e, 0, cap, pe, pe, g0, kTileShort});
     }
   }
 
@@ -413,7 +421,8 @@ class EngineImpl final : public Engine {
   int launch_block();
   // One step SpMV launch (1: A xbar + dual step, 2: A^T y' + interaction), instantiated for
   // the staging capacity the tile tables were built with.
-  // With column slices, A x-bar is the slices' product followed by their sum and the dual step.
+  // With slices, A x-bar is the column slices' product followed by their sum and the dual step,
+  // and A^T y' the row slices' product followed by their sum and the interaction.
   int launch_spmv(int kernel, const Args<T>& a) {
     hipStream_t s = stream_;
     const unsigned grid = static_cast<unsigned>(kernel == 1 ? n_ta_ : n_tt_);
@@ -425,6 +434,9 @@ class EngineImpl final : public Engine {
         SK_LAUNCH(k_slice_dual<T><<<static_cast<unsigned>(n_tr_), kBlock, 0, s>>>(a));
       } else if (kernel == 1) {
         SK_LAUNCH(k_step_dual<T, C, P><<<grid, kStepLanes, 0, s>>>(a));
+      } else if (slices_) {
+        SK_LAUNCH(k_rslice_product<T, C, P><<<static_cast<unsigned>(n_tq_), kStepLanes, 0, s>>>(a));
+        SK_LAUNCH(k_rslice_aty<T><<<static_cast<unsigned>(n_tc_), kBlock, 0, s>>>(a));
       } else {
         SK_LAUNCH(k_step_aty<T, C, P><<<grid, kStepLanes, 0, s>>>(a));
       }
@@ -461,9 +473,10 @@ class EngineImpl final : public Engine {
 
   // host staging of an upload (the sources of its copies), reused by the next one
   struct HostStage {
-    std::vector<int32_t> a_ptr, a_idx, t_ptr, t_idx, next, dev_of, s_ptr, s_idx, s_src;
+    std::vector<int32_t> a_ptr, a_idx, t_ptr, t_idx, next, dev_of, s_ptr, s_idx, s_src, q_ptr,
+        q_idx, q_src;
     std::vector<double> a_orig, t_orig, cmin, cl, cu, rl, ru, draw;
-    std::vector<Tile> ta, tt, tc, tr, ts;
+    std::vector<Tile> ta, tt, tc, tr, ts, tq;
     std::vector<InstDesc> cdesc;
   };
   HostStage stage_host_;
@@ -474,7 +487,7 @@ class EngineImpl final : public Engine {
   std::vector<double> v0_;  // power-iteration start, all instances
   std::vector<double> wx_host_, wy_host_;  // warm start in the device order (copy sources)
   std::vector<uint8_t> trivial_norm_;
-  int32_t n_ta_ = 0, n_tt_ = 0, n_tc_ = 0, n_tr_ = 0, n_ts_ = 0;
+  int32_t n_ta_ = 0, n_tt_ = 0, n_tc_ = 0, n_tr_ = 0, n_ts_ = 0, n_tq_ = 0;
   uint32_t initially_done_ = 0;
   bool setup_done_ = false;  // the device set-up of the uploaded batch has been enqueued
 
@@ -496,6 +509,7 @@ class EngineImpl final : public Engine {
   DevBuf b_a_orig_st_, b_t_orig_st_;
   DevBuf b_wx_, b_wy_;  // warm start, device order, NaN where cold
   DevBuf b_s_ptr_, b_s_idx_, b_s_src_, b_s_sidx_, b_s_pos_, b_s_val_, b_ts_, b_xpart_, b_ps_r_;
+  DevBuf b_q_ptr_, b_q_idx_, b_q_src_, b_q_sidx_, b_q_pos_, b_q_val_, b_tq_, b_ypart_, b_ps_q_;
   std::vector<DevBuf*> buffers() {
     return {&b_a_ptr_, &b_a_idx_, &b_a_orig_, &b_t_ptr_, &b_t_idx_, &b_t_orig_, &b_sa_, &b_st_,
             &b_a_val_, &b_t_val_, &b_a_sidx_, &b_t_sidx_, &b_a_pos_, &b_t_pos_, &b_cmin_, &b_cl_, &b_cu_, &b_rl_, &b_ru_, &b_dc_, &b_dr_,
@@ -506,7 +520,8 @@ class EngineImpl final : public Engine {
             &b_desc_, &b_cdesc_, &b_state_, &b_ps_c_, &b_ps_a_, &b_ps_t_, &b_pe_c_, &b_pe_r_, &b_pe_a_,
             &b_pe_t_, &b_done_, &b_v_, &b_v0_, &b_av_, &b_len_, &b_zero_, &b_fa_, &b_ft_,
             &b_fc_, &b_fr_, &b_a_orig_st_, &b_t_orig_st_, &b_wx_, &b_wy_, &b_s_ptr_, &b_s_idx_,
-            &b_s_src_, &b_s_sidx_, &b_s_pos_, &b_s_val_, &b_ts_, &b_xpart_, &b_ps_r_};
+            &b_s_src_, &b_s_sidx_, &b_s_pos_, &b_s_val_, &b_ts_, &b_xpart_, &b_ps_r_, &b_q_ptr_,
+            &b_q_idx_, &b_q_src_, &b_q_sidx_, &b_q_pos_, &b_q_val_, &b_tq_, &b_ypart_, &b_ps_q_};
   }
   std::vector<const DevBuf*> buffers() const {
     auto all = const_cast<EngineImpl<T>*>(this)->buffers();
@@ -516,7 +531,8 @@ class EngineImpl final : public Engine {
   Args<T> args_{};
   double* d_sa_ = nullptr;
   double* d_st_ = nullptr;
-  const int32_t* s_src_dev_ = nullptr;  // S's CSR(A) positions (build_slices)
+  const int32_t* s_src_dev_ = nullptr;  // the column slices' CSR(A) positions (build_stacked)
+  const int32_t* q_src_dev_ = nullptr;  // the row slices' CSR(A^T) positions
   double* d_rpass_ = nullptr;
   double* d_cpass_ = nullptr;
   double* d_lhat64_ = nullptr;
@@ -579,7 +595,9 @@ int EngineImpl<T>::upload() {
                     (opt_.column_slices == 1 || lps_[0]->num_cols >= kSliceMinCols)
                 ? kColumnSlices
                 : 0;
-  if (static_cast<int64_t>(slices_) * m_ >= INT32_MAX) slices_ = 0;
+  if (static_cast<int64_t>(slices_) * m_ >= INT32_MAX || static_cast<int64_t>(kRowSlices) * n_ >= INT32_MAX ||
+      m_ < kRowSlices)
+    slices_ = 0;
   const auto M = static_cast<size_t>(m_), N = static_cast<size_t>(n_), Z = static_cast<size_t>(nnz_);
 
   // Block-diagonal CSR(A) (columns ascending in each row, the order of the reference's CSC
@@ -792,7 +810,16 @@ int EngineImpl<T>::upload() {
   n_tt_ = static_cast<int32_t>(tt.size());
   std::vector<Tile>& ts = h.ts;
   ts.clear();
-  if (slices_) build_slices(cap_);
+  std::vector<Tile>& tq = h.tq;
+  tq.clear();
+  if (slices_) {
+    const InstDesc& d = desc_[0];
+    build_stacked(h.a_ptr, h.a_idx, d.row0, d.rows, d.col0, d.cols, slices_, cap_, h.s_ptr, h.s_idx,
+                  h.s_src, ts);
+    build_stacked(h.t_ptr, h.t_idx, d.col0, d.cols, d.row0, d.rows, kRowSlices, cap_, h.q_ptr,
+                  h.q_idx, h.q_src, tq);
+  }
+  n_tq_ = static_cast<int32_t>(tq.size());
   n_ts_ = static_cast<int32_t>(ts.size());
   // A step of an instance with a few tiles is a chain of short dependent launches; one
   // workgroup then runs its tiles one after another in a single launch
@@ -976,10 +1003,35 @@ int EngineImpl<T>::upload() {
       a.s_val = sv;
       a.xpart = xp;
       s_src_dev_ = ss;
+      // the stacked row slices of CSR(A^T), the same way
+      const size_t zq = h.q_idx.size();
+      int32_t *qp, *qi, *qs, *qsx;
+      uint16_t* qpos;
+      Tile* ptq;
+      T *qv, *yp;
+      SK_TRY(put(b_q_ptr_, h.q_ptr, &qp));
+      SK_TRY(put(b_q_idx_, h.q_idx, &qi));
+      SK_TRY(put(b_q_src_, h.q_src, &qs));
+      SK_TRY(put(b_tq_, h.tq, &ptq));
+      SK_TRY(alloc(b_q_sidx_, zq + kStagePad, &qsx));
+      SK_TRY(alloc(b_q_pos_, pack_ ? 0 : zq + kStagePad, &qpos));
+      SK_TRY(alloc(b_q_val_, zq + kStagePad, &qv));
+      SK_TRY(alloc(b_ypart_, static_cast<size_t>(kRowSlices) * N, &yp));
+      SK_LAUNCH(k_sort_tiles<<<n_tq_, kBlock, 0, stream_>>>(ptq, qp, qi, sorted, pack, qsx, qpos));
+      a.tq = ptq;
+      a.q_ptr = qp;
+      a.q_sidx = qsx;
+      a.q_pos = qpos;
+      a.q_val = qv;
+      a.ypart = yp;
+      q_src_dev_ = qs;
     }
     a.n_ts = n_ts_;
     a.slices = slices_;
     a.slice_rows = slices_ ? static_cast<int32_t>(M) : 0;
+    a.n_tq = n_tq_;
+    a.row_slices = slices_ ? kRowSlices : 0;
+    a.slice_cols = slices_ ? static_cast<int32_t>(N) : 0;
   }
   InstDesc* pdesc;
   SK_TRY(put(b_desc_, desc_, &pdesc));
@@ -1004,6 +1056,7 @@ int EngineImpl<T>::upload() {
   SK_TRY(alloc(b_ps_a_, static_cast<size_t>(n_ta_), &a.ps_a));
   SK_TRY(alloc(b_ps_t_, static_cast<size_t>(n_tt_), &a.ps_t));
   SK_TRY(alloc(b_ps_r_, slices_ ? static_cast<size_t>(n_tr_) : 0, &a.ps_r));
+  SK_TRY(alloc(b_ps_q_, slices_ ? static_cast<size_t>(n_tc_) : 0, &a.ps_q));
   SK_TRY(alloc(b_pe_c_, static_cast<size_t>(n_tc_) * 2, &a.pe_c));
   SK_TRY(alloc(b_pe_r_, static_cast<size_t>(n_tr_) * 2, &a.pe_r));
   SK_TRY(alloc(b_pe_a_, static_cast<size_t>(n_ta_) * 8, &a.pe_a));
@@ -1106,9 +1159,12 @@ int EngineImpl<T>::enqueue_setup() {
                                                        const_cast<T*>(a.a_val)));
   SK_LAUNCH(k_stage_values<T><<<n_tt_, kBlock, 0, s>>>(a.tt, a.t_ptr, a.t_sidx, a.t_pos, pack, d_st_,
                                                        const_cast<T*>(a.t_val)));
-  if (slices_)
+  if (slices_) {
     SK_LAUNCH(k_stage_slice_values<T><<<n_ts_, kBlock, 0, s>>>(a.ts, a.s_ptr, a.s_sidx, a.s_pos, pack,
                                                                s_src_dev_, d_sa_, const_cast<T*>(a.s_val)));
+    SK_LAUNCH(k_stage_slice_values<T><<<n_tq_, kBlock, 0, s>>>(a.tq, a.q_ptr, a.q_sidx, a.q_pos, pack,
+                                                               q_src_dev_, d_st_, const_cast<T*>(a.q_val)));
+  }
   // power iteration on A^T A, 30 steps (scaling.cpp:161-194)
   SK_LAUNCH(k_copy<<<blocks_for(N), kBlock, 0, s>>>(d_v0_, d_v_, N));
   SK_LAUNCH(k_fill_i32<<<blocks_for(K_), kBlock, 0, s>>>(d_zero_seen_, 0, K_));
@@ -1134,8 +1190,15 @@ int EngineImpl<T>::enqueue_setup() {
     const int st = with_layout([&](auto cap, auto pack) -> int {
       constexpr int C = decltype(cap)::value;
       constexpr bool P = decltype(pack)::value;
-      SK_LAUNCH(k_probe_product<T, C, P><<<static_cast<unsigned>(n_tt_), kStepLanes, 0, s>>>(
-          a.tt, a.t_ptr, a.t_sidx, a.t_pos, a.t_val, a.y0, a.aty0));
+      if (slices_) {
+        // the product the steps form: the row slices' partials, added in slice order
+        SK_LAUNCH(k_probe_product<T, C, P><<<static_cast<unsigned>(n_tq_), kStepLanes, 0, s>>>(
+            a.tq, a.q_ptr, a.q_sidx, a.q_pos, a.q_val, a.y0, a.ypart));
+        SK_LAUNCH(k_slice_sum<T><<<blocks_for(N), kBlock, 0, s>>>(a.ypart, a.aty0, a.slice_cols, a.row_slices));
+      } else {
+        SK_LAUNCH(k_probe_product<T, C, P><<<static_cast<unsigned>(n_tt_), kStepLanes, 0, s>>>(
+            a.tt, a.t_ptr, a.t_sidx, a.t_pos, a.t_val, a.y0, a.aty0));
+      }
       return kStatusSuccess;
     });
     if (st) return st;
@@ -1717,10 +1780,14 @@ int EngineImpl<T>::probe_spmv(int32_t index, int which, const double* in, double
       SK_LAUNCH(k_probe_product<T, C, P><<<static_cast<unsigned>(n_ts_), kStepLanes, 0, s>>>(
           a.ts, a.s_ptr, a.s_sidx, a.s_pos, a.s_val, din, a.xpart));
       SK_LAUNCH(k_slice_sum<T><<<blocks_for(m_), kBlock, 0, s>>>(a.xpart, dout, a.slice_rows, a.slices));
-    } else if (which == 1)
+    } else if (which == 1) {
       SK_LAUNCH(k_probe_product<T, C, P><<<static_cast<unsigned>(n_ta_), kStepLanes, 0, s>>>(
           a.ta, a.a_ptr, a.a_sidx, a.a_pos, a.a_val, din, dout));
-    else
+    } else if (slices_) {
+      SK_LAUNCH(k_probe_product<T, C, P><<<static_cast<unsigned>(n_tq_), kStepLanes, 0, s>>>(
+          a.tq, a.q_ptr, a.q_sidx, a.q_pos, a.q_val, din, a.ypart));
+      SK_LAUNCH(k_slice_sum<T><<<blocks_for(n_), kBlock, 0, s>>>(a.ypart, dout, a.slice_cols, a.row_slices));
+    } else
       SK_LAUNCH(k_probe_product<T, C, P><<<static_cast<unsigned>(n_tt_), kStepLanes, 0, s>>>(
           a.tt, a.t_ptr, a.t_sidx, a.t_pos, a.t_val, din, dout));
     return kStatusSuccess;

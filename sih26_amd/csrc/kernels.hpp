@@ -166,6 +166,21 @@ struct Args {
   int32_t n_ts;
   int32_t slices;      // 0: A x-bar by k_step_dual
   int32_t slice_rows;
+  // Row slices of A (the same batches): A^T y' as `row_slices` partial products, one per range
+  // of A's rows (y' entries), then their sum and the interaction per column chunk
+  // (k_rslice_product, k_rslice_aty). Q row k * slice_cols + j holds column j's entries in row
+  // slice k; tq is its row-block table, laid out so that XCDs 2k and 2k + 1 run slice k; ypart
+  // holds Q's row sums; ps_q the column chunks' interaction partials, which replace ps_t.
+  const Tile* __restrict__ tq;
+  const int32_t* __restrict__ q_ptr;
+  const int32_t* __restrict__ q_sidx;
+  const T* __restrict__ q_val;
+  const uint16_t* __restrict__ q_pos;
+  T* ypart;
+  double* ps_q;
+  int32_t n_tq;
+  int32_t row_slices;  // 0: A^T y' by k_step_aty
+  int32_t slice_cols;
 };
 
 __device__ __forceinline__ bool stepping(const InstState& s) { return !s.done && !s.eval_pending; }
@@ -658,6 +673,50 @@ __global__ __launch_bounds__(kStepLanes) void k_step_aty(Args<T> a) {
   aty_tile<T, PACK>(a, xcd_tile(blockIdx.x, gridDim.x), stage, red);
 }
 
+// A^T y' over row slices: the row sums of one tile of Q (the stacked row slices of CSR(A^T))
+// into ypart. XCDs 2k and 2k + 1 run the tiles of slice k, so the entries of y' their gathers
+// touch, one slice's rows, stay in their L2.
+template <class T, int CAP, bool PACK>
+__global__ __launch_bounds__(kStepLanes) void k_rslice_product(Args<T> a) {
+  __shared__ T stage[CAP];
+  // the batch holds one instance, and every tile of an instance carries the same flags
+  const uint32_t f = a.ft[0];
+  if (!a.force && !(f & kFlagStepping)) return;
+  const T* __restrict__ yn = (f & kFlagParity) ? a.y0 : a.y1;
+  const Tile t = a.tq[xcd_tile(blockIdx.x, gridDim.x)];
+  step_product<T, PACK>(
+      t, a.q_ptr, a.q_sidx, a.q_pos, a.q_val, yn, [](int, int) {},
+      [&](int r, int, T acc) { a.ypart[r] = acc; }, stage);
+}
+
+// A^T y' of column chunk `tile` from the row slices' partials, added in slice order, and the
+// chunk's share of the interaction (x'-x)^T (A^T y' - A^T y) in ps_q (pdhg.cpp:410-438).
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_rslice_aty(Args<T> a) {
+  __shared__ double red[kBlock / kWave];
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const uint32_t f = a.fc[tile];
+  if (!a.force && !(f & kFlagStepping)) return;
+  const Tile t = a.tc[tile];
+  const int p = (f & kFlagParity) ? 1 : 0;
+  const T* __restrict__ x = p ? a.x1 : a.x0;
+  const T* __restrict__ xn = p ? a.x0 : a.x1;
+  const T* __restrict__ aty = p ? a.aty1 : a.aty0;
+  T* __restrict__ atyn = p ? a.aty0 : a.aty1;
+  double inter = 0.0;
+#pragma unroll
+  for (int u = 0; u < kChunkCols; ++u) {
+    const int j = t.begin + u * kBlock + static_cast<int>(threadIdx.x);
+    if (j < t.end) {
+      const T r = slice_sum(a.ypart, j, a.slice_cols, a.row_slices);
+      atyn[j] = r;
+      inter += static_cast<double>(xn[j] - x[j]) * static_cast<double>(r - aty[j]);
+    }
+  }
+  inter = block_sum(inter, red);
+  if (threadIdx.x == 0) a.ps_q[tile] = inter;
+}
+
 // Copies an instance record between global memory and a workgroup's LDS copy, in 8-byte
 // words over the lanes of the workgroup.
 __device__ __forceinline__ void copy_state(InstState* dst, const InstState* src) {
@@ -690,7 +749,8 @@ __device__ __forceinline__ void decide_instance(const Args<T>& a, const int inst
   v[0] = lane_sum(ps_c + d.tc0, d.tc1 - d.tc0, 1, 0.0);
   v[1] = a.slices ? lane_sum(a.ps_r + d.tr0, d.tr1 - d.tr0, 1, 0.0)
                   : lane_sum(a.ps_a + d.ta0, d.ta1 - d.ta0, 1, 0.0);
-  v[2] = lane_sum(a.ps_t + d.tt0, d.tt1 - d.tt0, 1, 0.0);
+  v[2] = a.row_slices ? lane_sum(a.ps_q + d.tc0, d.tc1 - d.tc0, 1, 0.0)
+                      : lane_sum(a.ps_t + d.tt0, d.tt1 - d.tt0, 1, 0.0);
   copy_state(sm, in + inst);
   __syncthreads();
   const bool go = stepping(*sm);

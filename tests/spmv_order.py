@@ -71,25 +71,34 @@ def step_product(lp: dict, scaled_values: np.ndarray, vec: np.ndarray, which: in
     return fixed_order_sums(ptr, vals[entry] * v[cols])
 
 
-def sliced_step_product(lp: dict, scaled_values: np.ndarray, vec: np.ndarray, slices: int,
-                        dtype) -> np.ndarray:
-    """A-hat vec as the engine forms it over column slices (sk_pdhg_options.column_slices):
-    slice k holds the columns [k n / slices, (k + 1) n / slices) of the LP's order (integer
-    division), each row's entries in a slice are summed in the fixed per-row order (a row of the
-    stacked slices), and a row's slice sums are added in slice order, all in `dtype`."""
+def sliced_step_product(lp: dict, scaled_values: np.ndarray, vec: np.ndarray, which: int,
+                        slices: int, dtype) -> np.ndarray:
+    """What the step kernels compute over slices (sk_pdhg_options.column_slices): which 1,
+    A-hat vec over column slices (slice k: columns [k n / slices, (k + 1) n / slices) of the
+    LP's order, integer division); which 2, A-hat^T vec over row slices (rows [k m / slices,
+    (k + 1) m / slices)). Each row's entries in a slice are summed in the fixed per-row order
+    (a row of the stacked slices), and a row's slice sums are added in slice order, all in
+    `dtype`."""
     vals = np.asarray(scaled_values, np.float64).astype(dtype)
     v = np.asarray(vec, np.float64).astype(dtype)
     m, n = lp["num_rows"], lp["num_cols"]
-    ptr, cols, entry = csr_rows(lp)
-    rows = np.repeat(np.arange(m), np.diff(ptr))
-    terms = vals[entry] * v[cols]
-    bounds = (np.arange(slices + 1, dtype=np.int64) * n) // slices
-    slice_of = np.searchsorted(bounds, cols, side="right") - 1
+    if which == 1:
+        ptr, gather, entry = csr_rows(lp)
+        terms = vals[entry] * v[gather]
+        count, rows = n, m
+    else:
+        ptr = np.asarray(lp["col_starts"], np.int64)
+        gather = np.asarray(lp["row_indices"], np.int64)
+        terms = vals * v[gather]
+        count, rows = m, n
+    owner = np.repeat(np.arange(rows), np.diff(ptr))
+    bounds = (np.arange(slices + 1, dtype=np.int64) * count) // slices
+    slice_of = np.searchsorted(bounds, gather, side="right") - 1
     out = None
     for k in range(slices):
-        sel = slice_of == k  # CSR order: each row's entries of the slice stay one run, in order
-        sptr = np.zeros(m + 1, np.int64)
-        np.cumsum(np.bincount(rows[sel], minlength=m), out=sptr[1:])
+        sel = slice_of == k  # stored order: each row's entries of the slice stay one run, in order
+        sptr = np.zeros(rows + 1, np.int64)
+        np.cumsum(np.bincount(owner[sel], minlength=rows), out=sptr[1:])
         part = fixed_order_sums(sptr, terms[sel])
         out = part if out is None else out + part
     return out

@@ -166,11 +166,13 @@ def test_batch_member_products_are_its_own(gpu, oracle_mod):
 
 
 SLICED_CASES = {
-    # A x-bar over 8 column slices (sk_pdhg_options.column_slices = 1): short-row tiles of the
+    # sk_pdhg_options.column_slices = 1. A x-bar over 8 column slices: short-row tiles of the
     # stacked slices (power law), long-row tiles (rows of ~125 nonzeros per slice) and vector
-    # tiles (the same rows above a capacity of 100)
+    # tiles (the same rows above a capacity of 100). A^T y' over 4 row slices: the power law's
+    # columns of up to 2,000 entries give long-row tiles, and vector tiles at 100
     "power_law": (power_law, {}),
     "power_law_cap512": (power_law, {"tile_nnz": 512}),
+    "power_law_cap100": (power_law, {"tile_nnz": 100}),
     "long_rows": (long_rows, {}),
     "long_rows_cap100": (long_rows, {"tile_nnz": 100}),
     "power_law_fp64": (power_law, {"pdhg_precision": "fp64"}),
@@ -179,9 +181,10 @@ SLICED_CASES = {
 
 @pytest.mark.parametrize("case", list(SLICED_CASES))
 def test_sliced_products_match_the_oracle(gpu, oracle_mod, case):
-    """The sliced A x-bar: bit for bit the restatement of its order (each slice's row sums in
-    the fixed per-row order, then the slices added in order), within the rounding bound of the
-    FP64 oracle product; A^T y' and the device scaling are those of the unsliced engine."""
+    """The sliced products, A x-bar over 8 column slices and A^T y' over 4 row slices: bit for
+    bit the restatement of their order (each slice's row sums in the fixed per-row order, then
+    the slices added in order), within the rounding bound of the FP64 oracle product; the
+    device scaling is that of the unsliced engine."""
     make, opts = SLICED_CASES[case]
     lp = make(gpu)
     d = lp.export()
@@ -199,13 +202,18 @@ def test_sliced_products_match_the_oracle(gpu, oracle_mod, case):
     def mult(vals, vec, transpose):
         return oracle_mod.multiply(dict(d, values=vals), vec, transpose=transpose)
 
-    got = b.probe_spmv(0, 1, x)
-    np.testing.assert_array_equal(got, sliced_step_product(d, scaled, x, 8, dtype).astype(np.float64))
-    exact = mult(scaled, x, False)
-    bound = error_bound(d, scaled, x, 1, unit, mult) * (2 if fp64 else 1)
-    assert np.all(np.abs(got - exact) <= bound), np.max(np.abs(got - exact) / bound)
-    np.testing.assert_array_equal(b.probe_spmv(0, 2, y),
-                                  step_product(d, scaled, y, 2, dtype).astype(np.float64))
+    for which, vec, slices in ((1, x, 8), (2, y, 4)):
+        got = b.probe_spmv(0, which, vec)
+        want = sliced_step_product(d, scaled, vec, which, slices, dtype).astype(np.float64)
+        np.testing.assert_array_equal(got, want, err_msg=f"{case} product {which}")
+        exact = mult(scaled, vec, which == 2)
+        bound = error_bound(d, scaled, vec, which, unit, mult) * (2 if fp64 else 1)
+        assert np.all(np.abs(got - exact) <= bound), (which, np.max(np.abs(got - exact) / bound))
+    if case == "power_law_cap100":
+        cs, ri = np.asarray(d["col_starts"]), np.asarray(d["row_indices"])
+        per = [np.diff(np.searchsorted(ri[cs[j]:cs[j + 1]], [k * m // 4 for k in range(5)]))
+               for j in range(n)]
+        assert max(v.max() for v in per) > 100  # a column's entries in one row slice: a vector tile
     if case.startswith("long_rows"):
         cs, ri = np.asarray(d["col_starts"]), np.asarray(d["row_indices"])
         lens = [np.bincount(ri[cs[k * n // 8]:cs[(k + 1) * n // 8]], minlength=m) for k in range(8)]

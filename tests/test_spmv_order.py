@@ -103,11 +103,14 @@ def test_one_slice_is_the_unsliced_order_and_eight_are_slice_sums():
     rng = np.random.default_rng(17)
     lp = random_csc(rng, 40, 400, lambda: rng.integers(1, 12))
     x = rng.standard_normal(400)
-    np.testing.assert_array_equal(sliced_step_product(lp, lp["values"], x, 1, np.float32),
+    np.testing.assert_array_equal(sliced_step_product(lp, lp["values"], x, 1, 1, np.float32),
                                   step_product(lp, lp["values"], x, 1, np.float32))
+    y = rng.standard_normal(40)
+    np.testing.assert_array_equal(sliced_step_product(lp, lp["values"], y, 2, 1, np.float32),
+                                  step_product(lp, lp["values"], y, 2, np.float32))
     a = dense(lp).astype(np.float32)
     xf = x.astype(np.float32)
-    got = sliced_step_product(lp, lp["values"], x, 8, np.float32)
+    got = sliced_step_product(lp, lp["values"], x, 1, 8, np.float32)
     bounds = [k * 400 // 8 for k in range(9)]
     for i in range(40):
         total = None
@@ -118,3 +121,24 @@ def test_one_slice_is_the_unsliced_order_and_eight_are_slice_sums():
                     part = np.float32(part + np.float32(a[i, j] * xf[j]))
             total = part if total is None else np.float32(total + part)
         assert got[i] == total, i
+
+
+def test_row_slices_are_slice_sums_of_each_column():
+    # A^T y over 4 row slices: per column, the serial sums of its entries in each row range,
+    # added in range order
+    rng = np.random.default_rng(19)
+    lp = random_csc(rng, 400, 30, lambda: rng.integers(1, 60))
+    y = rng.standard_normal(400)
+    got = sliced_step_product(lp, lp["values"], y, 2, 4, np.float32)
+    a = dense(lp).astype(np.float32)
+    yf = y.astype(np.float32)
+    bounds = [k * 400 // 4 for k in range(5)]
+    for j in range(30):
+        total = None
+        for k in range(4):
+            part = np.float32(0)
+            for i in range(bounds[k], bounds[k + 1]):  # at most 59 entries per column
+                if a[i, j] != 0:
+                    part = np.float32(part + np.float32(a[i, j] * yf[i]))
+            total = part if total is None else np.float32(total + part)
+        assert got[j] == total, j
