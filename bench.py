@@ -426,6 +426,8 @@ def main() -> int:
     ap.add_argument("--iteration-limit", type=int, default=-1,
                     help="cap every solve at this many PDHG iterations (counter-profiling runs "
                          "only; capped solves are not the metric)")
+    ap.add_argument("--column-slices", type=int, default=0,
+                    help="products over XCD slices for a batch of one LP: 0 automatic, 1 on, 2 off")
     ap.add_argument("--no-compare", action="store_true",
                     help="skip the FP32/FP64 comparison of the single-LP workloads")
     args = ap.parse_args()
@@ -445,7 +447,7 @@ def main() -> int:
     lps = instances(sk, args.workload, args.instances)
     layout = dict(spmv_stage=args.spmv_stage, tile_nnz=args.tile_nnz, eval_stride=args.eval_stride,
                   steps_per_block=args.steps_per_block, use_graph=0 if args.no_graph else 1,
-                  iteration_limit=args.iteration_limit)
+                  iteration_limit=args.iteration_limit, column_slices=args.column_slices)
     batch = sk.Batch(solve_options(sk, streams=args.streams, **layout))
     for lp in lps:
         batch.add(lp)
