@@ -165,19 +165,6 @@ constexpr int32_t kResidentRounds = 4;
 // still saves ~2 us per step.
 constexpr int64_t kFoldMaxReads = int64_t{1} << 24;
 
-// Column slices (EngineOptions::column_slices): one per XCD, used automatically for a batch of
-// one instance of at least kSliceMinCols columns. Slice s holds the columns
-// [s * n / kColumnSlices, (s + 1) * n / kColumnSlices) of the LP's column order. Measured on the
-// 2M x 5M LP as a batch of its 8 column slices (each slice's tiles on one XCD): A x-bar 228 us
-// against 262 us in the most-referenced-first order and 359 us in the LP's order, with the dual
-// step's per-row work done eight times over.
-constexpr int32_t kColumnSlices = 8;
-constexpr int32_t kSliceMinCols = 1 << 22;
-// The same batches run A^T y' over kRowSlices slices of A's rows (y' entries), two XCDs per
-// slice. Measured as a batch of the 2M x 5M LP's 4 row slices: A^T y' 247 us against 289 us
-// unsliced (LP column order), with the interaction's per-column work done four times over; 8
-// slices (1 MB of y' each, but 40M partial sums): 322 us.
-constexpr int32_t kRowSlices = 4;
 
 // Column order on the device. For an instance of at least kRelabelColumns columns (mode 0) or
 // any instance (mode 1), its kHotBytes / w columns with the most nonzeros (the x-bar entries

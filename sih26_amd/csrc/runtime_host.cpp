@@ -78,9 +78,16 @@ int64_t estimate_device_bytes(int64_t rows, int64_t cols, int64_t nnz, int64_t i
   // each includes the iterate-precision running sum and its FP64 total
   const double per_row = 84.0 + 6.0 * w + 0.5;
   const double per_col = 116.0 + 10.0 * w + 0.5;
-  const double bytes = per_nnz * static_cast<double>(nnz) + per_row * static_cast<double>(rows) +
-                       per_col * static_cast<double>(cols) + 1664.0 * static_cast<double>(instances) +
-                       262144.0;
+  double bytes = per_nnz * static_cast<double>(nnz) + per_row * static_cast<double>(rows) +
+                 per_col * static_cast<double>(cols) + 1664.0 * static_cast<double>(instances) +
+                 262144.0;
+  if (instances == 1 && cols >= kSliceMinCols) {
+    // the stacked column and row slices of an LP that runs sliced by default: per table and
+    // nonzero the gather index, the map, the staging index and slot and the staged value; row
+    // pointers and partial sums per stacked row
+    bytes += 2.0 * (14.0 + w) * static_cast<double>(nnz) +
+             (4.0 + w) * (kColumnSlices * static_cast<double>(rows) + kRowSlices * static_cast<double>(cols));
+  }
   return static_cast<int64_t>(std::ceil(bytes));
 }
 
