@@ -566,6 +566,33 @@ __global__ __launch_bounds__(kBlock) void k_slice_sum(const T* __restrict__ part
   if (i < rows) out[i] = slice_sum(part, i, rows, slices);
 }
 
+// Three lane_sums whose loads are issued together, B per table per round, so three short
+// tables cost one memory round trip instead of three; each sum bit for bit its lane_sum. Every
+// count is at least 1 (every instance has a tile in each table).
+template <int B>
+__device__ __forceinline__ void lane_sums3(const double* __restrict__ p0, int n0,
+                                           const double* __restrict__ p1, int n1,
+                                           const double* __restrict__ p2, int n2, double* v) {
+  const int most = max(n0, max(n1, n2));
+  for (int k0 = static_cast<int>(threadIdx.x); k0 < most; k0 += B * kBlock) {
+    double w0[B], w1[B], w2[B];
+#pragma unroll
+    for (int u = 0; u < B; ++u) {
+      const int k = k0 + u * kBlock;
+      w0[u] = p0[k < n0 ? k : n0 - 1];
+      w1[u] = p1[k < n1 ? k : n1 - 1];
+      w2[u] = p2[k < n2 ? k : n2 - 1];
+    }
+#pragma unroll
+    for (int u = 0; u < B; ++u) {
+      const int k = k0 + u * kBlock;
+      v[0] = k < n0 ? v[0] + w0[u] : v[0];
+      v[1] = k < n1 ? v[1] + w1[u] : v[1];
+      v[2] = k < n2 ? v[2] + w2[u] : v[2];
+    }
+  }
+}
+
 // Reduce `count` per-tile partials starting at tile `base` with the whole workgroup in a
 // fixed order (result valid in every thread).
 __device__ __forceinline__ double tile_sum(const double* part, int base, int count, int stride,
@@ -744,14 +771,18 @@ __device__ __forceinline__ void decide_instance(const Args<T>& a, const int inst
                                                 InstState* sm, double* red) {
   // the tiles' partials and the record are loaded together (the partials of an instance that
   // did not step are read and dropped): ||x'-x||^2, ||y'-y||^2 and the interaction as lane
-  // sums over the tiles, then one reduction
-  double v[3];
-  v[0] = lane_sum(ps_c + d.tc0, d.tc1 - d.tc0, 1, 0.0);
-  v[1] = a.slices ? lane_sum(a.ps_r + d.tr0, d.tr1 - d.tr0, 1, 0.0)
-                  : lane_sum(a.ps_a + d.ta0, d.ta1 - d.ta0, 1, 0.0);
-  v[2] = a.row_slices ? lane_sum(a.ps_q + d.tc0, d.tc1 - d.tc0, 1, 0.0)
-                      : lane_sum(a.ps_t + d.tt0, d.tt1 - d.tt0, 1, 0.0);
-  copy_state(sm, in + inst);
+  // sums over the tiles, then one reduction. The record's words are loaded first and stored to
+  // LDS after the partials' loads are issued, so one memory round trip covers both.
+  constexpr int W = static_cast<int>(sizeof(InstState) / sizeof(uint64_t));
+  static_assert(sizeof(InstState) % sizeof(uint64_t) == 0 && W <= kBlock, "one record word per lane");
+  uint64_t word = 0;
+  if (static_cast<int>(threadIdx.x) < W) word = reinterpret_cast<const uint64_t*>(in + inst)[threadIdx.x];
+  double v[3] = {0.0, 0.0, 0.0};
+  lane_sums3<4>(ps_c + d.tc0, d.tc1 - d.tc0,
+                a.slices ? a.ps_r + d.tr0 : a.ps_a + d.ta0, a.slices ? d.tr1 - d.tr0 : d.ta1 - d.ta0,
+                a.row_slices ? a.ps_q + d.tc0 : a.ps_t + d.tt0,
+                a.row_slices ? d.tc1 - d.tc0 : d.tt1 - d.tt0, v);
+  if (static_cast<int>(threadIdx.x) < W) reinterpret_cast<uint64_t*>(sm)[threadIdx.x] = word;
   __syncthreads();
   const bool go = stepping(*sm);
   if (go) {
