@@ -13,6 +13,7 @@
 #include <thread>
 #include <vector>
 
+#include "threads.hpp"
 #include "engine.hpp"
 #include "kernels.hpp"
 #include "status.hpp"
@@ -231,13 +232,15 @@ void for_each_instance(int32_t count, size_t nnz, std::vector<V>* scratch, F&& f
     }
   };
   std::vector<std::thread> pool;
-  for (int64_t t = 1; t < threads; ++t)
-    pool.emplace_back([&]() {
-      std::vector<V> mine;
-      work(mine);
-    });
-  work(*scratch);
-  for (std::thread& th : pool) th.join();
+  {
+    JoinAll joiner{pool};
+    for (int64_t t = 1; t < threads; ++t)
+      pool.emplace_back([&]() {
+        std::vector<V> mine;
+        work(mine);
+      });
+    work(*scratch);
+  }
   if (failed) std::rethrow_exception(failed);
 }
 
@@ -768,11 +771,11 @@ int EngineImpl<T>::upload() {
         return;
       }
       std::vector<std::thread> pool;
+      JoinAll joiner{pool};
       for (int32_t t = 1; t < row_threads; ++t)
         pool.emplace_back(scatter, static_cast<int32_t>(static_cast<int64_t>(d.rows) * t / row_threads),
                           static_cast<int32_t>(static_cast<int64_t>(d.rows) * (t + 1) / row_threads));
       scatter(0, static_cast<int32_t>(d.rows / row_threads));
-      for (std::thread& th : pool) th.join();
     });
   }
   // Staging order of the SpMV tiles: sorted by gather index unless told otherwise.
@@ -1763,7 +1766,8 @@ int EngineImpl<T>::probe_spmv(int32_t index, int which, const double* in, double
     constexpr int C = decltype(cap)::value;
     constexpr bool P = decltype(pack)::value;
     if (which == 1 && slices_) {
-      // the step kernels' sliced product: each slice's row sums, then their sum per row
+      // the step kernels' sliced product: each slice's row sums, then their sum per row (the
+      // solve's partial buffers serve as scratch: every step rewrites them before reading)
       SK_LAUNCH(k_probe_product<T, C, P><<<static_cast<unsigned>(n_ts_), kStepLanes, 0, s>>>(
           a.ts, a.s_ptr, a.s_sidx, a.s_pos, a.s_val, din, a.xpart));
       SK_LAUNCH(k_slice_sum<T><<<blocks_for(m_), kBlock, 0, s>>>(a.xpart, dout, a.slice_rows, a.slices));

@@ -10,6 +10,8 @@
 #include <unordered_map>
 #include <utility>
 
+#include "threads.hpp"
+
 namespace sk::presolve {
 namespace {
 
@@ -50,9 +52,9 @@ void parallel_ranges(int64_t n, F&& f) {
     return;
   }
   std::vector<std::thread> pool;
+  JoinAll joiner{pool};
   for (int64_t t = 0; t < threads; ++t)
     pool.emplace_back([&, t]() { f(n * t / threads, n * (t + 1) / threads); });
-  for (std::thread& th : pool) th.join();
 }
 
 // The rows of the working LP: the original entries of each row in ascending column order (the
@@ -84,9 +86,9 @@ struct RowStore {
     for (int64_t t = 0; t <= chunks; ++t) c0[static_cast<size_t>(t)] = t == chunks ? n : col_of_chunk(t);
     auto each_chunk = [&](auto&& f) {
       std::vector<std::thread> pool;
+      JoinAll joiner{pool};
       for (int64_t t = 1; t < chunks; ++t) pool.emplace_back([&, t]() { f(t); });
       f(int64_t{0});
-      for (std::thread& th : pool) th.join();
     };
     each_chunk([&](int64_t t) {
       int32_t* c = count.data() + t * m;

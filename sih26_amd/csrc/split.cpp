@@ -217,7 +217,14 @@ class SplitEngine final : public Engine {
       if (st) errors[p] = last_error();
     };
     std::vector<std::thread> workers;
-    for (size_t p = 0; p < n; ++p) workers.emplace_back(run, p);
+    try {
+      for (size_t p = 0; p < n; ++p) workers.emplace_back(run, p);
+    } catch (const std::exception& e) {
+      go.store(true);  // the started workers time their part and end
+      for (auto& w : workers) w.join();
+      set_last_error(std::string("cannot start a sub-batch thread: ") + e.what());
+      return kStatusInitFailed;
+    }
     while (ready.load() < n) std::this_thread::yield();
     const auto t0 = std::chrono::steady_clock::now();
     go.store(true);
