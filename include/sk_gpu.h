@@ -244,9 +244,11 @@ typedef struct {
   int32_t steps_per_block;   /* PDHG steps per captured graph, default 40 */
   int32_t use_graph;         /* 1: hipGraph-captured step blocks (default) */
   int32_t spmv_stage;        /* SpMV staging order: -1 default (1), 0 CSR order, 1 sorted by gather index */
-  int32_t tile_nnz;          /* row-block capacity in nonzeros: 0 the default (2,560), -1 picked
-                                by the autotuner at sk_batch_upload (each of 1,536, 2,048,
-                                2,560, 3,072 and 4,096 is uploaded and the two SpMV step
+  int32_t tile_nnz;          /* row-block capacity in nonzeros: 0 the default (2,560; 1,536
+                                for one LP of at least 2^21 nonzeros whose CSR(A^T) tiles at
+                                2,560 would not fill one round of resident workgroups), -1
+                                picked by the autotuner at sk_batch_upload (each of 1,536,
+                                2,048, 2,560, 3,072 and 4,096 is uploaded and the two SpMV step
                                 kernels timed; the fastest stays), at most 4,096 */
   int32_t compute_quality;   /* 1: fill the recompute_quality measures (default) */
   int32_t eval_stride;       /* an evaluation stage follows every this many steps inside a

@@ -829,11 +829,6 @@ int32_t sk_batch_add(sk_batch* b, const sk_lp* lp, int32_t* index) {
 
 namespace {
 
-// The capacity a batch uploaded with `tile_nnz` runs with (the engine's rule).
-int32_t capacity_for(int32_t tile_nnz) {
-  return tile_nnz > 0 ? std::min(tile_nnz, sk::kMaxTileNnz) : sk::kDefaultTileNnz;
-}
-
 // tile_nnz = -1: the batch is uploaded with each candidate capacity and the two SpMV step
 // kernels are timed with every instance stepping, as the solve schedules them (all sub-batch
 // streams at once): the median of three timings of 10 launches each. The default capacity is
@@ -888,7 +883,7 @@ int32_t sk_batch_upload(sk_batch* b) {
       st = autotune_upload(b);
     } else {
       st = b->engine->upload();
-      b->tile_nnz = capacity_for(b->options.tile_nnz < 0 ? 0 : b->options.tile_nnz);
+      b->tile_nnz = b->engine->tile_capacity();
     }
     if (!st) b->uploaded = true;
     return st;

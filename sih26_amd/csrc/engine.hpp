@@ -118,6 +118,8 @@ class Engine {
   virtual int32_t fused() const = 0;
   // column slices A x-bar runs over (EngineOptions::column_slices), 0 when unsliced
   virtual int32_t column_slices() const = 0;
+  // the row-block tile capacity the uploaded batch runs with (the largest over sub-batches)
+  virtual int32_t tile_capacity() const = 0;
 
   // Probes for tests, valid once a solve (or start()) has run the device set-up. They return
   // what the device holds or computes, in the LP's own row and column order.
@@ -143,6 +145,10 @@ std::unique_ptr<Engine> make_engine(const EngineOptions& options);
 // measured on the 256-LP batch, 2,560 nonzeros per tile gives ~1.9 rounds and step launches
 // ~8% shorter than 2,048 (~2.4 rounds, a long partly-occupied last round).
 constexpr int32_t kDefaultTileNnz = 2560;
+// The capacity for one LP of at least kSingleLpMinNnz nonzeros whose CSR(A^T) tiles at the
+// default capacity are fewer than one round of resident workgroups (EngineImpl::upload).
+constexpr int32_t kSingleLpTileNnz = 1536;
+constexpr int64_t kSingleLpMinNnz = int64_t{1} << 21;
 
 // Column slices (EngineOptions::column_slices): one per XCD, used automatically for a batch of
 // one instance of at least kSliceMinCols columns. Slice s holds the columns

@@ -268,6 +268,11 @@ class SplitEngine final : public Engine {
     for (const auto& e : parts_) least = std::min(least, e->fused());
     return least;
   }
+  int32_t tile_capacity() const override {
+    int32_t most = 0;
+    for (const auto& e : parts_) most = std::max(most, e->tile_capacity());
+    return most;
+  }
   int32_t column_slices() const override {
     // a part slices only when it holds a single instance
     int32_t most = 0;

@@ -52,6 +52,7 @@ class StubEngine final : public Engine {
   int32_t streams() const override { return 1; }
   int32_t fused() const override { return 0; }
   int32_t column_slices() const override { return 0; }
+  int32_t tile_capacity() const override { return 0; }
   int probe_spmv(int32_t, int, const double*, double*) override { return unavailable(); }
   int probe_scaling(int32_t, double*, double*, double*) override { return unavailable(); }
   int column_order(int32_t, int32_t*) const override { return unavailable(); }
