@@ -804,14 +804,19 @@ int EngineImpl<T>::upload() {
     }
   };
   build_tables(opt_.tile_nnz > 0 ? std::min(opt_.tile_nnz, kMaxTileNnz) : kDefaultTileNnz);
-  // One large LP alone whose A^T launch would not fill one round of the workgroups the device
-  // holds at once (8 per CU) is bound by its slowest tile, and smaller tiles shorten that: the
-  // 200k x 500k power-law LP (1,634 tiles of A^T at 2,560) solves in 3.22 s at 1,536 against
-  // 3.25 at 2,048, 3.40 at 2,560 and 3.53 at 3,072 (1,024: 3.28), at the same iterations. Larger
-  // or many-instance batches keep the default, measured on the 256-LP batch.
-  if (opt_.tile_nnz == 0 && K_ == 1 && nnz_ >= kSingleLpMinNnz &&
+  // A batch whose A^T launch would not fill one round of the workgroups the device holds at
+  // once (8 per CU) is bound by its slowest tile, and smaller tiles shorten that. Measured at
+  // 1,024 / 1,280 / 1,536 / 2,048 / 2,560 / 3,072 nonzeros on the 200k x 500k power-law LP
+  // (1,634 tiles of A^T at 2,560): 3.28 / 3.29 / 3.22 / 3.25 / 3.40 / 3.53 s at the same
+  // iterations; on the serving stream (3,000 requests, batches of ~5) 1,536 for every batch gives
+  // latency p50 35.0-35.5 / p99 79-80 ms against 36.9-37.7 / 87 ms at 2,560, and 35.8 / 92-94 ms
+  // when batches that can run as the resident solve at 2,560 kept it. A batch that fills a round
+  // (the 256-LP batch: 3,931 tiles) keeps the default, and so does a sub-batch of a split batch,
+  // whose launches share the round with its siblings' (three sub-batches of the 256-LP batch at
+  // 1,536: 61.0 ms per solve against 56.3 ms at 2,560).
+  if (opt_.tile_nnz == 0 && opt_.split_parts <= 1 &&
       static_cast<int64_t>(tt.size()) < 8 * static_cast<int64_t>(cus_))
-    build_tables(kSingleLpTileNnz);
+    build_tables(kOneRoundTileNnz);
   n_ta_ = static_cast<int32_t>(ta.size());
   n_tt_ = static_cast<int32_t>(tt.size());
   std::vector<Tile>& ts = h.ts;

@@ -40,6 +40,9 @@ struct EngineOptions {
   // L2), for a batch of one instance: 0 automatic (instances of at least 2^22 columns, whose
   // x-bar is several times an L2), 1 on, 2 off. A sliced instance keeps the LP's column order.
   int32_t column_slices = 0;
+  // set by a split batch for its parts: how many sub-batches run side by side (they share the
+  // device's resident workgroups, so the one-round capacity rule of upload does not apply)
+  int32_t split_parts = 1;
 };
 
 // A warm start held by an engine until its next upload (Engine::warm_start).
@@ -145,10 +148,9 @@ std::unique_ptr<Engine> make_engine(const EngineOptions& options);
 // measured on the 256-LP batch, 2,560 nonzeros per tile gives ~1.9 rounds and step launches
 // ~8% shorter than 2,048 (~2.4 rounds, a long partly-occupied last round).
 constexpr int32_t kDefaultTileNnz = 2560;
-// The capacity for one LP of at least kSingleLpMinNnz nonzeros whose CSR(A^T) tiles at the
-// default capacity are fewer than one round of resident workgroups (EngineImpl::upload).
-constexpr int32_t kSingleLpTileNnz = 1536;
-constexpr int64_t kSingleLpMinNnz = int64_t{1} << 21;
+// The capacity for a batch whose CSR(A^T) tiles at the default capacity are fewer than one
+// round of resident workgroups (EngineImpl::upload).
+constexpr int32_t kOneRoundTileNnz = 1536;
 
 // Column slices (EngineOptions::column_slices): one per XCD, used automatically for a batch of
 // one instance of at least kSliceMinCols columns. Slice s holds the columns

@@ -74,6 +74,7 @@ class SplitEngine final : public Engine {
       parts_.clear();
       EngineOptions one = opt_;
       one.streams = 1;
+      one.split_parts = parts;
       for (int32_t p = 0; p < parts; ++p) parts_.push_back(make_single_engine(one));
     } else {
       for (auto& e : parts_) e->clear();

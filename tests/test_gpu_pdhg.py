@@ -322,9 +322,10 @@ def test_resident_solve_equals_separate_launches(gpu, precision):
     launch, with the per-tile code of the separate step kernels): the answers are bit for bit
     those of the separate launches (with the decision folded into the next primal launch,
     step_fusion = 1, or in a launch of its own every step, 3), alone and as a batch of small
-    instances."""
+    instances. The capacity is pinned to 2,560, at which the 500 x 1,000 LPs have four tiles (at
+    the default one-round capacity of 1,536 they have eight and run as separate launches)."""
     kw = dict(pdhg_tolerance=1e-4, pdhg_stop_at_request=True, pdhg_absolute_primal=False,
-              pdhg_precision=precision)
+              pdhg_precision=precision, tile_nnz=2560)
     groups = [[gpu.Lp.generate(1, 1)],
               [gpu.Lp.generate(1, s) for s in (2, 3, 4)],
               [to_lp(gpu, closed_form(name)[0]) for name in ("sum", "range", "maximise")]]
