@@ -247,7 +247,7 @@ typedef struct {
   int32_t tile_nnz;          /* row-block capacity in nonzeros: 0 the default (2,560; 1,536
                                 for a batch, other than a sub-batch of a split one, whose
                                 CSR(A^T) tiles at 2,560 would not fill one round of resident
-                                workgroups), -1
+                                workgroups, 1,024 below a quarter of a round), -1
                                 picked by the autotuner at sk_batch_upload (each of 1,536,
                                 2,048, 2,560, 3,072 and 4,096 is uploaded and the two SpMV step
                                 kernels timed; the fastest stays), at most 4,096 */

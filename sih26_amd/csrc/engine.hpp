@@ -149,8 +149,10 @@ std::unique_ptr<Engine> make_engine(const EngineOptions& options);
 // ~8% shorter than 2,048 (~2.4 rounds, a long partly-occupied last round).
 constexpr int32_t kDefaultTileNnz = 2560;
 // The capacity for a batch whose CSR(A^T) tiles at the default capacity are fewer than one
-// round of resident workgroups (EngineImpl::upload).
+// round of resident workgroups, and for one whose tiles are fewer than a quarter of a round
+// (EngineImpl::upload).
 constexpr int32_t kOneRoundTileNnz = 1536;
+constexpr int32_t kQuarterRoundTileNnz = 1024;
 
 // Column slices (EngineOptions::column_slices): one per XCD, used automatically for a batch of
 // one instance of at least kSliceMinCols columns. Slice s holds the columns
