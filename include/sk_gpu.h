@@ -546,7 +546,9 @@ void sk_presolve_free(sk_presolved* presolve);
 
 /* ---- batches: many LPs of uneven size in one block-diagonal solve --------------------------
  * No counterpart in the reference, which solves one LP per call (src/core/solve.cpp:385-399);
- * every instance of a batch runs the engine of sk_pdhg_solve and gets its answer bit for bit. */
+ * every instance of a batch runs the engine of sk_pdhg_solve, and gets its answer bit for bit
+ * when both run at the same tile capacity (sk_pdhg_options.tile_nnz; the default follows the
+ * size of the batch), else the same to the request's tolerance. */
 typedef struct sk_batch sk_batch;
 
 typedef struct {
@@ -625,7 +627,9 @@ void sk_batch_destroy(sk_batch* batch);
  * limits as one block-diagonal batch, so batches overlap on the device. Each request completes
  * (retires) as soon as its own instance stops, while the rest of its batch keeps running, and
  * once a batch is in its tail its slot admits the next batch beside it. A request's answer is
- * bitwise the same as sk_pdhg_solve's with the same options. */
+ * bitwise sk_pdhg_solve's with the same options when its batch and the solo solve run at the
+ * same tile capacity (a fixed sk_pdhg_options.tile_nnz, or default capacities that agree), else
+ * the same to the request's tolerance. */
 typedef struct sk_server sk_server;
 
 typedef struct {
