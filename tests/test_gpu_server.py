@@ -49,13 +49,46 @@ def test_served_answers_equal_solo_solves(gpu, oracle_mod):
         assert rec["error"] == 0 and res.converged, (k, res.info["message"])
         assert rec["finished"] >= rec["admitted"] >= rec["arrival"] - 1e-9
         solo = gpu.solve(gpu.Lp.generate(3, k), **OPTS)
-        # bitwise: per-instance tiles and a fixed reduction order make the batch irrelevant
+        # bitwise: per-instance tiles and a fixed reduction order make the batch irrelevant at
+        # one tile capacity (these batches and the solo solves are all below a quarter of a
+        # round of resident workgroups, so all run at 1,024 nonzeros per tile)
         assert solo.iterations == res.iterations, k
         np.testing.assert_array_equal(solo.col_value, res.col_value)
         np.testing.assert_array_equal(solo.row_dual, res.row_dual)
         np.testing.assert_array_equal(solo.col_dual, res.col_dual)
         ev = oracle_mod.evaluate(exported[k], res.col_value, -res.row_dual)
         assert ev["worst"] <= 1e-4 * 1.000001, (k, ev["worst"])
+
+
+def test_answers_at_different_default_capacities_agree_to_the_tolerance(gpu, oracle_mod):
+    """The default tile capacity follows a batch's size: 64 C3 members as one launch (streams=1)
+    fill more than a quarter of a round of resident workgroups and run at 1,536, each alone at
+    1,024. The per-tile partials then group differently, so answers need not agree bit for
+    bit; each meets the request (oracle relative KKT <= 1e-4) in both, and the objectives agree
+    to 1e-3 of the objectives' scale."""
+    ks = list(range(64))
+    lps = [gpu.Lp.generate(3, k) for k in ks]
+    b = gpu.Batch(streams=1, **OPTS)
+    for lp in lps:
+        b.add(lp)
+    b.upload()
+    b.solve()
+    assert b.stats()["tile_nnz"] == 1536
+    for k in (0, 21, 42, 63):
+        d = lps[k].export()
+        mixed = b.result(k)
+        one = gpu.Batch(streams=1, **OPTS)
+        one.add(lps[k])
+        one.upload()
+        one.solve()
+        assert one.stats()["tile_nnz"] == 1024
+        alone = one.result(0)
+        for r in (mixed, alone):
+            assert r.converged, (k, r.info["message"])
+            ev = oracle_mod.evaluate(d, r.col_value, -d["sense"] * r.row_dual)
+            assert ev["worst"] <= 1e-4 * 1.000001, (k, ev["worst"])
+        scale = 1.0 + abs(alone.primal_objective) + abs(alone.dual_objective)
+        assert abs(mixed.objective - alone.objective) <= 1e-3 * scale, (k, mixed.objective, alone.objective)
 
 
 def test_instances_retire_mid_batch(gpu):
